@@ -1,0 +1,67 @@
+"""Builds the native libraries in-tree (python -m beamtrace2007_amd.build).
+
+  lib/libbt_host.so   host input stage (C++17, g++): scenes, kd-tree, camera, raster
+  lib/libbtg.so       the GPU path (HIP for gfx950, hipcc): kernels + C ABI
+
+Both are compiled with -ffp-contract=off: the kernels must round every float
+operation exactly as the reference's float build does (CMakeLists.txt:17-21 of
+the reference), so no multiply-add may be fused. Division and square root are
+IEEE (correctly rounded), denormals are kept. Packed FP32 instructions are
+disabled for the device code.
+"""
+from __future__ import annotations
+
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-shared"]
+DEVICE_FLAGS = [
+    "-std=c++17", "-O3", "-fPIC", "-shared", "--offload-arch=gfx950",
+    "-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-parameter",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_host() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libbt_host.so")
+    srcs = sorted(glob.glob(os.path.join(HERE, "host", "*.cpp")))
+    _run(["g++", *HOST_FLAGS, "-o", out + ".tmp", *srcs])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_device() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libbtg.so")
+    srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
+    _run([HIPCC, *DEVICE_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", *srcs])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all() -> None:
+    build_host()
+    build_device()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["host", "device"]
+    if "host" in what:
+        build_host()
+    if "device" in what:
+        build_device()
