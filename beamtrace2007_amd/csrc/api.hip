@@ -1,0 +1,283 @@
+// The C ABI of include/btg.h on top of the wavefront engine (engine.hip).
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+struct btg_ctx {
+  btg::Engine* engine = nullptr;
+  std::string err;
+  btg_scene_desc desc{};
+  float offset = 0;
+};
+
+struct btg_polys {
+  std::vector<btg::OutPoly> v;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int failWith(btg_ctx* c, int code, const std::string& m) {
+  if (c)
+    c->err = m;
+  else
+    g_err = m;
+  return code;
+}
+
+btg::HostCam camOf(const btg_camera& c) {
+  btg::HostCam h;
+  h.eye = btg::f3(c.eye[0], c.eye[1], c.eye[2]);
+  h.fwd = btg::f3(c.fwd[0], c.fwd[1], c.fwd[2]);
+  h.right = btg::f3(c.right[0], c.right[1], c.right[2]);
+  h.down = btg::f3(c.down[0], c.down[1], c.down[2]);
+  h.halfW = c.halfW;
+  h.halfH = c.halfH;
+  h.W = c.W;
+  h.H = c.H;
+  return h;
+}
+
+void fillInfo(const btg::RunInfo& r, double wall, btg_run_info* info) {
+  if (!info) return;
+  std::memset(info, 0, sizeof(*info));
+  info->primarySeconds = r.primaryMs * 1e-3;
+  info->traceSeconds = r.traceMs * 1e-3;
+  info->totalSeconds = wall;
+  info->rounds = r.rounds;
+  info->maxQueue = r.maxQueue;
+  info->killedRoots = r.killed;
+  info->retryPasses = r.retries;
+  auto put = [](const uint64_t* s, btg_trace_stats& t) {
+    t.kdSteps = s[btg::sKd];
+    t.leafVisits = s[btg::sLeaf];
+    t.triTests = s[btg::sTests];
+    t.hits = s[btg::sHits];
+    t.splits = s[btg::sSplits];
+    t.beams = s[btg::sBeams];
+    t.rootBeams = s[btg::sRoots];
+    t.presplitBeams = s[btg::sPresplit];
+    t.fiveSplits = s[btg::sFive];
+    t.climbs = s[btg::sClimbs];
+    t.rays = s[btg::sRays];
+  };
+  put(r.stats, info->stats);
+  info->stats.droppedArea = r.dropped;
+  put(r.rayStats, info->rayStats);
+  for (int k = 0; k < 5; ++k) {
+    info->kernelSeconds[k] = r.kernelMs[k] * 1e-3;
+    info->kernelLaunches[k] = r.launches[k];
+  }
+}
+
+double now() {
+  timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return double(ts.tv_sec) + 1e-9 * double(ts.tv_nsec);
+}
+
+bool validCam(const btg_camera* c) { return c && c->W > 0 && c->H > 0 && c->halfW > 0 && c->halfH > 0; }
+
+}  // namespace
+
+extern "C" {
+
+int btg_device_query(btg_device_info* info) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (info) {
+    hipDeviceProp_t p;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return 0;
+    std::memset(info, 0, sizeof(*info));
+    std::strncpy(info->name, p.name, sizeof(info->name) - 1);
+    std::strncpy(info->arch, p.gcnArchName, sizeof(info->arch) - 1);
+    info->computeUnits = p.multiProcessorCount;
+    info->wavefrontSize = p.warpSize;
+    info->memGB = double(p.totalGlobalMem) / double(1ull << 30);
+  }
+  return 1;
+}
+
+const char* btg_last_error(const btg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int btg_create(btg_ctx** out, const btg_scene_desc* d) {
+  if (!out || !d) return failWith(nullptr, BTG_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (d->numTris <= 0 || d->numNodes <= 0 || !d->pos || !d->tri3 || !d->triN || !d->nodes || !d->parent ||
+      !d->nodeBox || (d->numRefs > 0 && !d->triIndices))
+    return failWith(nullptr, BTG_ERR_ARG, "scene description is incomplete (no triangles or no kd-tree)");
+  if (d->numTris >= (1LL << 31) || d->numNodes >= (1LL << 30) || d->numRefs >= (1LL << 32))
+    return failWith(nullptr, BTG_ERR_ARG, "scene too large for 32-bit indices");
+  // Validate indices on the host: the kernels trust them.
+  for (int64_t t = 0; t < 3 * d->numTris; ++t)
+    if (d->tri3[t] >= uint64_t(d->numVerts)) return failWith(nullptr, BTG_ERR_ARG, "vertex index out of range");
+  for (int64_t i = 0; i < d->numRefs; ++i)
+    if (d->triIndices[i] >= uint64_t(d->numTris)) return failWith(nullptr, BTG_ERR_ARG, "leaf triangle out of range");
+  const btg::KdNodeD* nodes = static_cast<const btg::KdNodeD*>(d->nodes);
+  for (int64_t i = 0; i < d->numNodes; ++i) {
+    const uint32_t data = nodes[i].data;
+    if ((data & 3u) == 3u) {
+      if (uint64_t(nodes[i].offset) + (data >> 2) > uint64_t(d->numRefs))
+        return failWith(nullptr, BTG_ERR_ARG, "kd leaf references out of range");
+    } else if ((data >> 2) <= uint64_t(i) + 1 || (data >> 2) >= uint64_t(d->numNodes)) {
+      return failWith(nullptr, BTG_ERR_ARG, "kd child index out of range");
+    }
+    if (i > 0 && d->parent[i] >= uint64_t(d->numNodes)) return failWith(nullptr, BTG_ERR_ARG, "kd parent out of range");
+  }
+  if (!btg_device_query(nullptr)) return failWith(nullptr, BTG_ERR_DEVICE, "no HIP device available");
+  try {
+    btg_ctx* c = new btg_ctx();
+    std::string err;
+    c->engine = btg::engineCreate(*d, err);
+    if (!c->engine) {
+      delete c;
+      return failWith(nullptr, BTG_ERR_DEVICE, err);
+    }
+    c->desc = *d;
+    // apex offset of the soft-shadow beams: |scene diagonal| * kEpsRel (cuda/gpu_render.cu:574)
+    const btg::F3 diag = btg::f3(d->sceneBounds[3] - d->sceneBounds[0], d->sceneBounds[4] - d->sceneBounds[1],
+                                 d->sceneBounds[5] - d->sceneBounds[2]);
+    c->offset = btg::length(diag) * btg::kEpsRel;
+    *out = c;
+    return BTG_OK;
+  } catch (const std::bad_alloc&) {
+    return failWith(nullptr, BTG_ERR_MEMORY, "host allocation failed");
+  }
+}
+
+void btg_destroy(btg_ctx* ctx) {
+  if (!ctx) return;
+  btg::engineDestroy(ctx->engine);
+  delete ctx;
+}
+
+int btg_get_options(const btg_ctx* ctx, btg_options* o) {
+  if (!ctx || !o) return BTG_ERR_ARG;
+  const btg::Options& x = btg::engineOptions(ctx->engine);
+  o->budget = x.budget;
+  o->primaryBudget = x.primaryBudget;
+  o->capacity = x.capacity;
+  o->roundsPerCheck = x.roundsPerCheck;
+  o->profileKernels = x.profile ? 1 : 0;
+  return BTG_OK;
+}
+
+int btg_set_options(btg_ctx* ctx, const btg_options* o) {
+  if (!ctx || !o) return BTG_ERR_ARG;
+  if (o->budget < 1 || o->primaryBudget < 1 || o->capacity < 1024 || o->capacity > (int64_t(1) << 31) ||
+      o->roundsPerCheck < 1)
+    return failWith(ctx, BTG_ERR_ARG, "invalid options");
+  btg::Options& x = btg::engineOptions(ctx->engine);
+  x.budget = o->budget;
+  x.primaryBudget = o->primaryBudget;
+  x.capacity = o->capacity;
+  x.roundsPerCheck = o->roundsPerCheck;
+  x.profile = o->profileKernels != 0;
+  return BTG_OK;
+}
+
+int btg_soft(btg_ctx* ctx, const btg_camera* cam, const btg_area_light* light, int exact, int32_t* tri, float* vis,
+             float* E, float* Eu, btg_run_info* info) {
+  if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");
+  if (!validCam(cam) || !light) return failWith(ctx, BTG_ERR_ARG, "invalid camera or light");
+  const double t0 = now();
+  btg::RunInfo ri;
+  btg::HostLight L;
+  L.c = btg::f3(light->c[0], light->c[1], light->c[2]);
+  L.U = btg::f3(light->U[0], light->U[1], light->U[2]);
+  L.V = btg::f3(light->V[0], light->V[1], light->V[2]);
+  try {
+    const int rc = btg::engineSoft(ctx->engine, camOf(*cam), L, ctx->offset, exact != 0, tri, vis, E, Eu, ri, ctx->err);
+    if (rc) return rc;
+  } catch (const std::bad_alloc&) {
+    return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
+  }
+  fillInfo(ri, now() - t0, info);
+  return BTG_OK;
+}
+
+int64_t btg_polys_count(const btg_polys* p) { return p ? int64_t(p->v.size()) : 0; }
+const btg_poly* btg_polys_data(const btg_polys* p) {
+  static_assert(sizeof(btg_poly) == sizeof(btg::OutPoly), "polygon layout");
+  return p && !p->v.empty() ? reinterpret_cast<const btg_poly*>(p->v.data()) : nullptr;
+}
+void btg_polys_free(btg_polys* p) { delete p; }
+
+}  // extern "C"
+
+namespace {
+
+bool polyLess(const btg::OutPoly& a, const btg::OutPoly& b) {
+  if (a.tri != b.tri) return a.tri < b.tri;
+  if (a.n != b.n) return a.n < b.n;
+  for (int k = 0; k < a.n; ++k) {
+    if (a.x[k] != b.x[k]) return a.x[k] < b.x[k];
+    if (a.y[k] != b.y[k]) return a.y[k] < b.y[k];
+  }
+  return a.area < b.area;
+}
+
+}  // namespace
+
+extern "C" {
+
+int btg_primary(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, int canonical, btg_polys** out,
+                btg_run_info* info) {
+  if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");
+  if (!validCam(cam) || !out || tile <= 0) return failWith(ctx, BTG_ERR_ARG, "invalid camera, tile or output");
+  *out = nullptr;
+  const double t0 = now();
+  try {
+    btg_polys* P = new btg_polys();
+    btg::RunInfo ri;
+    const int rc = btg::enginePrimary(ctx->engine, camOf(*cam), cull != 0, tile, P->v, ri, ctx->err);
+    if (rc) {
+      delete P;
+      return rc;
+    }
+    if (canonical) std::sort(P->v.begin(), P->v.end(), polyLess);
+    *out = P;
+    fillInfo(ri, now() - t0, info);
+    if (info) info->stats.hits = uint64_t(P->v.size());
+    return BTG_OK;
+  } catch (const std::bad_alloc&) {
+    return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
+  }
+}
+
+int btg_point(btg_ctx* ctx, const btg_camera* cam, const btg_poly* prim, int64_t numPrim, const float* light3,
+              int canonical, btg_polys** out, btg_run_info* info) {
+  if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");
+  if (!validCam(cam) || !out || !light3 || numPrim < 0 || (numPrim > 0 && !prim))
+    return failWith(ctx, BTG_ERR_ARG, "invalid camera, polygons, light or output");
+  *out = nullptr;
+  const double t0 = now();
+  try {
+    btg_polys* P = new btg_polys();
+    btg::RunInfo ri;
+    const int rc = btg::enginePoint(ctx->engine, camOf(*cam), reinterpret_cast<const btg::OutPoly*>(prim), numPrim,
+                                    btg::f3(light3[0], light3[1], light3[2]), P->v, ri, ctx->err);
+    if (rc) {
+      delete P;
+      return rc;
+    }
+    if (canonical) std::sort(P->v.begin(), P->v.end(), polyLess);
+    *out = P;
+    fillInfo(ri, now() - t0, info);
+    return BTG_OK;
+  } catch (const std::bad_alloc&) {
+    return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
+  }
+}
+
+}  // extern "C"

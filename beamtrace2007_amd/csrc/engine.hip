@@ -1,0 +1,2079 @@
+// Wavefront beam tracing on gfx950 (MI355X): sub-beam records in HBM queues,
+// processed round by round by persistent wave64 kernels.
+//
+// One round = three launches on the context's stream:
+//   admitKernel   one lane per root (a pixel in 8x8 tiles for soft shadows,
+//                 a tile for primary visibility, a primary hit polygon for
+//                 point shadows): per-root setup, direction-sign pre-split,
+//                 records appended to the trace queue;
+//   traceKernel   one lane per sub-beam, refilled from the queue as soon as its
+//                 sub-beam finishes or needs a clip (64-bit ballot + mbcnt
+//                 rank, one atomic per wave): kd descent, leaf triangle tests,
+//                 restart-trail climb;
+//   splitKernel   one lane per clip: the clip runs twice, first counting the
+//                 pieces (so the wave reserves exactly the queue slots it
+//                 needs with one atomic), then writing them in place - no
+//                 per-lane piece buffer.
+// The last workgroup of splitKernel to finish schedules the next round
+// (admission by reserved budget, as the reference's scheduleKernel,
+// cuda/wavefront.cu:327-345), so no single-thread launch is needed.
+//
+// Results are exact and order-independent: soft-shadow visibility is summed
+// as integer multiples of the smallest representable piece area (every
+// output area is >= minArea, so each is an exact multiple of ulp(minArea)),
+// and the Lambert integral in 2^-60 fixed point. Roots that exceed their
+// sub-beam budget are traced again on the GPU with a larger budget (no CPU
+// fallback).
+//
+// The per-sub-beam arithmetic restates src/beam/beam_core.h (BeamCtx) and
+// src/beam/wavefront_core.h of the reference; the per-root setups restate
+// src/render/soft_core.h and point_core.h; the primary rays restate
+// src/ray/ray_core.h. Function comments cite the lines.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "engine.h"
+
+namespace btg {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+enum App { kSoft = 0, kPrimary = 1, kPoint = 2 };
+enum RootState { kTraced = 0, kKilled = 1, kNoBeam = 2 };
+enum { kNear = 0, kFar = 1, kBoth = 2 };
+enum { kKeep = 0, kAllIn = 1, kClip = 2 };
+enum { kFinished = 0, kSplit = 1, kContinue = 2 };
+enum { kDescend = 0, kTest = 1, kClimb = 2 };
+enum { kHeadAdmit = 0, kHeadTrace = 1, kHeadSplit = 2 };
+enum { fAnyHit = 1, fFarIsPlane = 2, fCull = 4 };
+
+// A sub-beam record (64 bytes): corners on its root's plane, restart-trail
+// continuation (kd node, depth, pending far children), leaf cursor, hit.
+// The same information as the reference's WfBeam (wavefront_core.h:45-55).
+struct alignas(16) Rec {
+  float qx[4], qy[4];
+  uint64_t trail;
+  uint32_t node;
+  int32_t root;
+  int32_t hit;
+  int32_t cursor;  // next triangle in the leaf; -1: descend from node
+  int16_t depth;
+  int8_t n;        // corners; 0 = a dropped sliver (only decrements the root's live count)
+  int8_t negBits;  // bit a: rays move toward -a
+};
+static_assert(sizeof(Rec) == 64, "record layout");
+
+// Per-root query (BeamQuery + BeamCtx::setup, beam_core.h:123-132, plus the
+// application's shading data).
+struct RootQ {
+  Plane plane;
+  float ext, eps, minArea, farCq;
+  int flags;
+  int exclude;
+  int unitExp;  // soft: exponent of the fixed-point unit of piece areas
+  int tri;      // point: receiver triangle
+  float lightArea;
+  F3 x, ns;     // soft: shading point and normal
+};
+
+struct WState {
+  int next;
+  int admitBegin, admitCount;
+  int count[2];
+  int splitCount;
+  int head[3];
+  int done;
+  int killed;
+  int maxQueue;
+  int overflow;
+  int outCount;
+  int blocksDone;
+  int round;
+  int pad;
+  unsigned long long reserved;
+  unsigned long long traceItems, splitItems;
+};
+
+struct Params {
+  DevScene sv;
+  int app;
+  int R;
+  const int* rootList;
+  int budget;
+  unsigned long long capacity;
+  Rec* tq[2];
+  Rec* sq;
+  RootQ* roots;
+  int* live;
+  int* state;
+  int* epoch;
+  int pass;
+  unsigned long long* missFx;
+  unsigned long long* gFx;
+  unsigned long long* stats;
+  double* dropped;
+  WState* st;
+  int W, H;
+  Plane camPlane;
+  F3 eye, fwd, right, down;
+  float halfW, halfH;
+  int tilesX;
+  F3 lc, lU, lV;
+  float offset;
+  int exact;
+  const int* pixTri;
+  const float* pixX;
+  const float* pixN;
+  float* vis;
+  float* E;
+  float* Eu;
+  int tile, tilesW, cull;
+  F3 pointLight;
+  const OutPoly* prim;
+  OutPoly* out;
+  int* outRoot;
+  int* outTag;
+  int outCap;
+};
+
+// ------------------------------------------------------------ wave helpers
+__device__ __forceinline__ int lane() { return int(__lane_id()); }
+__device__ __forceinline__ int rankBelow(uint64_t mask) {
+  return int(__builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u)));
+}
+// Exclusive prefix of n over the wave, one atomic on `counter` for the
+// wave's total; returns this lane's first slot. All 64 lanes must call it.
+__device__ __forceinline__ int waveAppend(int* counter, int n) {
+  int x = n;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int y = __shfl_up(x, o, kWave);
+    if (lane() >= o) x += y;
+  }
+  const int total = __shfl(x, kWave - 1, kWave);
+  int base = 0;
+  if (lane() == kWave - 1 && total > 0) base = atomicAdd(counter, total);
+  base = __shfl(base, kWave - 1, kWave);
+  return base + x - n;
+}
+
+struct Cnt {
+  uint32_t v[sNum];
+  double dropped;
+  __device__ void zero() {
+#pragma unroll
+    for (int i = 0; i < sNum; ++i) v[i] = 0;
+    dropped = 0;
+  }
+};
+
+__device__ void flushCnt(const Params& p, const Cnt& c) {
+#pragma unroll
+  for (int f = 0; f < sNum; ++f) {
+    unsigned long long s = c.v[f];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_down(s, o, kWave);
+    if (lane() == 0 && s) atomicAdd(&p.stats[f], s);
+  }
+  double d = c.dropped;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) d += __shfl_down(d, o, kWave);
+  if (lane() == 0 && d != 0) atomicAdd(p.dropped, d);
+}
+
+// ------------------------------------------------------------ beam context
+struct Ctx {
+  Plane P;
+  float ext, eps, minArea, farCq;
+  int flags, exclude;
+  __device__ bool anyHit() const { return flags & fAnyHit; }
+  __device__ bool farIsPlane() const { return flags & fFarIsPlane; }
+};
+
+__device__ __forceinline__ Ctx ctxOf(const RootQ& q) {
+  Ctx c;
+  c.P = q.plane;
+  c.ext = q.ext;
+  c.eps = q.eps;
+  c.minArea = q.minArea;
+  c.farCq = q.farCq;
+  c.flags = q.flags;
+  c.exclude = q.exclude;
+  return c;
+}
+
+// BeamCtx::setup (beam_core.h:123-132).
+__device__ __forceinline__ void setupQuery(RootQ& q, float extent) {
+  q.ext = extent > 0 ? extent : 1.0f;
+  q.eps = kEpsRel * q.ext;
+  q.minArea = q.eps * q.eps;
+  q.farCq = dot(q.plane.n, q.plane.d);
+}
+
+// CoreBeam (beam_core.h:30-40).
+struct Beam {
+  C4 qx, qy, aw0, aw1, aw2, farA;
+  float farC;
+  int n, hit;
+  int s0, s1, s2;  // +-1 per axis
+  bool hasFar;
+};
+
+__device__ __forceinline__ int sgnOf(const Beam& b, int a) { return a == 0 ? b.s0 : (a == 1 ? b.s1 : b.s2); }
+__device__ __forceinline__ C4 awOf(const Beam& b, int a) { return a == 0 ? b.aw0 : (a == 1 ? b.aw1 : b.aw2); }
+
+// Plane of the current hit cached with the beam (WfWork, wavefront_core.h:58-61).
+struct Work {
+  Beam b;
+  float hitA[3], hitC;
+  F3 hitN;
+};
+
+// decide (beam_core.h:139-164): near / far / both children of an inner node.
+__device__ int decide(const Ctx& c, const Beam& b, int a, float Ns, const Box& box) {
+  const C4 awa = awOf(b, a);
+#pragma unroll
+  for (int k = 1; k <= 2; ++k) {
+    const int cc = a + k >= 3 ? a + k - 3 : a + k;
+    const int sg = sgnOf(b, cc);
+    const float s = float(sg);
+    const float entry = sg > 0 ? comp(box.lo, cc) : comp(box.hi, cc);
+    const float exit = sg > 0 ? comp(box.hi, cc) : comp(box.lo, cc);
+    const float oc = comp(c.P.O, cc);
+    const float nmin = s * (entry - oc), nmax = s * (exit - oc);
+    const C4 lhs = scaled(awOf(b, cc), Ns);
+    if (allBelow(lhs, scaled(awa, nmin), kKdRel)) return kFar;
+    if (allAbove(lhs, scaled(awa, nmax), kKdRel)) return kNear;
+  }
+  if (b.hasFar && allAbove(scaled(b.farA, Ns), scaled(awa, b.farC), kKdRel)) return kNear;
+  return kBoth;
+}
+
+// overlaps (beam_core.h:166-195): false if the beam certainly misses the box.
+__device__ bool overlaps(const Ctx& c, const Beam& b, const Box& box) {
+  float nmin[3], nmax[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int sg = sgnOf(b, k);
+    const float s = float(sg), o = comp(c.P.O, k);
+    nmin[k] = s * ((sg > 0 ? comp(box.lo, k) : comp(box.hi, k)) - o);
+    nmax[k] = s * ((sg > 0 ? comp(box.hi, k) : comp(box.lo, k)) - o);
+    if (nmax[k] < 0) return false;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      if (k == e) continue;
+      if (allBelow(scaled(awOf(b, e), nmax[k]), scaled(awOf(b, k), nmin[e]), kKdRel)) return false;
+    }
+  if (b.hasFar) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (allAbove(scaled(b.farA, nmin[k]), scaled(awOf(b, k), b.farC), kKdRel)) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void setAw(Beam& b, const Ctx& c) {
+  const Plane& P = c.P;
+  C4 w[3] = {affine(b.qx, P.u.x, b.qy, P.v.x, P.d.x), affine(b.qx, P.u.y, b.qy, P.v.y, P.d.y),
+             affine(b.qx, P.u.z, b.qy, P.v.z, P.d.z)};
+  const int sg[3] = {b.s0, b.s1, b.s2};
+  C4 aw[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float ws = w[a].v[i] * float(sg[a]);
+      aw[a].v[i] = 0.0f > ws ? 0.0f : ws;
+    }
+  b.aw0 = aw[0];
+  b.aw1 = aw[1];
+  b.aw2 = aw[2];
+}
+
+// beamGeom (beam_core.h:200-221): corners (lanes past n repeat the last) and
+// |w| per axis; slivers below minArea are dropped.
+template <int N>
+__device__ bool beamGeom(const Ctx& c, const Pg<N>& p, int s0, int s1, int s2, Beam& b, float& dropped) {
+  Pg<N> q = p;
+  cleanPoly(q, c.eps);
+  const float A = q.n >= 3 ? q.area() : 0.0f;
+  if (q.n < 3 || A < c.minArea) {
+    dropped = selAbs(p.area());
+    return false;
+  }
+  const int l3 = q.n > 3 ? 3 : 2;
+  b.qx = c4(q.x[0], q.x[1], q.x[2], q.x[l3]);
+  b.qy = c4(q.y[0], q.y[1], q.y[2], q.y[l3]);
+  b.n = q.n;
+  b.s0 = s0;
+  b.s1 = s1;
+  b.s2 = s2;
+  setAw(b, c);
+  return true;
+}
+
+// setStatus (beam_core.h:225-240).
+__device__ __forceinline__ void setStatus(const Ctx& c, Beam& b, int hit, const float* A, float C) {
+  b.hit = hit;
+  if (hit >= 0) {
+    const C4 f = affine(b.qx, A[0], b.qy, A[1], A[2]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b.farA.v[i] = __builtin_fabsf(f.v[i]);
+    b.farC = selAbs(C);
+    b.hasFar = true;
+  } else if (c.farIsPlane()) {
+    const float c0 = selAbs(c.farCq);
+    b.farA = splat(c0);
+    b.farC = c0;
+    b.hasFar = true;
+  } else {
+    b.farC = 0;
+    b.hasFar = false;
+  }
+}
+
+// hitPlane (beam_core.h:290-297) from the packed per-triangle normal/vertex.
+__device__ __forceinline__ void hitPlane(const Params& p, const Ctx& c, int t, Work& w) {
+  const float4 nn = p.sv.triNV[2 * size_t(t)], vv = p.sv.triNV[2 * size_t(t) + 1];
+  const F3 nT = f3(nn.x, nn.y, nn.z);
+  w.hitN = nT;
+  w.hitA[0] = dot(nT, c.P.u);
+  w.hitA[1] = dot(nT, c.P.v);
+  w.hitA[2] = dot(nT, c.P.d);
+  w.hitC = dot(nT, f3(vv.x, vv.y, vv.z) - c.P.O);
+}
+
+// wfExpand (wavefront_core.h:92-112).
+__device__ void expand(const Params& p, const Ctx& c, const Rec& r, Work& w) {
+  Beam& b = w.b;
+  const int n = r.n, l3 = n > 3 ? 3 : 2;
+  b.qx = c4(r.qx[0], r.qx[1], r.qx[2], r.qx[l3]);
+  b.qy = c4(r.qy[0], r.qy[1], r.qy[2], r.qy[l3]);
+  b.n = n;
+  b.s0 = (r.negBits & 1) ? -1 : 1;
+  b.s1 = (r.negBits & 2) ? -1 : 1;
+  b.s2 = (r.negBits & 4) ? -1 : 1;
+  setAw(b, c);
+  if (r.hit >= 0) {
+    hitPlane(p, c, r.hit, w);
+    setStatus(c, b, r.hit, w.hitA, w.hitC);
+  } else {
+    w.hitA[0] = w.hitA[1] = w.hitA[2] = w.hitC = 0;
+    w.hitN = f3(0, 0, 0);
+    setStatus(c, b, -1, nullptr, 0);
+  }
+}
+
+// wfStoreGeom (wavefront_core.h:114-121).
+__device__ __forceinline__ void storeGeom(const Beam& b, Rec& r) {
+  r.n = int8_t(b.n);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r.qx[i] = b.qx.v[i];
+    r.qy[i] = b.qy.v[i];
+  }
+  r.negBits = int8_t((b.s0 < 0 ? 1 : 0) | (b.s1 < 0 ? 2 : 0) | (b.s2 < 0 ? 4 : 0));
+}
+
+// Triangle setup (computeTriInfo + triSetup, beam_core.h:251-286): edge
+// planes through the apex in canonical vertex order, plane coefficients.
+struct TriS {
+  int tri;
+  bool ok;
+  Ln edge[3];
+  float A[3], C;
+  F3 r[3], n;
+};
+
+__device__ TriS triSetup(const Ctx& c, const LeafTri& ref) {
+  TriS ti;
+  const F3& O = c.P.O;
+  ti.r[0] = f3(ref.v[0], ref.v[1], ref.v[2]) - O;
+  ti.r[1] = f3(ref.v[3], ref.v[4], ref.v[5]) - O;
+  ti.r[2] = f3(ref.v[6], ref.v[7], ref.v[8]) - O;
+  ti.n = f3(ref.n[0], ref.n[1], ref.n[2]);
+  ti.tri = ref.tri;
+  ti.ok = false;
+  const F3 m0 = ref.vi[0] < ref.vi[1] ? cross(ti.r[0], ti.r[1]) : -cross(ti.r[1], ti.r[0]);
+  const F3 m1 = ref.vi[1] < ref.vi[2] ? cross(ti.r[1], ti.r[2]) : -cross(ti.r[2], ti.r[1]);
+  const F3 m2 = ref.vi[2] < ref.vi[0] ? cross(ti.r[2], ti.r[0]) : -cross(ti.r[0], ti.r[2]);
+  const float D = dot(ti.r[2], m0);
+  const float scale = length(m0) * length(ti.r[2]);
+  if (!(selAbs(D) > kDegRel * scale)) return ti;
+  if ((c.flags & fCull) && D > 0) return ti;
+  const float s = D > 0 ? 1.0f : -1.0f;
+  const Plane& P = c.P;
+  ti.edge[0] = makeLine(s * dot(m0, P.u), s * dot(m0, P.v), s * dot(m0, P.d), c.ext);
+  ti.edge[1] = makeLine(s * dot(m1, P.u), s * dot(m1, P.v), s * dot(m1, P.d), c.ext);
+  ti.edge[2] = makeLine(s * dot(m2, P.u), s * dot(m2, P.v), s * dot(m2, P.d), c.ext);
+  ti.A[0] = dot(ti.n, P.u);
+  ti.A[1] = dot(ti.n, P.v);
+  ti.A[2] = dot(ti.n, P.d);
+  ti.C = dot(ti.n, ti.r[0]);
+  ti.ok = true;
+  return ti;
+}
+
+// closerLine (beam_core.h:302-317).
+__device__ Ln closerLine(const Ctx& c, const TriS& T, const float* AF, float CF, F3 nF, int fId) {
+  const float cosang = dot(T.n, nF);
+  const float scaleC = selMax(selAbs(T.C), selAbs(CF));
+  const float ca = selAbs(cosang);
+  const float diff = T.C - cosang * CF;
+  if (ca > 1 - kCopCos && selAbs(diff) <= kCopRel * scaleC) return Ln{0, 0, (fId >= 0 && T.tri < fId) ? 1.0f : -1.0f};
+  const float sg = T.C * CF > 0 ? 1.0f : -1.0f;
+  return makeLine(sg * (CF * T.A[0] - T.C * AF[0]), sg * (CF * T.A[1] - T.C * AF[1]), sg * (CF * T.A[2] - T.C * AF[2]),
+                  c.ext);
+}
+
+// separated (beam_core.h:321-342): some beam side plane has all three
+// vertices strictly outside. Lane k holds side k -> k+1 (the reference's rot1).
+__device__ bool separated(const Ctx& c, const Beam& b, const TriS& ti) {
+  const Plane& P = c.P;
+  const C4 wx = affine(b.qx, P.u.x, b.qy, P.v.x, P.d.x);
+  const C4 wy = affine(b.qx, P.u.y, b.qy, P.v.y, P.d.y);
+  const C4 wz = affine(b.qx, P.u.z, b.qy, P.v.z, P.d.z);
+  float nx[4], ny[4], nz[4], nl[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int j = (i + 1) & 3;
+    nx[i] = wy.v[i] * wz.v[j] - wz.v[i] * wy.v[j];
+    ny[i] = wz.v[i] * wx.v[j] - wx.v[i] * wz.v[j];
+    nz[i] = wx.v[i] * wy.v[j] - wy.v[i] * wx.v[j];
+    nl[i] = __builtin_sqrtf(nx[i] * nx[i] + ny[i] * ny[i] + nz[i] * nz[i]);
+  }
+  int outAll = b.n == 4 ? 0xF : 0xB;  // a triangle's lane 2 side is degenerate
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const F3 r = ti.r[k];
+    const float lr = length(r) * kEpsRel;
+    int m = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float s = nx[i] * r.x + ny[i] * r.y + nz[i] * r.z;
+      const float tol = nl[i] * lr;
+      if (s < -tol) m |= 1 << i;
+    }
+    outAll &= m;
+  }
+  return outAll != 0;
+}
+
+// testLine (beam_core.h:356-362): -1 no corner definitely inside; otherwise
+// bit 0 set if some corner is definitely outside.
+__device__ __forceinline__ int lineTest(const Ctx& c, const Beam& B, const Ln& L) {
+  const C4 dv = affine(B.qx, L.a, B.qy, L.b, L.c);
+  bool in = false, out = false;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    in = in || dv.v[i] > c.eps;
+    out = out || dv.v[i] < -c.eps;
+  }
+  return in ? (out ? 1 : 0) : -1;
+}
+
+// classifyTri (beam_core.h:374-409). Lines that the clip needs are returned
+// in L[0..nl) when L is given.
+__device__ int classifyTri(const Ctx& c, const Work& w, const TriS& ti, Ln* L, int& nl) {
+  const Beam& B = w.b;
+  nl = 0;
+  bool allIn = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int r = lineTest(c, B, ti.edge[k]);
+    if (r < 0) return kKeep;
+    allIn = allIn && r == 0;
+    if (L) L[nl] = ti.edge[k];
+    ++nl;
+  }
+  if (!c.anyHit() && B.hit >= 0) {
+    const Ln cl = closerLine(c, ti, w.hitA, w.hitC, w.hitN, B.hit);
+    const int r = lineTest(c, B, cl);
+    if (r < 0) return kKeep;
+    allIn = allIn && r == 0;
+    if (L) L[nl] = cl;
+    ++nl;
+  }
+  if (c.farIsPlane()) {
+    const float farA[3] = {0, 0, c.farCq};
+    const Ln cl = closerLine(c, ti, farA, c.farCq, c.P.n, -1);
+    const int r = lineTest(c, B, cl);
+    if (r < 0) return kKeep;
+    allIn = allIn && r == 0;
+    if (L) L[nl] = cl;
+    ++nl;
+  }
+  if (allIn) return kAllIn;
+  if (separated(c, B, ti)) return kKeep;
+  return kClip;
+}
+
+// clipTri (beam_core.h:419-479), pieces delivered to sink.push(list, poly):
+// list 0 keeps the beam's status, list 1 takes the triangle. Edges are
+// ordered by their positive corner sums first.
+template <class Sink>
+__device__ void clipTri(const Ctx& c, const Beam& B, Ln* L, int nl, Sink& sink, uint32_t& fiveSplits) {
+  float score[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const C4 dv = affine(B.qx, L[k].a, B.qy, L[k].b, L[k].c);
+    float ps[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ps[i] = dv.v[i] > 0.0f ? dv.v[i] : 0.0f;
+    score[k] = ps[0] + ps[1] + ps[2] + (B.n == 4 ? ps[3] : 0.0f);
+  }
+  for (int i = 1; i < 3; ++i)
+    for (int j = i; j > 0 && score[j] < score[j - 1]; --j) {
+      const float t = score[j];
+      score[j] = score[j - 1];
+      score[j - 1] = t;
+      const Ln l = L[j];
+      L[j] = L[j - 1];
+      L[j - 1] = l;
+    }
+  Pg<6> stack[8];
+  int stackLine[8];
+  int sp = 1;
+  stack[0].n = B.n;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    stack[0].x[i] = B.qx.v[i];
+    stack[0].y[i] = B.qy.v[i];
+  }
+  stackLine[0] = 0;
+  while (sp > 0) {
+    --sp;
+    Pg<6> p = stack[sp];
+    int li = stackLine[sp];
+    bool inside = true;
+    for (; li < nl; ++li) {
+      const int cs = sidesOf(p, L[li], c.eps);
+      if (!(cs & 2)) continue;
+      if (!(cs & 1)) {
+        sink.push(0, p);
+        inside = false;
+        break;
+      }
+      Pg<6> in, out;
+      splitPoly(p, L[li], c.eps, in, out);  // pieces of a <= 4-gon have <= 5 corners: never fails
+      cleanPoly(out, c.eps);
+      if (out.n >= 3) sink.push(0, out);
+      cleanPoly(in, c.eps);
+      if (in.n < 3) {
+        inside = false;
+        break;
+      }
+      if (in.n == 5) {
+        Pg<6> a, b;
+        splitFive(in, a, b);
+        ++fiveSplits;
+        stack[sp] = b;  // depth <= 1 + 5 lines < 8
+        stackLine[sp] = li + 1;
+        ++sp;
+        p = a;
+      } else {
+        p = in;
+      }
+    }
+    if (inside) sink.push(1, p);
+  }
+}
+
+// Counting pass: records each list produces (pentagons become two) and the
+// total area of the hit pieces (summed in piece order, as the reference).
+struct CountSink {
+  int recs[2] = {0, 0};
+  float hitArea = 0;
+  __device__ void push(int l, const Pg<6>& q) {
+    recs[l] += q.n == 5 ? 2 : 1;
+    if (l == 1) hitArea += q.area();
+  }
+};
+
+// -------------------------------------------------------------- outputs
+__device__ __forceinline__ double pow2(int e) { return __builtin_ldexp(1.0, e); }
+
+// polygonFormFactorG (soft_core.h:18-33).
+__device__ float formFactor(F3 x, F3 n, const F3* poly, int count) {
+  double sum = 0;
+  for (int i = 0; i < count; ++i) {
+    const F3 a = poly[i] - x, b = poly[i + 1 == count ? 0 : i + 1] - x;
+    const double la = double(length(a)), lb = double(length(b));
+    if (la == 0 || lb == 0) continue;
+    double cc = double(dot(a, b)) / (la * lb);
+    cc = cc < -1 ? -1 : (cc > 1 ? 1 : cc);
+    const double theta = acos(cc);
+    const F3 g = cross(a, b);
+    const double lg = double(length(g));
+    if (lg == 0) continue;
+    sum += theta * double(dot(g, n)) / lg;
+  }
+  return float((sum < 0 ? -sum : sum) * 0.5);
+}
+
+// Appends an output polygon (primary / point shadows).
+__device__ void appendPoly(const Params& p, int root, const OutPoly& ob) {
+  const int k = atomicAdd(&p.st->outCount, 1);
+  if (k >= p.outCap) return;  // the host traces again with a larger buffer
+  p.out[k] = ob;
+  p.outRoot[k] = root;
+  p.outTag[k] = p.pass;
+}
+
+// pointShadowToImage (point_core.h:41-64).
+__device__ bool pointToImage(const Plane& camPlane, const Plane& sp, const float* sx, const float* sy, int n, int T,
+                             OutPoly& ip) {
+  ip.n = n;
+  ip.tri = T;
+  ip.area = 0;
+  for (int i = 0; i < n; ++i)
+    if (!camPlane.project(sp.point(sx[i], sy[i]), ip.x[i], ip.y[i])) return false;
+  Pg<4> q;
+  q.n = 0;
+  for (int i = 0; i < n; ++i) q.push(ip.x[i], ip.y[i]);
+  const float a = q.area();
+  if (!(a > 0 || a < 0)) return false;
+  if (a < 0) {
+    q.reverse();
+    for (int i = 0; i < n; ++i) {
+      ip.x[i] = q.x[i];
+      ip.y[i] = q.y[i];
+    }
+  }
+  for (int i = n; i < 4; ++i) {
+    ip.x[i] = ip.x[n - 1];
+    ip.y[i] = ip.y[n - 1];
+  }
+  return true;
+}
+
+// A finished piece (wfOutput, wavefront_core.h:124-132, and the per-app
+// handling of cuda/wavefront.cu:193-227).
+__device__ void emitOutput(const Params& p, const Ctx& c, const Beam& b, int root, int tri, Cnt& cnt) {
+  Pg<4> q;
+  q.n = b.n;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    q.x[i] = b.qx.v[i];
+    q.y[i] = b.qy.v[i];
+  }
+  const float area = q.area();
+  if (tri >= 0) cnt.v[sHits]++;
+  cnt.v[sBeams]++;
+  if (p.app == kSoft) {
+    if (tri >= 0) return;  // occluded
+    const RootQ& R = p.roots[root];
+    if (area > 0) atomicAdd(&p.missFx[root], (unsigned long long)(double(area) * pow2(-R.unitExp)));
+    if (p.exact) {
+      F3 P[4];
+      for (int i = 0; i < q.n; ++i) P[i] = c.P.point(q.x[i], q.y[i]);
+      const float G = formFactor(R.x, R.ns, P, q.n);
+      atomicAdd(&p.gFx[root], (unsigned long long)rint(double(G) * pow2(60)));
+    }
+    return;
+  }
+  if (tri < 0) return;
+  OutPoly ob;
+  if (p.app == kPrimary) {
+    ob.n = q.n;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = i < q.n ? i : q.n - 1;
+      ob.x[i] = q.x[k];
+      ob.y[i] = q.y[k];
+    }
+    ob.tri = tri;
+    ob.area = area;
+  } else {
+    const RootQ& R = p.roots[root];
+    if (!pointToImage(p.camPlane, c.P, q.x, q.y, q.n, R.tri, ob)) return;
+  }
+  appendPoly(p, root, ob);
+}
+
+// ---------------------------------------------------------- root budgets
+__device__ __forceinline__ void rootDone(const Params& p) {
+  atomicAdd(&p.st->reserved, (unsigned long long)(-(long long)p.budget));
+}
+__device__ __forceinline__ void liveSub(const Params& p, int root, int n) {
+  if (atomicSub(&p.live[root], n) == n) rootDone(p);
+}
+__device__ __forceinline__ void killRoot(const Params& p, int root) {
+  if (atomicExch(&p.state[root], int(kKilled)) != int(kKilled)) atomicAdd(&p.st->killed, 1);
+}
+
+// ------------------------------------------------------------- the trace
+struct Stepper {
+  int phase;
+  Box box;
+};
+
+__device__ __forceinline__ int highestBit(uint64_t x) { return 63 - __builtin_clzll(x); }
+
+// wfAdvance (wavefront_core.h:157-231): descend to a leaf, skip the excluded
+// triangle, climb to the next pending far child; returns kContinue with a
+// triangle pending at r.cursor, or kFinished after emitting the output.
+__device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& ss, Cnt& cnt) {
+  const DevScene& sv = p.sv;
+  const Beam& B = w.b;
+  for (;;) {
+    if (ss.phase == kDescend) {
+      for (;;) {
+        const KdNodeD kn = sv.nodes[r.node];
+        if ((kn.data & 3u) == 3u) break;
+        cnt.v[sKd]++;
+        const int d = r.depth++;
+        const int a = int(kn.data & 3u);
+        const float s = kn.split;
+        const int sga = sgnOf(B, a);
+        const float Ns = float(sga) * (s - comp(c.P.O, a));
+        const uint32_t left = r.node + 1, right = kn.data >> 2;
+        const bool toRight = sga > 0;
+        int dec = kFar;
+        if (Ns > 0) dec = decide(c, B, a, Ns, ss.box);
+        // kFar: into the child the rays move into; otherwise the apex side
+        const bool goRight = (dec == kFar) == toRight;
+        if (dec == kBoth) r.trail |= uint64_t(1) << d;
+        r.node = goRight ? right : left;
+        if (goRight)
+          setComp(ss.box.lo, a, s);
+        else
+          setComp(ss.box.hi, a, s);
+      }
+      r.cursor = 0;
+      cnt.v[sLeaf]++;
+      ss.phase = kTest;
+    }
+    if (ss.phase == kTest) {
+      const KdNodeD leaf = sv.nodes[r.node];
+      const uint32_t count = leaf.data >> 2;
+      uint32_t i = uint32_t(r.cursor);
+      while (i < count && sv.refs[leaf.offset + i].tri == c.exclude) ++i;
+      r.cursor = int32_t(i);
+      if (i < count) return kContinue;
+      ss.phase = kClimb;
+    }
+    for (;;) {
+      if (r.trail == 0) {
+        emitOutput(p, c, B, r.root, c.anyHit() ? -1 : B.hit, cnt);
+        return kFinished;
+      }
+      const int d = highestBit(r.trail);
+      uint32_t node = r.node;
+      for (int depth = r.depth; depth > d; --depth) {
+        node = sv.parent[node];
+        cnt.v[sClimbs]++;
+      }
+      r.trail &= (uint64_t(1) << d) - 1;
+      const KdNodeD kn = sv.nodes[node];
+      const uint32_t cs = sgnOf(B, int(kn.data & 3u)) > 0 ? (kn.data >> 2) : node + 1;
+      r.node = node;
+      r.depth = int16_t(d);
+      const Box cb = sv.nodeBox[cs];
+      if (overlaps(c, B, cb)) {
+        r.node = cs;
+        r.depth = int16_t(d + 1);
+        r.cursor = -1;
+        ss.phase = kDescend;
+        ss.box = cb;
+        break;
+      }
+    }
+  }
+}
+
+// wfTestOne (wavefront_core.h:236-268).
+__device__ int testOne(const Params& p, const Ctx& c, Rec& r, Work& w, Cnt& cnt) {
+  const KdNodeD leaf = p.sv.nodes[r.node];
+  const uint32_t i = uint32_t(r.cursor);
+  r.cursor = int32_t(i + 1);
+  const LeafTri ref = p.sv.refs[leaf.offset + i];
+  const TriS ti = triSetup(c, ref);
+  if (!ti.ok) return kContinue;
+  cnt.v[sTests]++;
+  const int t = ti.tri;
+  if (w.b.hit == t) return kContinue;
+  int nl;
+  const int res = classifyTri(c, w, ti, nullptr, nl);
+  if (res == kKeep) return kContinue;
+  if (res == kAllIn) {
+    if (c.anyHit()) {
+      emitOutput(p, c, w.b, r.root, t, cnt);
+      return kFinished;
+    }
+    setStatus(c, w.b, t, ti.A, ti.C);
+    w.hitA[0] = ti.A[0];
+    w.hitA[1] = ti.A[1];
+    w.hitA[2] = ti.A[2];
+    w.hitC = ti.C;
+    w.hitN = ti.n;
+    r.hit = t;
+    return kContinue;
+  }
+  r.cursor = int32_t(i);
+  return kSplit;
+}
+
+// wfMoreInLeaf (wavefront_core.h:272-279).
+__device__ __forceinline__ bool moreInLeaf(const Params& p, const Ctx& c, Rec& r) {
+  const KdNodeD leaf = p.sv.nodes[r.node];
+  const uint32_t count = leaf.data >> 2;
+  uint32_t i = uint32_t(r.cursor);
+  while (i < count && p.sv.refs[leaf.offset + i].tri == c.exclude) ++i;
+  r.cursor = int32_t(i);
+  return i < count;
+}
+
+__global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
+  Cnt cnt;
+  cnt.zero();
+  const int n = int(min((unsigned long long)p.st->count[cur], p.capacity));
+  const Rec* q = p.tq[cur];
+  bool active = false, exhausted = false;
+  Rec r;
+  Work w;
+  Ctx c;
+  Stepper ss;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const int nIdle = __popcll(idle);
+    if (!exhausted && nIdle > 0) {
+      int base = 0;
+      if (lane() == 0) base = atomicAdd(&p.st->head[kHeadTrace], nIdle);
+      base = __shfl(base, 0, kWave);
+      if (base + nIdle >= n) exhausted = true;
+      if (!active) {
+        const int i = base + rankBelow(idle);
+        if (i < n) {
+          r = q[i];
+          if (r.n == 0 || p.state[r.root] == kKilled) {  // a dropped sliver, or an abandoned root
+            liveSub(p, r.root, 1);
+          } else {
+            c = ctxOf(p.roots[r.root]);
+            expand(p, c, r, w);
+            if (r.cursor < 0) {
+              ss.phase = kDescend;
+              ss.box = p.sv.nodeBox[r.node];
+            } else {
+              ss.phase = kTest;
+            }
+            active = true;
+          }
+        }
+      }
+    }
+    if (!__any(active)) {
+      if (exhausted) break;
+      continue;
+    }
+    int res = kContinue;
+    if (active) {
+      res = advance(p, c, r, w, ss, cnt);
+      while (res == kContinue) {
+        res = testOne(p, c, r, w, cnt);
+        if (res != kContinue || !moreInLeaf(p, c, r)) break;
+      }
+      if (res == kFinished) liveSub(p, r.root, 1);
+      if (res != kContinue) active = false;
+    }
+    const uint64_t splits = __ballot(res == kSplit);
+    if (splits) {
+      int base = 0;
+      if (lane() == 0) base = atomicAdd(&p.st->splitCount, __popcll(splits));
+      base = __shfl(base, 0, kWave);
+      if (res == kSplit) {
+        storeGeom(w.b, r);
+        p.sq[base + rankBelow(splits)] = r;
+      }
+    }
+  }
+  flushCnt(p, cnt);
+}
+
+// ------------------------------------------------------------- the split
+// Emitting pass: writes records (or outputs) exactly as wfSplitEmit
+// (wavefront_core.h:371-422) into dst[0..c0) for list 0 and dst[c0..) for
+// list 1.
+struct EmitSink {
+  const Params* p;
+  const Ctx* c;
+  const Beam* parent;
+  const Rec* base;
+  Rec* dst;
+  int k[2];
+  int t;
+  Cnt* cnt;
+  __device__ void push(int l, const Pg<6>& q0) {
+    const bool queued = !(l == 1 && c->anyHit());
+    const int hit = l == 1 ? t : parent->hit;
+    Pg<6> sub[2];
+    int ns = 1;
+    if (q0.n == 5) {
+      splitFive(q0, sub[0], sub[1]);
+      cnt->v[sFive]++;
+      ns = 2;
+    } else {
+      sub[0] = q0;
+    }
+    for (int j = 0; j < ns; ++j) {
+      Beam nb;
+      float dropped = 0;
+      const bool ok = beamGeom(*c, sub[j], parent->s0, parent->s1, parent->s2, nb, dropped);
+      if (!ok) {
+        cnt->dropped += double(dropped);
+      } else if (!queued) {
+        nb.hit = t;
+        emitOutput(*p, *c, nb, base->root, t, *cnt);
+      }
+      if (!queued) continue;
+      Rec rr = *base;
+      if (ok) {
+        storeGeom(nb, rr);
+        rr.hit = hit;
+      } else {
+        rr.n = 0;
+      }
+      dst[k[l]++] = rr;
+    }
+  }
+};
+
+__device__ void scheduleNext(const Params& p, int cur);
+
+__global__ void __launch_bounds__(kBlock) splitKernel(Params p, int cur) {
+  Cnt cnt;
+  cnt.zero();
+  const int n = int(min((unsigned long long)p.st->splitCount, p.capacity));
+  const int nxt = cur ^ 1;
+  for (;;) {
+    int base = 0;
+    if (lane() == 0) base = atomicAdd(&p.st->head[kHeadSplit], kWave);
+    base = __shfl(base, 0, kWave);
+    if (base >= n) break;
+    const int i = base + lane();
+    int slots = 0;
+    bool have = false, emit = false, commit = false;
+    Rec r;
+    Work w;
+    Ctx c;
+    Ln L[5];
+    int nl = 0;
+    int recs0 = 0;
+    int tri = -1;
+    if (i < n) {
+      r = p.sq[i];
+      if (p.state[r.root] == kKilled) {
+        liveSub(p, r.root, 1);
+      } else {
+        c = ctxOf(p.roots[r.root]);
+        expand(p, c, r, w);
+        have = true;
+      }
+    }
+    if (have) {
+      const KdNodeD leaf = p.sv.nodes[r.node];
+      const LeafTri ref = p.sv.refs[leaf.offset + uint32_t(r.cursor)];
+      const TriS ti = triSetup(c, ref);
+      tri = ti.tri;
+      classifyTri(c, w, ti, L, nl);  // kClip, as found by the trace
+      CountSink cs;
+      Ln Lc[5];
+      for (int k = 0; k < 5; ++k) Lc[k] = L[k];
+      clipTri(c, w.b, Lc, nl, cs, cnt.v[sFive]);
+      commit = cs.hitArea >= c.minArea;
+      recs0 = cs.recs[0];
+      slots = commit ? cs.recs[0] + (c.anyHit() ? 0 : cs.recs[1]) : 1;
+      const int old = atomicAdd(&p.live[r.root], slots - 1);
+      if (old + slots - 1 > p.budget) {
+        killRoot(p, r.root);
+        liveSub(p, r.root, slots);
+        slots = 0;
+      } else {
+        emit = true;
+        if (slots == 0 && old == 1) rootDone(p);
+      }
+    }
+    const int slot = waveAppend(&p.st->count[nxt], slots);
+    if (slots > 0 && (unsigned long long)(slot + slots) > p.capacity) {
+      p.st->overflow = 1;
+      emit = false;
+    }
+    if (emit) {
+      Rec b = r;
+      b.cursor = r.cursor + 1;
+      Rec* dst = p.tq[nxt] + slot;
+      if (!commit) {
+        dst[0] = b;
+      } else {
+        cnt.v[sSplits]++;
+        EmitSink es;
+        es.p = &p;
+        es.c = &c;
+        es.parent = &w.b;
+        es.base = &b;
+        es.dst = dst;
+        es.k[0] = 0;
+        es.k[1] = recs0;
+        es.t = tri;
+        es.cnt = &cnt;
+        uint32_t dummyFive = 0;
+        clipTri(c, w.b, L, nl, es, dummyFive);
+      }
+    }
+  }
+  flushCnt(p, cnt);
+  // The last workgroup to finish schedules the next round.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int done = atomicAdd(&p.st->blocksDone, 1);
+    if (done == int(gridDim.x) - 1) {
+      __threadfence();
+      scheduleNext(p, cur);
+    }
+  }
+}
+
+// scheduleKernel (cuda/wavefront.cu:327-345), run by one thread after a
+// round: resets the round's counters and admits roots while their budgets fit.
+__device__ void scheduleNext(const Params& p, int cur) {
+  WState* s = p.st;
+  const int nxt = cur ^ 1;
+  volatile WState* vs = s;
+  if (vs->done) {
+    s->blocksDone = 0;
+    s->head[0] = s->head[1] = s->head[2] = 0;
+    __threadfence();
+    return;
+  }
+  s->traceItems += (unsigned long long)vs->count[cur];
+  s->splitItems += (unsigned long long)vs->splitCount;
+  s->count[cur] = 0;
+  s->splitCount = 0;
+  s->head[0] = s->head[1] = s->head[2] = 0;
+  s->blocksDone = 0;
+  const unsigned long long reserved = vs->reserved;
+  const unsigned long long freeSlots = p.capacity > reserved ? p.capacity - reserved : 0;
+  long long nAdmit = (long long)(p.R - vs->next);
+  const long long fit = (long long)(freeSlots / (unsigned long long)p.budget);
+  if (fit < nAdmit) nAdmit = fit;
+  s->admitBegin = vs->next;
+  s->admitCount = int(nAdmit);
+  s->next = vs->next + int(nAdmit);
+  s->reserved = reserved + (unsigned long long)nAdmit * (unsigned long long)p.budget;
+  const int q = vs->count[nxt];
+  if (q > vs->maxQueue) s->maxQueue = q;
+  s->done = (nAdmit == 0 && vs->next >= p.R && q == 0) ? 1 : 0;
+  s->round = vs->round + 1;
+  __threadfence();
+}
+
+__global__ void initKernel(Params p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    WState* s = p.st;
+    s->next = 0;
+    s->count[0] = s->count[1] = 0;
+    s->splitCount = 0;
+    s->head[0] = s->head[1] = s->head[2] = 0;
+    s->done = 0;
+    s->killed = 0;
+    s->maxQueue = 0;
+    s->overflow = 0;
+    if (p.pass == 0) s->outCount = 0;  // retry passes append to the first pass's output
+    s->blocksDone = 0;
+    s->round = 0;
+    s->reserved = 0;
+    s->traceItems = s->splitItems = 0;
+    long long nAdmit = p.R;
+    const long long fit = (long long)(p.capacity / (unsigned long long)p.budget);
+    if (fit < nAdmit) nAdmit = fit;
+    s->admitBegin = 0;
+    s->admitCount = int(nAdmit);
+    s->next = int(nAdmit);
+    s->reserved = (unsigned long long)nAdmit * (unsigned long long)p.budget;
+    s->done = p.R == 0 ? 1 : 0;
+  }
+}
+
+// ------------------------------------------------------------ the admit
+// presplit (beam_core.h:487-546) + wfAdmit (wavefront_core.h:428-471):
+// split the root by the lines where a ray-direction component is zero, fan
+// each sign region into <= 4-gons, queue the pieces at the kd root.
+struct AdmitOut {
+  int pushed;
+  bool ok;
+};
+
+__device__ bool admitPiece(const Params& p, const Ctx& c, const Pg<8>& qd, const int* sg, int root, int cur,
+                           AdmitOut& ao, Cnt& cnt) {
+  Beam b;
+  float dropped = 0;
+  if (!beamGeom(c, qd, sg[0], sg[1], sg[2], b, dropped)) {
+    cnt.dropped += double(dropped);
+    return true;
+  }
+  setStatus(c, b, -1, nullptr, 0);
+  cnt.v[sRoots]++;
+  if (!overlaps(c, b, p.sv.bounds)) {
+    emitOutput(p, c, b, root, -1, cnt);
+    return true;
+  }
+  Rec r;
+  storeGeom(b, r);
+  r.trail = 0;
+  r.node = 0;
+  r.root = root;
+  r.hit = -1;
+  r.cursor = -1;
+  r.depth = 0;
+  const int slot = atomicAdd(&p.st->count[cur], 1);
+  if ((unsigned long long)slot >= p.capacity) {
+    p.st->overflow = 1;
+    ao.ok = false;
+    return false;
+  }
+  p.tq[cur][slot] = r;
+  ao.pushed++;
+  return true;
+}
+
+__device__ void admitPoly(const Params& p, const Ctx& c, const Pg<8>& rootPoly, int root, int cur, AdmitOut& ao,
+                          Cnt& cnt) {
+  struct Item {
+    Pg<8> p;
+    int axis;
+    int sg[3];
+  };
+  Item stack[16];
+  int sp = 0;
+  Pg<8> p0 = rootPoly;
+  if (p0.n < 3) return;
+  if (p0.area() < 0) p0.reverse();
+  stack[0].p = p0;
+  stack[0].axis = 0;
+  stack[0].sg[0] = stack[0].sg[1] = stack[0].sg[2] = 1;
+  sp = 1;
+  const Plane& P = c.P;
+  while (sp > 0) {
+    Item it = stack[--sp];
+    const Pg<8>& q = it.p;
+    if (q.n < 3) continue;
+    if (it.axis == 3) {
+      for (int k = 1; k + 1 < q.n; k += 2) {
+        Pg<8> qd;
+        qd.n = 0;
+        qd.push(q.x[0], q.y[0]);
+        qd.push(q.x[k], q.y[k]);
+        qd.push(q.x[k + 1], q.y[k + 1]);
+        if (k + 2 < q.n) qd.push(q.x[k + 2], q.y[k + 2]);
+        if (!admitPiece(p, c, qd, it.sg, root, cur, ao, cnt)) return;
+      }
+      continue;
+    }
+    const int axis = it.axis;
+    const Ln L = makeLine(comp(P.u, axis), comp(P.v, axis), comp(P.d, axis), c.ext);
+    const int cs = sidesOf(q, L, c.eps);
+    if (cs == 3) {
+      if (sp + 2 > 16) {
+        ao.ok = false;
+        return;
+      }
+      Pg<8> in, out;
+      splitPoly(q, L, c.eps, in, out);
+      cleanPoly(in, c.eps);
+      cleanPoly(out, c.eps);
+      Item a = it, b = it;
+      a.p = in;
+      a.axis = axis + 1;
+      a.sg[axis] = 1;
+      b.p = out;
+      b.axis = axis + 1;
+      b.sg[axis] = -1;
+      stack[sp++] = b;
+      stack[sp++] = a;
+      cnt.v[sPresplit]++;
+    } else if (cs != 0) {
+      it.axis = axis + 1;
+      it.sg[axis] = (cs & 1) ? 1 : -1;
+      stack[sp++] = it;
+    } else {
+      cnt.dropped += double(selAbs(q.area()));
+    }
+  }
+}
+
+// lightBeamSetup (soft_core.h:38-73).
+__device__ bool lightSetup(const Params& p, F3 x, F3 ns, int tri, RootQ& q, Pg<8>& root) {
+  const F3 nL = normalize(cross(p.lU, p.lV));
+  if (!(dot(x - p.lc, nL) > 0)) return false;
+  const F3 apex = x + ns * p.offset;
+  q.plane = Plane::make(apex, p.lc, p.lU, p.lV);
+  q.flags = fAnyHit | fFarIsPlane;
+  q.exclude = tri;
+  const F3 hu = p.lU * 0.5f, hv = p.lV * 0.5f;
+  const F3 corners[4] = {p.lc - hu - hv, p.lc + hu - hv, p.lc + hu + hv, p.lc - hu + hv};
+  Pg<8> quad;
+  quad.n = 0;
+  for (int i = 0; i < 4; ++i) {
+    float qx, qy;
+    q.plane.coords(corners[i], qx, qy);
+    quad.push(qx, qy);
+  }
+  if (quad.area() < 0) quad.reverse();
+  q.lightArea = quad.area();
+  const Ln hz{dot(ns, q.plane.u), dot(ns, q.plane.v), dot(ns, p.lc - x)};
+  int above = 0;
+  for (int i = 0; i < quad.n; ++i) above += hz.at(quad.x[i], quad.y[i]) > 0;
+  if (above == 0) return false;
+  if (above == quad.n) {
+    root = quad;
+  } else {
+    Pg<8> rest;
+    splitPoly(quad, hz, 0.0f, root, rest);
+  }
+  return root.n >= 3 && root.area() > 0;
+}
+
+// lightCenterG (soft_core.h:76-84).
+__device__ float centerG(const Params& p, F3 x, F3 ns) {
+  const F3 nL = normalize(cross(p.lU, p.lV));
+  if (!(dot(x - p.lc, nL) > 0)) return 0;
+  const F3 w = p.lc - x;
+  const float r2 = dot(w, w);
+  const F3 wn = w / __builtin_sqrtf(r2);
+  const float ci = dot(ns, wn), cl = -dot(nL, wn);
+  return (ci > 0 ? ci : 0) * (cl > 0 ? cl : 0) / r2;
+}
+
+// pointShadowSetup (point_core.h:15-36).
+__device__ bool pointSetup(const Params& p, F3 nT, F3 P0, int T, const float* px, const float* py, int n, RootQ& q,
+                           Pg<8>& root) {
+  const float sideCam = dot(nT, p.eye - P0), sideL = dot(nT, p.pointLight - P0);
+  if (!(sideCam * sideL > 0)) return false;
+  const F3 ax = __builtin_fabsf(nT.x) < 0.6f ? f3(1, 0, 0) : (__builtin_fabsf(nT.y) < 0.6f ? f3(0, 1, 0) : f3(0, 0, 1));
+  const F3 tu = normalize(cross(nT, ax));
+  q.plane = Plane::make(p.pointLight, P0, tu, cross(nT, tu));
+  q.flags = fAnyHit | fFarIsPlane;
+  q.exclude = T;
+  root.n = 0;
+  for (int i = 0; i < n; ++i) {
+    const F3 d = p.fwd + p.right * px[i] + p.down * py[i];
+    const F3 X = p.eye + d * (dot(nT, P0 - p.eye) / dot(nT, d));
+    float qx, qy;
+    q.plane.coords(X, qx, qy);
+    root.push(qx, qy);
+  }
+  return true;
+}
+
+// Soft-shadow roots are pixels in 8 x 8 tiles: a wave's 64 lanes trace one tile.
+__device__ __forceinline__ bool softPixel(const Params& p, int k, int& pix) {
+  const int t = k >> 6, l = k & 63;
+  const int x = (t % p.tilesX) * 8 + (l & 7), y = (t / p.tilesX) * 8 + (l >> 3);
+  pix = y * p.W + x;
+  return x < p.W && y < p.H;
+}
+
+__device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
+  p.live[k] = 0;
+  p.state[k] = kNoBeam;
+  p.epoch[k] = p.pass;
+  RootQ R;
+  R.flags = 0;
+  R.exclude = -1;
+  R.unitExp = 0;
+  R.tri = -1;
+  R.lightArea = 0;
+  R.x = R.ns = f3(0, 0, 0);
+  Pg<8> poly;
+  poly.n = 0;
+  if (p.app == kSoft) {
+    p.missFx[k] = 0;
+    p.gFx[k] = 0;
+    int pix;
+    if (!softPixel(p, k, pix)) {
+      rootDone(p);
+      return;
+    }
+    const int T = p.pixTri[pix];
+    if (T < 0) {
+      p.vis[pix] = -1;
+      p.E[pix] = 0;
+      p.Eu[pix] = 0;
+      rootDone(p);
+      return;
+    }
+    const F3 x = f3(p.pixX[3 * pix], p.pixX[3 * pix + 1], p.pixX[3 * pix + 2]);
+    const F3 ns = f3(p.pixN[3 * pix], p.pixN[3 * pix + 1], p.pixN[3 * pix + 2]);
+    p.Eu[pix] = length(cross(p.lU, p.lV)) * centerG(p, x, ns);
+    if (!lightSetup(p, x, ns, T, R, poly)) {
+      p.vis[pix] = 0;
+      p.E[pix] = 0;
+      rootDone(p);
+      return;
+    }
+    R.x = x;
+    R.ns = ns;
+  } else if (p.app == kPrimary) {
+    const int tx = k % p.tilesW, ty = k / p.tilesW;
+    const int x0 = tx * p.tile, y0 = ty * p.tile;
+    const int x1 = min(p.W, x0 + p.tile), y1 = min(p.H, y0 + p.tile);
+    const float qx0 = (2 * float(x0) / float(p.W) - 1) * p.halfW, qx1 = (2 * float(x1) / float(p.W) - 1) * p.halfW;
+    const float qy0 = (2 * float(y0) / float(p.H) - 1) * p.halfH, qy1 = (2 * float(y1) / float(p.H) - 1) * p.halfH;
+    poly.push(qx0, qy0);
+    poly.push(qx1, qy0);
+    poly.push(qx1, qy1);
+    poly.push(qx0, qy1);
+    R.plane = p.camPlane;
+    R.flags = p.cull ? fCull : 0;
+  } else {
+    const OutPoly pb = p.prim[k];
+    const int T = pb.tri;
+    R.tri = T;
+    const float4 nn = p.sv.triNV[2 * size_t(T)], vv = p.sv.triNV[2 * size_t(T) + 1];
+    if (!pointSetup(p, f3(nn.x, nn.y, nn.z), f3(vv.x, vv.y, vv.z), T, pb.x, pb.y, pb.n, R, poly)) {
+      appendPoly(p, k, pb);  // the receiver faces away from the light: all in shadow
+      rootDone(p);
+      return;
+    }
+  }
+  setupQuery(R, poly.extent());
+  if (p.app == kSoft) {
+    // Unit of the exact area sum: ulp(minArea); every output area is a
+    // multiple of it (areas >= minArea).
+    int e = 0;
+    (void)__builtin_frexpf(R.minArea, &e);
+    R.unitExp = e - 24 < -149 ? -149 : e - 24;
+  }
+  p.roots[k] = R;
+  p.state[k] = kTraced;
+  const Ctx c = ctxOf(R);
+  AdmitOut ao{0, true};
+  admitPoly(p, c, poly, k, cur, ao, cnt);
+  p.live[k] = ao.pushed;
+  if (!ao.ok || ao.pushed > p.budget) killRoot(p, k);
+  if (ao.pushed == 0) rootDone(p);
+}
+
+__global__ void __launch_bounds__(kBlock) admitKernel(Params p, int cur) {
+  Cnt cnt;
+  cnt.zero();
+  const int begin = p.st->admitBegin, n = p.st->admitCount;
+  for (;;) {
+    int base = 0;
+    if (lane() == 0) base = atomicAdd(&p.st->head[kHeadAdmit], kWave);
+    base = __shfl(base, 0, kWave);
+    if (base >= n) break;
+    const int i = base + lane();
+    if (i < n) {
+      const int slot = begin + i;
+      admitRoot(p, p.rootList ? p.rootList[slot] : slot, cur, cnt);
+    }
+  }
+  flushCnt(p, cnt);
+}
+
+// finalizeSoftKernel (cuda/wavefront.cu:539-561) with the exact sums:
+// vis = float(missArea) / lightArea as the CPU computes it
+// (src/render/render.cpp:291).
+__global__ void finalizeSoftKernel(Params p) {
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= p.R) return;
+  const int k = p.rootList ? p.rootList[i] : i;
+  int pix;
+  if (!softPixel(p, k, pix)) return;
+  const int s = p.state[k];
+  if (s != kTraced) return;  // no beam (written at admission) or killed (traced again)
+  const RootQ& R = p.roots[k];
+  const double miss = double(p.missFx[k]) * pow2(R.unitExp);
+  const float V0 = float(miss) / R.lightArea;
+  const float m = V0 < 1.0f ? V0 : 1.0f;  // std::max(0, std::min(1, V)) of render.cpp:302
+  const float V = 0.0f < m ? m : 0.0f;
+  p.vis[pix] = V;
+  p.E[pix] = p.exact ? float(double(p.gFx[k]) * pow2(-60)) : p.Eu[pix] * V;
+}
+
+__global__ void clearInt(int* x) {
+  if (threadIdx.x == 0) *x = 0;
+}
+
+// Killed roots, compacted for a retry pass.
+__global__ void collectKilled(Params p, int* list, int* count) {
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= p.R) return;
+  const int k = p.rootList ? p.rootList[i] : i;
+  if (p.state[k] == kKilled) list[atomicAdd(count, 1)] = k;
+}
+
+// ------------------------------------------------------------ primary rays
+// rayHitTri (src/ray/ray_core.h:22-37), Moller-Trumbore on a leaf reference.
+__device__ __forceinline__ float rayTri(const LeafTri& t, F3 o, F3 d) {
+  const F3 v0 = f3(t.v[0], t.v[1], t.v[2]);
+  const F3 e1 = f3(t.v[3], t.v[4], t.v[5]) - v0, e2 = f3(t.v[6], t.v[7], t.v[8]) - v0;
+  const F3 pv = cross(d, e2);
+  const float det = dot(e1, pv);
+  if (det == 0) return __builtin_inff();
+  const float inv = 1 / det;
+  const F3 tv = o - v0;
+  const float u = dot(tv, pv) * inv;
+  if (u < 0 || u > 1) return __builtin_inff();
+  const F3 qv = cross(tv, e1);
+  const float v = dot(d, qv) * inv;
+  if (v < 0 || u + v > 1) return __builtin_inff();
+  return dot(e2, qv) * inv;
+}
+
+// rayTraverse<false> (ray_core.h:42-132): nearest hit in (0, inf), coplanar
+// ties to the lower id within 4 ulp.
+__device__ bool rayNearest(const DevScene& sv, F3 o, F3 d, int& hitTri, float& hitT, Cnt& cnt) {
+  cnt.v[sRays]++;
+  float t0 = 0, t1 = __builtin_inff();
+  float inv[3];
+  const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+  const float blo[3] = {sv.bounds.lo.x, sv.bounds.lo.y, sv.bounds.lo.z};
+  const float bhi[3] = {sv.bounds.hi.x, sv.bounds.hi.y, sv.bounds.hi.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    inv[a] = 1 / dd[a];
+    if (dd[a] == 0) {
+      if (oo[a] < blo[a] || oo[a] > bhi[a]) return false;
+      continue;
+    }
+    float ta = (blo[a] - oo[a]) * inv[a];
+    float tb = (bhi[a] - oo[a]) * inv[a];
+    if (ta > tb) {
+      const float t = ta;
+      ta = tb;
+      tb = t;
+    }
+    t0 = selMax(t0, ta);
+    t1 = selMin(t1, tb);
+  }
+  if (t0 > t1) return false;
+  uint32_t stNode[64];
+  float stT0[64], stT1[64];
+  int sp = 0;
+  uint32_t node = 0;
+  float best = __builtin_inff();
+  int bestTri = -1;
+  for (;;) {
+    const KdNodeD n = sv.nodes[node];
+    if ((n.data & 3u) != 3u) {
+      cnt.v[sKd]++;
+      const int a = int(n.data & 3u);
+      const float oa = a == 0 ? o.x : (a == 1 ? o.y : o.z);
+      const float da = a == 0 ? d.x : (a == 1 ? d.y : d.z);
+      const float ia = a == 0 ? inv[0] : (a == 1 ? inv[1] : inv[2]);
+      const bool belowFirst = oa < n.split || (oa == n.split && da <= 0);
+      const uint32_t first = belowFirst ? node + 1 : (n.data >> 2);
+      const uint32_t second = belowFirst ? (n.data >> 2) : node + 1;
+      if (da == 0) {
+        node = first;
+        continue;
+      }
+      const float tp = (n.split - oa) * ia;
+      if (tp > t1 || tp <= 0) {
+        node = first;
+      } else if (tp < t0) {
+        node = second;
+      } else {
+        if (sp < 64) {
+          stNode[sp] = second;
+          stT0[sp] = tp;
+          stT1[sp] = t1;
+          ++sp;
+        }
+        node = first;
+        t1 = tp;
+      }
+      continue;
+    }
+    cnt.v[sLeaf]++;
+    const uint32_t count = n.data >> 2;
+    for (uint32_t i = 0; i < count; ++i) {
+      const LeafTri ref = sv.refs[n.offset + i];
+      cnt.v[sTests]++;
+      const float th = rayTri(ref, o, d);
+      if (!(th > 0 && th < __builtin_inff())) continue;
+      const float tol = bestTri < 0 ? 0.0f : best * (4 * 1.1920928955078125e-7f);
+      if (bestTri < 0 || th < best - tol || (th <= best + tol && ref.tri < bestTri)) {
+        best = th;
+        bestTri = ref.tri;
+      }
+    }
+    if (bestTri >= 0 && best <= t1) break;
+    if (sp == 0) break;
+    --sp;
+    node = stNode[sp];
+    t0 = stT0[sp];
+    t1 = stT1[sp];
+  }
+  if (bestTri < 0) return false;
+  cnt.v[sHits]++;
+  hitTri = bestTri;
+  hitT = best;
+  return true;
+}
+
+// The soft-shadow primary pass (cuda/gpu_render.cu:222-244): one ray through
+// each pixel centre; X = o + d t, N = the triangle normal facing the eye.
+__global__ void __launch_bounds__(kBlock) primaryRayKernel(Params p) {
+  Cnt cnt;
+  cnt.zero();
+  const int x = int(blockIdx.x) * 16 + int(threadIdx.x & 15), y = int(blockIdx.y) * 16 + int(threadIdx.x >> 4);
+  if (x < p.W && y < p.H) {
+    const size_t k = size_t(y) * size_t(p.W) + size_t(x);
+    const float qx = (2 * (float(x) + 0.5f) / float(p.W) - 1) * p.halfW;
+    const float qy = (2 * (float(y) + 0.5f) / float(p.H) - 1) * p.halfH;
+    const F3 d = p.fwd + p.right * qx + p.down * qy;
+    int t = -1;
+    float th = 0;
+    int* triOut = const_cast<int*>(p.pixTri);
+    float* X = const_cast<float*>(p.pixX);
+    float* N = const_cast<float*>(p.pixN);
+    triOut[k] = -1;
+    if (rayNearest(p.sv, p.eye, d, t, th, cnt)) {
+      triOut[k] = t;
+      const F3 hp = p.eye + d * th;
+      const float4 nn = p.sv.triNV[2 * size_t(t)];
+      F3 nrm = f3(nn.x, nn.y, nn.z);
+      if (dot(nrm, d) > 0) nrm = -nrm;
+      X[3 * k] = hp.x;
+      X[3 * k + 1] = hp.y;
+      X[3 * k + 2] = hp.z;
+      N[3 * k] = nrm.x;
+      N[3 * k + 1] = nrm.y;
+      N[3 * k + 2] = nrm.z;
+    }
+  }
+  flushCnt(p, cnt);
+}
+
+}  // namespace
+
+// ======================================================================
+// Host side
+// ======================================================================
+
+#define BTG_TRY(x)                                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) {                                                               \
+      err = std::string(#x) + ": " + hipGetErrorString(e_);                               \
+      return BTG_ERR_DEVICE_CODE;                                                         \
+    }                                                                                     \
+  } while (0)
+
+constexpr int BTG_ERR_DEVICE_CODE = -2;
+constexpr int BTG_ERR_MEMORY_CODE = -3;
+constexpr int BTG_ERR_LIMIT_CODE = -4;
+constexpr int BTG_ERR_ARG_CODE = -1;
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+class Engine {
+ public:
+  DevScene sv;
+  Options opt;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  int cus = 0;
+  // scene storage
+  void* sceneMem[5] = {};
+  // engine buffers (grown on demand)
+  DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
+  DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry;
+  WState* hostState = nullptr;  // pinned
+  // per-launch timing (Options::profile): event pairs and the kernel kind
+  std::vector<hipEvent_t> evPool;
+  std::vector<int> evKind;
+  size_t evUsed = 0;
+  int beginLaunch(int kind) {
+    if (!opt.profile) return -1;
+    while (evPool.size() < 2 * (evUsed + 1)) {
+      hipEvent_t a;
+      if (hipEventCreate(&a) != hipSuccess) return -1;
+      evPool.push_back(a);
+    }
+    (void)hipEventRecord(evPool[2 * evUsed], stream);
+    evKind.resize(evUsed + 1);
+    evKind[evUsed] = kind;
+    return int(evUsed++);
+  }
+  void endLaunch(int slot) {
+    if (slot >= 0) (void)hipEventRecord(evPool[2 * size_t(slot) + 1], stream);
+  }
+  // Sums the recorded launch durations into info (after the stream synced).
+  void collectLaunches(RunInfo& info) {
+    for (size_t i = 0; i < evUsed; ++i) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, evPool[2 * i], evPool[2 * i + 1]) == hipSuccess) {
+        info.kernelMs[evKind[i]] += ms;
+        info.launches[evKind[i]]++;
+      }
+    }
+    evUsed = 0;
+  }
+
+  int ensure(DevBuf& b, size_t bytes, std::string& err) {
+    if (b.bytes >= bytes && b.ptr) return 0;
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    const size_t want = bytes ? bytes : 16;
+    if (hipMalloc(&b.ptr, want) != hipSuccess) {
+      (void)hipGetLastError();
+      err = "device allocation of " + std::to_string(want) + " bytes failed";
+      return BTG_ERR_MEMORY_CODE;
+    }
+    b.bytes = want;
+    return 0;
+  }
+  ~Engine() {
+    DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
+                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixN, &vis, &E, &Eu, &prim, &retry};
+    for (DevBuf* b : all)
+      if (b->ptr) (void)hipFree(b->ptr);
+    for (void* m : sceneMem)
+      if (m) (void)hipFree(m);
+    if (hostState) (void)hipHostFree(hostState);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Persistent grids: as many workgroups as can be resident at once.
+template <class K>
+int residentGrid(const Engine& e, K kernel) {
+  int perCU = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kBlock, 0) != hipSuccess || perCU < 1) perCU = 1;
+  return e.cus * perCU;
+}
+
+// Prepares engine buffers for R roots and the common parameters.
+int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) {
+  const size_t cap = size_t(e.opt.capacity);
+  int rc;
+  if ((rc = e.ensure(e.tq0, cap * sizeof(Rec), err)) || (rc = e.ensure(e.tq1, cap * sizeof(Rec), err)) ||
+      (rc = e.ensure(e.sq, cap * sizeof(Rec), err)) || (rc = e.ensure(e.roots, size_t(R) * sizeof(RootQ), err)) ||
+      (rc = e.ensure(e.live, size_t(R) * 4, err)) || (rc = e.ensure(e.state, size_t(R) * 4, err)) ||
+      (rc = e.ensure(e.epoch, size_t(R) * 4, err)) || (rc = e.ensure(e.missFx, size_t(R) * 8, err)) ||
+      (rc = e.ensure(e.gFx, size_t(R) * 8, err)) || (rc = e.ensure(e.stats, sizeof(uint64_t) * sNum, err)) ||
+      (rc = e.ensure(e.dropped, sizeof(double), err)) || (rc = e.ensure(e.st, sizeof(WState), err)) ||
+      (rc = e.ensure(e.list, size_t(R) * 4, err)) || (rc = e.ensure(e.listCount, 4, err)))
+    return rc;
+  p = Params();
+  p.sv = e.sv;
+  p.app = app;
+  p.R = R;
+  p.rootList = nullptr;
+  p.budget = budget;
+  p.capacity = (unsigned long long)cap;
+  p.tq[0] = static_cast<Rec*>(e.tq0.ptr);
+  p.tq[1] = static_cast<Rec*>(e.tq1.ptr);
+  p.sq = static_cast<Rec*>(e.sq.ptr);
+  p.roots = static_cast<RootQ*>(e.roots.ptr);
+  p.live = static_cast<int*>(e.live.ptr);
+  p.state = static_cast<int*>(e.state.ptr);
+  p.epoch = static_cast<int*>(e.epoch.ptr);
+  p.pass = 0;
+  p.missFx = static_cast<unsigned long long*>(e.missFx.ptr);
+  p.gFx = static_cast<unsigned long long*>(e.gFx.ptr);
+  p.stats = static_cast<unsigned long long*>(e.stats.ptr);
+  p.dropped = static_cast<double*>(e.dropped.ptr);
+  p.st = static_cast<WState*>(e.st.ptr);
+  return 0;
+}
+
+void setCamera(const HostCam& cam, Params& p) {
+  p.W = cam.W;
+  p.H = cam.H;
+  p.eye = cam.eye;
+  p.fwd = cam.fwd;
+  p.right = cam.right;
+  p.down = cam.down;
+  p.halfW = cam.halfW;
+  p.halfH = cam.halfH;
+  // Camera::plane (render_types.h:28); host code rounds like the device
+  // (IEEE single, no contraction).
+  p.camPlane = Plane::make(cam.eye, cam.eye + cam.fwd, cam.right, cam.down);
+}
+
+
+// Runs the rounds of one pass until the state reports done.
+int runPass(Engine& e, Params& p, RunInfo& info, std::string& err) {
+  const int gA = residentGrid(e, admitKernel), gT = residentGrid(e, traceKernel), gS = residentGrid(e, splitKernel);
+  initKernel<<<1, 64, 0, e.stream>>>(p);
+  BTG_TRY(hipGetLastError());
+  int rounds = 0;
+  for (;;) {
+    const int k = std::max(1, e.opt.roundsPerCheck);
+    for (int i = 0; i < k; ++i, ++rounds) {
+      const int cur = rounds & 1;
+      int ls = e.beginLaunch(0);
+      admitKernel<<<gA, kBlock, 0, e.stream>>>(p, cur);
+      e.endLaunch(ls);
+      ls = e.beginLaunch(1);
+      traceKernel<<<gT, kBlock, 0, e.stream>>>(p, cur);
+      e.endLaunch(ls);
+      ls = e.beginLaunch(2);
+      splitKernel<<<gS, kBlock, 0, e.stream>>>(p, cur);
+      e.endLaunch(ls);
+    }
+    BTG_TRY(hipGetLastError());
+    BTG_TRY(hipMemcpyAsync(e.hostState, p.st, sizeof(WState), hipMemcpyDeviceToHost, e.stream));
+    BTG_TRY(hipStreamSynchronize(e.stream));
+    if (e.hostState->overflow) {
+      err = "sub-beam queue overflow (capacity " + std::to_string(e.opt.capacity) + ")";
+      return BTG_ERR_LIMIT_CODE;
+    }
+    if (e.hostState->done) break;
+    if (rounds > 200000) {
+      err = "wavefront engine did not finish";
+      return BTG_ERR_LIMIT_CODE;
+    }
+  }
+  info.rounds += e.hostState->round;
+  info.maxQueue = std::max(info.maxQueue, e.hostState->maxQueue);
+  return 0;
+}
+
+int readStats(Engine& e, uint64_t* dst, double* dropped, std::string& err) {
+  uint64_t s[sNum];
+  double d = 0;
+  BTG_TRY(hipMemcpyAsync(s, e.stats.ptr, sizeof(s), hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipMemcpyAsync(&d, e.dropped.ptr, sizeof(d), hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipStreamSynchronize(e.stream));
+  for (int i = 0; i < sNum; ++i) dst[i] += s[i];
+  if (dropped) *dropped += d;
+  return 0;
+}
+
+int zeroStats(Engine& e, std::string& err) {
+  const uint64_t z[sNum + 1] = {};
+  BTG_TRY(hipMemcpyAsync(e.stats.ptr, z, sizeof(uint64_t) * sNum, hipMemcpyHostToDevice, e.stream));
+  BTG_TRY(hipMemcpyAsync(e.dropped.ptr, z, sizeof(double), hipMemcpyHostToDevice, e.stream));
+  return 0;
+}
+
+// The main pass plus GPU retry passes for roots over budget. `finalize`
+// runs after each pass over that pass's roots.
+template <class Finalize>
+int runAll(Engine& e, Params& p, int baseBudget, RunInfo& info, Finalize finalize, std::string& err) {
+  int rc = runPass(e, p, info, err);
+  if (rc) return rc;
+  if ((rc = finalize(p))) return rc;
+  info.killed = e.hostState->killed;
+  int budget = baseBudget;
+  while (e.hostState->killed > 0) {
+    // Collect the killed roots and trace them again with a larger budget.
+    clearInt<<<1, 64, 0, e.stream>>>(static_cast<int*>(e.listCount.ptr));
+    collectKilled<<<(p.R + 255) / 256, 256, 0, e.stream>>>(p, static_cast<int*>(e.list.ptr),
+                                                             static_cast<int*>(e.listCount.ptr));
+    BTG_TRY(hipGetLastError());
+    int nk = 0;
+    BTG_TRY(hipMemcpyAsync(&nk, e.listCount.ptr, 4, hipMemcpyDeviceToHost, e.stream));
+    BTG_TRY(hipStreamSynchronize(e.stream));
+    if (nk == 0) break;
+    if ((long long)budget >= e.opt.capacity) {
+      err = "a root needs more sub-beams than the queue capacity (" + std::to_string(e.opt.capacity) + ")";
+      return BTG_ERR_LIMIT_CODE;
+    }
+    budget = int(std::min<long long>((long long)budget * 32, e.opt.capacity));
+    // The retry list must not alias the list collectKilled writes next: copy it.
+    int rc2;
+    Params q = p;
+    q.R = nk;
+    q.budget = budget;
+    q.pass = p.pass + 1;
+    if ((rc2 = e.ensure(e.retry, size_t(nk) * 4, err))) return rc2;
+    BTG_TRY(hipMemcpyAsync(e.retry.ptr, e.list.ptr, size_t(nk) * 4, hipMemcpyDeviceToDevice, e.stream));
+    q.rootList = static_cast<const int*>(e.retry.ptr);
+    info.retries++;
+    if ((rc = runPass(e, q, info, err))) return rc;
+    if ((rc = finalize(q))) return rc;
+    p.pass = q.pass;
+    p.rootList = q.rootList;
+    p.R = q.R;
+  }
+  return 0;
+}
+
+}  // namespace
+
+namespace {
+
+template <class T>
+int upload(Engine& e, int slot, const T* src, size_t n, const T*& dst, std::string& err) {
+  void* d = nullptr;
+  if (hipMalloc(&d, sizeof(T) * (n ? n : 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    err = "device allocation of the scene failed";
+    return BTG_ERR_MEMORY_CODE;
+  }
+  e.sceneMem[slot] = d;
+  if (n) BTG_TRY(hipMemcpy(d, src, sizeof(T) * n, hipMemcpyHostToDevice));
+  dst = static_cast<const T*>(d);
+  return 0;
+}
+
+}  // namespace
+
+// GpuRenderer::GpuRenderer (cuda/gpu_render.cu:489-528): the scene, kd-tree
+// and packed leaf triangles go to device memory once.
+Engine* engineCreate(const btg_scene_desc& d, std::string& err) {
+  Engine* e = new Engine();
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    err = "no HIP device";
+    delete e;
+    return nullptr;
+  }
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&e->hostState), sizeof(WState), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    err = "stream / pinned allocation failed";
+    delete e;
+    return nullptr;
+  }
+  for (hipEvent_t& ev : e->ev)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      err = "event creation failed";
+      delete e;
+      return nullptr;
+    }
+  // per-triangle (normal, 0), (vertex 0, 0) for hit planes and shading normals
+  std::vector<float4> nv(size_t(d.numTris) * 2);
+  for (int64_t t = 0; t < d.numTris; ++t) {
+    const uint32_t v0 = d.tri3[3 * t];
+    nv[size_t(2 * t)] = make_float4(d.triN[3 * t], d.triN[3 * t + 1], d.triN[3 * t + 2], 0.0f);
+    nv[size_t(2 * t + 1)] = make_float4(d.pos[3 * v0], d.pos[3 * v0 + 1], d.pos[3 * v0 + 2], 0.0f);
+  }
+  // leaf triangles in kd reference order (buildTriRefs, beam_types.h:100-112)
+  std::vector<LeafTri> refs(static_cast<size_t>(d.numRefs));
+  for (int64_t i = 0; i < d.numRefs; ++i) {
+    const uint32_t t = d.triIndices[i];
+    LeafTri& r = refs[size_t(i)];
+    for (int k = 0; k < 3; ++k) {
+      r.vi[k] = d.tri3[3 * size_t(t) + k];
+      for (int a = 0; a < 3; ++a) r.v[3 * k + a] = d.pos[3 * size_t(r.vi[k]) + a];
+      r.n[k] = d.triN[3 * size_t(t) + k];
+    }
+    r.tri = int32_t(t);
+  }
+  std::vector<Box> boxes(static_cast<size_t>(d.numNodes));
+  for (int64_t i = 0; i < d.numNodes; ++i) {
+    const float* b = d.nodeBox + 6 * i;
+    boxes[size_t(i)] = Box{f3(b[0], b[1], b[2]), f3(b[3], b[4], b[5])};
+  }
+  DevScene& sv = e->sv;
+  int rc = upload(*e, 0, nv.data(), nv.size(), sv.triNV, err);
+  if (!rc) rc = upload(*e, 1, static_cast<const KdNodeD*>(d.nodes), size_t(d.numNodes), sv.nodes, err);
+  if (!rc) rc = upload(*e, 2, boxes.data(), boxes.size(), sv.nodeBox, err);
+  if (!rc) rc = upload(*e, 3, d.parent, size_t(d.numNodes), sv.parent, err);
+  if (!rc) rc = upload(*e, 4, refs.data(), refs.size(), sv.refs, err);
+  if (rc) {
+    delete e;
+    return nullptr;
+  }
+  sv.bounds = Box{f3(d.kdBounds[0], d.kdBounds[1], d.kdBounds[2]), f3(d.kdBounds[3], d.kdBounds[4], d.kdBounds[5])};
+  sv.numTris = int(d.numTris);
+  return e;
+}
+
+void engineDestroy(Engine* e) { delete e; }
+
+Options& engineOptions(Engine* e) { return e->opt; }
+
+int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, float offset, bool exact, int32_t* tri, float* vis,
+               float* E, float* Eu, RunInfo& info, std::string& err) {
+  const auto t0 = std::chrono::steady_clock::now();
+  (void)t0;
+  const int W = cam.W, H = cam.H;
+  if (W <= 0 || H <= 0) {
+    err = "image size must be positive";
+    return BTG_ERR_ARG_CODE;
+  }
+  const size_t npix = size_t(W) * size_t(H);
+  const int tilesX = (W + 7) / 8, tilesY = (H + 7) / 8;
+  const long long R = (long long)tilesX * tilesY * 64;
+  if (R > (1LL << 30)) {
+    err = "image too large";
+    return BTG_ERR_ARG_CODE;
+  }
+  Params p;
+  int rc = prepare(*e, kSoft, int(R), e->opt.budget, p, err);
+  if (rc) return rc;
+  if ((rc = e->ensure(e->pixTri, npix * 4, err)) || (rc = e->ensure(e->pixX, npix * 12, err)) ||
+      (rc = e->ensure(e->pixN, npix * 12, err)) || (rc = e->ensure(e->vis, npix * 4, err)) ||
+      (rc = e->ensure(e->E, npix * 4, err)) || (rc = e->ensure(e->Eu, npix * 4, err)))
+    return rc;
+  setCamera(cam, p);
+  p.tilesX = tilesX;
+  p.lc = L.c;
+  p.lU = L.U;
+  p.lV = L.V;
+  p.offset = offset;
+  p.exact = exact ? 1 : 0;
+  p.pixTri = static_cast<const int*>(e->pixTri.ptr);
+  p.pixX = static_cast<const float*>(e->pixX.ptr);
+  p.pixN = static_cast<const float*>(e->pixN.ptr);
+  p.vis = static_cast<float*>(e->vis.ptr);
+  p.E = static_cast<float*>(e->E.ptr);
+  p.Eu = static_cast<float*>(e->Eu.ptr);
+  if ((rc = zeroStats(*e, err))) return rc;
+  BTG_TRY(hipEventRecord(e->ev[0], e->stream));
+  dim3 grid((W + 15) / 16, (H + 15) / 16);
+  const int lp = e->beginLaunch(3);
+  primaryRayKernel<<<grid, kBlock, 0, e->stream>>>(p);
+  e->endLaunch(lp);
+  BTG_TRY(hipGetLastError());
+  BTG_TRY(hipEventRecord(e->ev[1], e->stream));
+  // the primary pass's counters are reported separately
+  if ((rc = readStats(*e, info.rayStats, nullptr, err))) return rc;
+  if ((rc = zeroStats(*e, err))) return rc;
+  BTG_TRY(hipEventRecord(e->ev[2], e->stream));
+  auto fin = [&](Params& q) -> int {
+    finalizeSoftKernel<<<(q.R + 255) / 256, 256, 0, e->stream>>>(q);
+    BTG_TRY(hipGetLastError());
+    return 0;
+  };
+  if ((rc = runAll(*e, p, e->opt.budget, info, fin, err))) return rc;
+  BTG_TRY(hipEventRecord(e->ev[3], e->stream));
+  BTG_TRY(hipEventSynchronize(e->ev[3]));
+  float ms0 = 0, ms1 = 0;
+  BTG_TRY(hipEventElapsedTime(&ms0, e->ev[0], e->ev[1]));
+  BTG_TRY(hipEventElapsedTime(&ms1, e->ev[2], e->ev[3]));
+  info.primaryMs += ms0;
+  info.traceMs += ms1;
+  if ((rc = readStats(*e, info.stats, &info.dropped, err))) return rc;
+  e->collectLaunches(info);
+  if (tri) BTG_TRY(hipMemcpyAsync(tri, e->pixTri.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
+  if (vis) BTG_TRY(hipMemcpyAsync(vis, e->vis.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
+  if (E) BTG_TRY(hipMemcpyAsync(E, e->E.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
+  if (Eu) BTG_TRY(hipMemcpyAsync(Eu, e->Eu.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
+  BTG_TRY(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+namespace {
+
+// Downloads the polygons whose tag equals their root's last pass.
+int downloadPolys(Engine& e, int R, std::vector<OutPoly>& polys, std::string& err) {
+  const int n = std::min(e.hostState->outCount, int(e.out.bytes / sizeof(OutPoly)));
+  std::vector<OutPoly> out(static_cast<size_t>(std::max(n, 0)));
+  std::vector<int> root(out.size()), tag(out.size()), epoch(static_cast<size_t>(R));
+  if (n > 0) {
+    BTG_TRY(hipMemcpyAsync(out.data(), e.out.ptr, sizeof(OutPoly) * size_t(n), hipMemcpyDeviceToHost, e.stream));
+    BTG_TRY(hipMemcpyAsync(root.data(), e.outRoot.ptr, sizeof(int) * size_t(n), hipMemcpyDeviceToHost, e.stream));
+    BTG_TRY(hipMemcpyAsync(tag.data(), e.outTag.ptr, sizeof(int) * size_t(n), hipMemcpyDeviceToHost, e.stream));
+  }
+  if (R > 0) BTG_TRY(hipMemcpyAsync(epoch.data(), e.epoch.ptr, sizeof(int) * size_t(R), hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipStreamSynchronize(e.stream));
+  polys.clear();
+  polys.reserve(out.size());
+  for (int i = 0; i < n; ++i)
+    if (tag[size_t(i)] == epoch[size_t(root[size_t(i)])]) polys.push_back(out[size_t(i)]);
+  return 0;
+}
+
+// Primary and point shadows share the polygon-output driver; the output
+// buffer grows and the call repeats if it was too small.
+template <class Setup>
+int runPolys(Engine& e, int app, int R, int budget, size_t outCap, const HostCam& cam, Setup setup,
+             std::vector<OutPoly>& polys, RunInfo& info, std::string& err) {
+  for (;;) {
+    Params p;
+    int rc = prepare(e, app, R, budget, p, err);
+    if (rc) return rc;
+    if ((rc = e.ensure(e.out, outCap * sizeof(OutPoly), err)) || (rc = e.ensure(e.outRoot, outCap * 4, err)) ||
+        (rc = e.ensure(e.outTag, outCap * 4, err)))
+      return rc;
+    setCamera(cam, p);
+    p.out = static_cast<OutPoly*>(e.out.ptr);
+    p.outRoot = static_cast<int*>(e.outRoot.ptr);
+    p.outTag = static_cast<int*>(e.outTag.ptr);
+    p.outCap = int(std::min<size_t>(outCap, size_t(1) << 30));
+    setup(p);
+    RunInfo ri;
+    if ((rc = zeroStats(e, err))) return rc;
+    BTG_TRY(hipEventRecord(e.ev[0], e.stream));
+    auto fin = [&](Params&) -> int { return 0; };
+    if ((rc = runAll(e, p, budget, ri, fin, err))) return rc;
+    BTG_TRY(hipEventRecord(e.ev[1], e.stream));
+    BTG_TRY(hipEventSynchronize(e.ev[1]));
+    if (e.hostState->outCount > p.outCap) {
+      outCap = size_t(e.hostState->outCount) + size_t(e.hostState->outCount) / 4;
+      e.evUsed = 0;
+      continue;
+    }
+    float ms = 0;
+    BTG_TRY(hipEventElapsedTime(&ms, e.ev[0], e.ev[1]));
+    ri.traceMs = ms;
+    if ((rc = readStats(e, ri.stats, &ri.dropped, err))) return rc;
+    e.collectLaunches(ri);
+    if ((rc = downloadPolys(e, R, polys, err))) return rc;
+    info = ri;
+    return 0;
+  }
+}
+
+}  // namespace
+
+int enginePrimary(Engine* e, const HostCam& cam, bool cull, int tile, std::vector<OutPoly>& out, RunInfo& info,
+                  std::string& err) {
+  if (tile <= 0 || cam.W <= 0 || cam.H <= 0) {
+    err = "tile and image size must be positive";
+    return BTG_ERR_ARG_CODE;
+  }
+  const int tilesW = (cam.W + tile - 1) / tile, tilesH = (cam.H + tile - 1) / tile;
+  const size_t outCap = std::max<size_t>(size_t(1) << 16, size_t(cam.W) * size_t(cam.H) * 2);
+  auto setup = [&](Params& p) {
+    p.tile = tile;
+    p.tilesW = tilesW;
+    p.cull = cull ? 1 : 0;
+  };
+  return runPolys(*e, kPrimary, tilesW * tilesH, e->opt.primaryBudget, outCap, cam, setup, out, info, err);
+}
+
+int enginePoint(Engine* e, const HostCam& cam, const OutPoly* prim, int64_t nPrim, F3 light,
+                std::vector<OutPoly>& out, RunInfo& info, std::string& err) {
+  out.clear();
+  if (nPrim <= 0) return 0;
+  if (nPrim > (1LL << 30)) {
+    err = "too many primary polygons";
+    return BTG_ERR_ARG_CODE;
+  }
+  for (int64_t i = 0; i < nPrim; ++i)
+    if (prim[i].tri < 0 || prim[i].tri >= e->sv.numTris || prim[i].n < 3 || prim[i].n > 4) {
+      err = "primary polygon " + std::to_string(i) + " has no valid hit triangle or corner count";
+      return BTG_ERR_ARG_CODE;
+    }
+  int rc = e->ensure(e->prim, size_t(nPrim) * sizeof(OutPoly), err);
+  if (rc) return rc;
+  BTG_TRY(hipMemcpyAsync(e->prim.ptr, prim, size_t(nPrim) * sizeof(OutPoly), hipMemcpyHostToDevice, e->stream));
+  const size_t outCap = std::max<size_t>(size_t(1) << 16, size_t(nPrim) * 4);
+  auto setup = [&](Params& p) {
+    p.pointLight = light;
+    p.prim = static_cast<const OutPoly*>(e->prim.ptr);
+  };
+  return runPolys(*e, kPoint, int(nPrim), e->opt.budget, outCap, cam, setup, out, info, err);
+}
+
+}  // namespace btg

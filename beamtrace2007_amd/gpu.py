@@ -1,0 +1,247 @@
+"""Python mirror of the reference's GpuRenderer (cuda/gpu_render.h:26-73) over
+the C ABI of lib/libbtg.so (include/btg.h).
+
+    r = GpuRenderer(scene)                 # scene: beamtrace2007_amd.host.Scene with its kd-tree
+    res = r.soft_shadows(cam, light, exact=True)
+    hits = r.primary(cam, cull=False, tile=16)
+    shadow = r.point_shadows(cam, hits, light_pos)
+
+Every call runs the gfx950 kernels; there is no CPU path. If the library or a
+device is missing, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .host import LIB_DIR, Scene
+
+_lib = None
+
+
+class btg_device_info(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("computeUnits", C.c_int),
+                ("wavefrontSize", C.c_int), ("memGB", C.c_double)]
+
+
+class btg_scene_desc(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("numVerts", C.c_int64), ("tri3", C.c_void_p), ("triN", C.c_void_p),
+                ("numTris", C.c_int64), ("nodes", C.c_void_p), ("numNodes", C.c_int64), ("triIndices", C.c_void_p),
+                ("numRefs", C.c_int64), ("parent", C.c_void_p), ("nodeBox", C.c_void_p),
+                ("kdBounds", C.c_float * 6), ("sceneBounds", C.c_float * 6)]
+
+
+class btg_options(C.Structure):
+    _fields_ = [("budget", C.c_int), ("primaryBudget", C.c_int), ("capacity", C.c_int64),
+                ("roundsPerCheck", C.c_int), ("profileKernels", C.c_int)]
+
+
+class btg_camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("fwd", C.c_float * 3), ("right", C.c_float * 3), ("down", C.c_float * 3),
+                ("halfW", C.c_float), ("halfH", C.c_float), ("W", C.c_int), ("H", C.c_int)]
+
+
+class btg_area_light(C.Structure):
+    _fields_ = [("c", C.c_float * 3), ("U", C.c_float * 3), ("V", C.c_float * 3)]
+
+
+STAT_FIELDS = ["kdSteps", "leafVisits", "triTests", "hits", "splits", "beams", "rootBeams", "presplitBeams",
+               "mailboxSkips", "fiveSplits", "visibleTris", "rays", "climbs"]
+
+
+class btg_trace_stats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in STAT_FIELDS] + [("droppedArea", C.c_double)]
+
+
+class btg_run_info(C.Structure):
+    _fields_ = [("primarySeconds", C.c_double), ("traceSeconds", C.c_double), ("totalSeconds", C.c_double),
+                ("rounds", C.c_int), ("maxQueue", C.c_int), ("killedRoots", C.c_int), ("retryPasses", C.c_int),
+                ("stats", btg_trace_stats), ("rayStats", btg_trace_stats), ("kernelSeconds", C.c_double * 5),
+                ("kernelLaunches", C.c_int * 5)]
+
+
+POLY_DTYPE = np.dtype([("x", "<f4", (4,)), ("y", "<f4", (4,)), ("n", "<i4"), ("tri", "<i4"), ("area", "<f4")])
+assert POLY_DTYPE.itemsize == 44
+
+# Every entry point include/btg.h declares (checked by the tests).
+EXPORTS = ["btg_device_query", "btg_create", "btg_destroy", "btg_last_error", "btg_get_options", "btg_set_options",
+           "btg_soft", "btg_polys_count", "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point"]
+
+
+def lib() -> C.CDLL:
+    """Loads lib/libbtg.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(LIB_DIR, "libbtg.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run `python -m beamtrace2007_amd.build` first")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.btg_device_query.argtypes = [C.POINTER(btg_device_info)]
+        L.btg_create.argtypes = [C.POINTER(vp), C.POINTER(btg_scene_desc)]
+        L.btg_destroy.argtypes = [vp]
+        L.btg_last_error.argtypes = [vp]
+        L.btg_last_error.restype = C.c_char_p
+        L.btg_get_options.argtypes = [vp, C.POINTER(btg_options)]
+        L.btg_set_options.argtypes = [vp, C.POINTER(btg_options)]
+        L.btg_soft.argtypes = [vp, C.POINTER(btg_camera), C.POINTER(btg_area_light), C.c_int, vp, vp, vp, vp,
+                               C.POINTER(btg_run_info)]
+        L.btg_polys_count.argtypes = [vp]
+        L.btg_polys_count.restype = C.c_int64
+        L.btg_polys_data.argtypes = [vp]
+        L.btg_polys_data.restype = vp
+        L.btg_polys_free.argtypes = [vp]
+        L.btg_primary.argtypes = [vp, C.POINTER(btg_camera), C.c_int, C.c_int, C.c_int, C.POINTER(vp),
+                                  C.POINTER(btg_run_info)]
+        L.btg_point.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.POINTER(C.c_float), C.c_int,
+                                C.POINTER(vp), C.POINTER(btg_run_info)]
+        _lib = L
+    return _lib
+
+
+def device_info() -> dict | None:
+    """gpuAvailable (cuda/gpu_render.h:24): None without a usable device."""
+    info = btg_device_info()
+    if not lib().btg_device_query(C.byref(info)):
+        return None
+    return {"name": info.name.decode(), "arch": info.arch.decode(), "compute_units": info.computeUnits,
+            "wavefront": info.wavefrontSize, "mem_gb": info.memGB}
+
+
+def make_camera(cam14, W: int, H: int) -> btg_camera:
+    c = np.asarray(cam14, dtype=np.float32)
+    bc = btg_camera()
+    for i, name in enumerate(["eye", "fwd", "right", "down"]):
+        getattr(bc, name)[:] = [float(v) for v in c[3 * i:3 * i + 3]]
+    bc.halfW, bc.halfH, bc.W, bc.H = float(c[12]), float(c[13]), int(W), int(H)
+    return bc
+
+
+def info_dict(ri: btg_run_info) -> dict:
+    st = {f: int(getattr(ri.stats, f)) for f in STAT_FIELDS}
+    st["droppedArea"] = float(ri.stats.droppedArea)
+    rs = {f: int(getattr(ri.rayStats, f)) for f in STAT_FIELDS}
+    return {"primary_s": ri.primarySeconds, "trace_s": ri.traceSeconds, "total_s": ri.totalSeconds,
+            "rounds": ri.rounds, "max_queue": ri.maxQueue, "killed_roots": ri.killedRoots,
+            "retry_passes": ri.retryPasses, "stats": st, "ray_stats": rs,
+            "kernel_s": dict(zip(KERNELS, [float(v) for v in ri.kernelSeconds])),
+            "launches": dict(zip(KERNELS, [int(v) for v in ri.kernelLaunches]))}
+
+
+KERNELS = ["admit", "trace", "split", "primary_rays", "other"]
+
+
+def polys_to_dict(arr: np.ndarray) -> dict:
+    return {"x": arr["x"].copy(), "y": arr["y"].copy(), "n": arr["n"].copy(), "tri": arr["tri"].copy(),
+            "area": arr["area"].copy()}
+
+
+def dict_to_polys(p: dict) -> np.ndarray:
+    n = len(p["n"])
+    a = np.zeros(n, POLY_DTYPE)
+    a["x"], a["y"], a["n"], a["tri"], a["area"] = p["x"], p["y"], p["n"], p["tri"], p["area"]
+    return a
+
+
+class GpuRenderer:
+    """The reference's GpuRenderer over one device context. The scene must
+    have its kd-tree built (Scene.build_kd)."""
+
+    def __init__(self, scene: Scene):
+        L = lib()
+        a = scene.arrays()
+        k = scene.kd_arrays()
+        self._keep = (a, k)
+        d = btg_scene_desc()
+        d.pos, d.numVerts = a["pos"].ctypes.data, len(a["pos"])
+        d.tri3, d.triN, d.numTris = a["tris"].ctypes.data, a["triN"].ctypes.data, len(a["tris"])
+        d.nodes, d.numNodes = k["nodes"].ctypes.data, len(k["nodes"])
+        d.triIndices, d.numRefs = k["triIndices"].ctypes.data, len(k["triIndices"])
+        d.parent, d.nodeBox = k["parent"].ctypes.data, k["nodeBox"].ctypes.data
+        d.kdBounds[:] = [float(v) for v in k["bounds"]]
+        d.sceneBounds[:] = [float(v) for v in a["bounds"]]
+        h = C.c_void_p()
+        if L.btg_create(C.byref(h), C.byref(d)) != 0:
+            raise RuntimeError("btg_create: " + L.btg_last_error(None).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().btg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, what: str, rc: int):
+        raise RuntimeError(f"{what} failed ({rc}): " + lib().btg_last_error(self._h).decode())
+
+    @property
+    def options(self) -> dict:
+        o = btg_options()
+        lib().btg_get_options(self._h, C.byref(o))
+        return {"budget": o.budget, "primary_budget": o.primaryBudget, "capacity": o.capacity,
+                "rounds_per_check": o.roundsPerCheck, "profile": bool(o.profileKernels)}
+
+    def set_options(self, **kw) -> None:
+        cur = self.options
+        cur.update(kw)
+        o = btg_options(cur["budget"], cur["primary_budget"], cur["capacity"], cur["rounds_per_check"],
+                        int(cur["profile"]))
+        rc = lib().btg_set_options(self._h, C.byref(o))
+        if rc:
+            self._err("btg_set_options", rc)
+
+    def soft_shadows(self, cam14, W: int, H: int, light9, exact: bool = True, download: bool = True) -> dict:
+        """softShadows (cuda/gpu_render.h:36): tri, vis, E, Eu images (H, W) and run info.
+        download=False leaves the images in device memory (no PCIe transfer)."""
+        n = W * H if download else 0
+        tri, vis = np.empty(n, np.int32), np.empty(n, np.float32)
+        E, Eu = np.empty(n, np.float32), np.empty(n, np.float32)
+        lt = btg_area_light()
+        l9 = np.asarray(light9, np.float32)
+        lt.c[:], lt.U[:], lt.V[:] = [float(v) for v in l9[0:3]], [float(v) for v in l9[3:6]], [float(v) for v in l9[6:9]]
+        ri = btg_run_info()
+        ptr = (lambda a: a.ctypes.data) if download else (lambda a: None)
+        rc = lib().btg_soft(self._h, C.byref(make_camera(cam14, W, H)), C.byref(lt), int(exact), ptr(tri), ptr(vis),
+                            ptr(E), ptr(Eu), C.byref(ri))
+        if rc:
+            self._err("btg_soft", rc)
+        if not download:
+            return {"info": info_dict(ri)}
+        return {"tri": tri.reshape(H, W), "vis": vis.reshape(H, W), "E": E.reshape(H, W), "Eu": Eu.reshape(H, W),
+                "info": info_dict(ri)}
+
+    def _take(self, ph: C.c_void_p) -> np.ndarray:
+        L = lib()
+        n = L.btg_polys_count(ph)
+        out = np.zeros(n, POLY_DTYPE)
+        if n:
+            C.memmove(out.ctypes.data, L.btg_polys_data(ph), n * POLY_DTYPE.itemsize)
+        L.btg_polys_free(ph)
+        return out
+
+    def primary(self, cam14, W: int, H: int, cull: bool = False, tile: int = 16, canonical: bool = True):
+        """primary (cuda/gpu_render.h:41): hit polygons as a structured array, and run info."""
+        ph, ri = C.c_void_p(), btg_run_info()
+        rc = lib().btg_primary(self._h, C.byref(make_camera(cam14, W, H)), int(cull), int(tile), int(canonical),
+                               C.byref(ph), C.byref(ri))
+        if rc:
+            self._err("btg_primary", rc)
+        return self._take(ph), info_dict(ri)
+
+    def point_shadows(self, cam14, W: int, H: int, prim: np.ndarray, light3, canonical: bool = True):
+        """pointShadows (cuda/gpu_render.h:45): shadowed polygons for primary hit polygons."""
+        prim = np.ascontiguousarray(prim, dtype=POLY_DTYPE)
+        lt = (C.c_float * 3)(*[float(v) for v in light3])
+        ph, ri = C.c_void_p(), btg_run_info()
+        rc = lib().btg_point(self._h, C.byref(make_camera(cam14, W, H)), prim.ctypes.data if len(prim) else None,
+                             len(prim), lt, int(canonical), C.byref(ph), C.byref(ri))
+        if rc:
+            self._err("btg_point", rc)
+        return self._take(ph), info_dict(ri)

@@ -1,0 +1,170 @@
+"""Benchmark: exact soft shadows on one MI355X (the plain, single-job form).
+
+    python bench.py [--steps K] [--warmup W] [--workload plant] [--res 512]
+
+One step = one exact soft-shadow frame through the GPU path (GPU primary-ray
+pass + wavefront shadow beams, as the reference times it: pixels / (primary_s
++ trace_s), scripts/make_report.py:171), with the scene and kd-tree resident
+in HBM and the images left in device memory (no PCIe in the timed region).
+Prints one JSON line. Besides the headline value it reports:
+  roofline      the trace kernel's algorithmic bytes (one 12-B kd node per
+                inner step and leaf visit, one 64-B leaf triangle per test,
+                one 4-B parent link per climb) over its HIP-event time,
+                against MI355X's 8 TB/s HBM3E;
+  cpu_baseline  the reference's own CPU build (oracle/_ref, compiled from the
+                reference sources) on a band of rows of the same frame, all
+                host threads (up to 16), its default options (mailbox on);
+  parity        GPU vs the oracle on a band of rows (bit-exact fraction,
+                max |vis diff|) and the error against the double build.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    # name: (scene, default resolution, description)
+    "plant": ("plant", 512, "exact soft shadows, seeded procedural plant (curved stem, 292 bent 3x3-grid leaves "
+                            "2-6 cm, ground quad; 5378 tris), 0.5x0.5 m area light 1 m above, light-scale 1"),
+    "room": ("room", 512, "exact soft shadows, seeded 10x8x3 m room (888 tris), 1x1 m ceiling light"),
+}
+METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request latency"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="plant", choices=sorted(WORKLOADS))
+    ap.add_argument("--res", type=int, default=0)
+    ap.add_argument("--cpu-rows", type=int, default=48, help="rows of the frame the CPU baseline traces")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip per-launch HIP events")
+    return ap.parse_args()
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the tracer shards by independent roots and needs no collectives, see DESIGN.md", file=sys.stderr)
+        return 2
+    from beamtrace2007_amd import gpu
+    from beamtrace2007_amd.host import Scene, camera
+
+    scene_name, res0, desc = WORKLOADS[args.workload]
+    res = args.res or res0
+    W = H = res
+    t0 = time.perf_counter()
+    s = Scene.generate(scene_name)
+    s.build_kd(0.4)
+    build_s = time.perf_counter() - t0
+    sp = s.spec()
+    cam = camera(sp["view"], W, H)
+    light = sp["area_light"]
+    dev = gpu.device_info()
+    if dev is None:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    r = gpu.GpuRenderer(s)
+    r.set_options(profile=not args.no_profile)
+    for _ in range(args.warmup):
+        r.soft_shadows(cam, W, H, light, exact=True, download=False)
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        infos.append(r.soft_shadows(cam, W, H, light, exact=True, download=False)["info"])
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    px = W * H
+    value = px * args.steps / wall
+
+    st = infos[-1]["stats"]
+    b_trace = 12 * (st["kdSteps"] + st["leafVisits"]) + 64 * st["triTests"] + 4 * st["climbs"]
+    b_job = 40 * px + b_trace
+    trace_s = sum(i["kernel_s"]["trace"] for i in infos)
+    trace_launches = sum(i["launches"]["trace"] for i in infos)
+    roofline = None
+    if trace_s > 0:
+        achieved = b_trace * args.steps / trace_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                    "kernel": "traceKernel", "launches_per_step": trace_launches / args.steps,
+                    "avg_launch_ms": round(trace_s / max(1, trace_launches) * 1e3, 4),
+                    "alg_bytes_per_step": int(b_trace)}
+    kernel_ms = {k: round(sum(i["kernel_s"][k] for i in infos) / args.steps * 1e3, 3) for k in gpu.KERNELS}
+
+    # one downloaded frame for the parity band and error against the double build
+    full = r.soft_shadows(cam, W, H, light, exact=True, download=True)
+    r.close()
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 exact integral)", "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": f"{args.workload} {W}x{H}: {desc}", "mode": "plain (one job per call)",
+                   "tris": s.num_tris, "kd_nodes": len(s.kd_arrays()["nodes"]), "exact": True,
+                   "budget": 128},
+        "device": dev["name"], "arch": dev["arch"],
+        "beams_per_s": round(st["beams"] * args.steps / wall, 1),
+        "rounds": infos[-1]["rounds"], "killed_roots": infos[-1]["killed_roots"],
+        "kernel_ms_per_step": kernel_ms, "alg_bytes_per_step": int(b_job),
+        "roofline": roofline, "host_kd_build_s": round(build_s, 3),
+    }
+    out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_and_parity(args, s, sp, cam, W, H, light, full) -> dict:
+    """The CPU baseline and the parity band (test oracles, never the product)."""
+    res = {"cpu_baseline": None, "parity": None}
+    if args.no_cpu:
+        return res
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refpy import Oracle, Ref, ref_available
+
+    rows = max(1, min(args.cpu_rows, H))
+    y0 = (H - rows) // 2
+    y1 = y0 + rows
+    threads = max(1, min(16, os.cpu_count() or 1))
+    a = s.arrays()
+    if ref_available("f"):
+        ref = Ref("f").load(a["pos"], a["tris"])
+        cpu = ref.soft_rows(sp["view"], W, H, light, y0, y1, exact=True, mailbox=True, trail=False, threads=threads)
+        secs = float(cpu["seconds"][0] + cpu["seconds"][1])
+        res["cpu_baseline"] = {"value": round(rows * W / secs, 1), "unit": "pixels/s", "cores": threads,
+                               "kind": "reference",
+                               "sample": f"rows {y0}-{y1 - 1} of the same {W}x{H} frame ({rows * W} px, "
+                                         f"{secs:.2f} s: primary rays + shadow beams, mailbox on, frame stack)"}
+        ref.free()
+    orc = Oracle(a, s.kd_arrays())
+    o = orc.soft(cam, W, H, light, exact=True, threads=threads, rows=(y0, y1))
+    gv, ov = full["vis"][y0:y1], o["vis"][y0:y1]
+    par = {"rows": [y0, y1], "tri_bitexact": bool(np.array_equal(full["tri"][y0:y1], o["tri"][y0:y1])),
+           "vis_bitexact_frac": float(np.mean(gv == ov)),
+           "max_abs_vis_diff": float(np.max(np.abs(gv.astype(np.float64) - ov)))}
+    if ref_available("d"):
+        refd = Ref("d").load(a["pos"], a["tris"])
+        d = refd.soft_rows(sp["view"], W, H, light, y0, y1, exact=True, mailbox=False, trail=True, threads=threads)
+        m = d["vis"] >= 0
+        err = np.abs(gv.astype(np.float64) - d["vis"])[m & (gv >= 0)]
+        par["vis_err_vs_fp64"] = {"max": float(err.max()) if err.size else 0.0,
+                                  "mean": float(err.mean()) if err.size else 0.0,
+                                  "frac_above_1e-5": float(np.mean(err > 1e-5)) if err.size else 0.0}
+        refd.free()
+    res["parity"] = par
+    return res
+
+
+if __name__ == "__main__":
+    sys.exit(main())

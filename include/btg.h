@@ -1,0 +1,169 @@
+/* MI355X beam tracer: the GPU path as a C ABI (libbtg.so, gfx950 HIP).
+ *
+ * Drop-in boundary for the reference's GPU renderer, cuda/gpu_render.h of
+ * joelsalzman/beamtrace2007. Each entry point replaces one member of that
+ * class; a C++ GpuRenderer with the reference's exact signatures is layered
+ * on top in beamtrace2007_amd/shim/gpu_render_hip.cpp.
+ *
+ * Conventions: plain C types only; every function returning int returns 0 on
+ * success and a negative code on error (no abort inside the library), with
+ * the message in btg_last_error(ctx) (ctx may be NULL for errors of
+ * btg_create / btg_device_query). One context per device; calls on one
+ * context are serialised by the caller (as in the reference, gpu_render.h:26).
+ * All host arrays are owned by the caller and only read or written during the
+ * call.
+ */
+#ifndef BTG_H
+#define BTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BTG_OK = 0,
+  BTG_ERR_ARG = -1,     /* invalid argument */
+  BTG_ERR_DEVICE = -2,  /* HIP runtime error, or no device */
+  BTG_ERR_MEMORY = -3,  /* device allocation failed */
+  BTG_ERR_LIMIT = -4    /* a root needs more sub-beams than the queues hold */
+};
+
+typedef struct btg_ctx btg_ctx;
+
+/* gpuAvailable / GpuInfo (cuda/gpu_render.h:17-24). Returns 1 if a device
+ * is usable (and fills *info if given), 0 otherwise. */
+typedef struct {
+  char name[256];
+  char arch[64];
+  int computeUnits;
+  int wavefrontSize;
+  double memGB;
+} btg_device_info;
+int btg_device_query(btg_device_info* info);
+
+/* The scene and kd-tree the tracer reads (SceneView, src/beam/beam_types.h:86-97,
+ * uploaded as in GpuRenderer::GpuRenderer, cuda/gpu_render.cu:489-512).
+ * nodes: 12 bytes each {float split; uint32 data; uint32 offset}
+ * (src/accel/kdtree.h:13-21); nodeBox: 6 floats (lo, hi) per node; parent:
+ * parent index per node (root: ~0u). kdBounds: the kd root box; sceneBounds:
+ * the scene's bounds (sets the soft-shadow apex offset, gpu_render.cu:574).
+ * The library copies everything to device memory; the caller keeps ownership. */
+typedef struct {
+  const float* pos;
+  int64_t numVerts;
+  const uint32_t* tri3;
+  const float* triN;
+  int64_t numTris;
+  const void* nodes;
+  int64_t numNodes;
+  const uint32_t* triIndices;
+  int64_t numRefs;
+  const uint32_t* parent;
+  const float* nodeBox;
+  float kdBounds[6];
+  float sceneBounds[6];
+} btg_scene_desc;
+
+/* GpuRenderer::GpuRenderer / ~GpuRenderer (cuda/gpu_render.h:28-29). */
+int btg_create(btg_ctx** out, const btg_scene_desc* scene);
+void btg_destroy(btg_ctx* ctx);
+const char* btg_last_error(const btg_ctx* ctx);
+
+/* Engine knobs (cuda/gpu_render.h:48-58): budget = live sub-beams per soft /
+ * point root (wfBudget, 128), primaryBudget = per primary tile (4096),
+ * capacity = sub-beam records per queue (wfCapacity, 1 << 23),
+ * roundsPerCheck = rounds launched between completion checks.
+ * Roots over budget are not sent to the CPU: they are traced again on the GPU
+ * with a larger budget. */
+typedef struct {
+  int budget;
+  int primaryBudget;
+  int64_t capacity;
+  int roundsPerCheck;
+  int profileKernels; /* 1: time every launch with HIP events (btg_run_info.kernelSeconds) */
+} btg_options;
+int btg_get_options(const btg_ctx* ctx, btg_options* opt);
+int btg_set_options(btg_ctx* ctx, const btg_options* opt);
+
+/* Camera (src/render/render_types.h:15-39): image plane at distance 1 along
+ * fwd, q in [-halfW, halfW] x [-halfH, halfH]. */
+typedef struct {
+  float eye[3], fwd[3], right[3], down[3];
+  float halfW, halfH;
+  int W, H;
+} btg_camera;
+
+/* AreaLight (src/render/soft_core.h:10-14): centre and full edge vectors. */
+typedef struct {
+  float c[3], U[3], V[3];
+} btg_area_light;
+
+/* TraceStats (src/util/stats.h:10-42). */
+typedef struct {
+  uint64_t kdSteps, leafVisits, triTests, hits, splits, beams, rootBeams, presplitBeams, mailboxSkips, fiveSplits,
+      visibleTris, rays, climbs;
+  double droppedArea;
+} btg_trace_stats;
+
+/* Per-call diagnostics (GpuRenderer::last*, cuda/gpu_render.h:60-64). Times
+ * come from HIP events on the library's stream. */
+typedef struct {
+  double primarySeconds;  /* soft shadows: the primary-ray pass */
+  double traceSeconds;    /* beam kernels (all rounds, all passes) */
+  double totalSeconds;    /* host wall time of the call */
+  int rounds;
+  int maxQueue;
+  int killedRoots;        /* roots that went over budget in the first pass */
+  int retryPasses;        /* extra GPU passes that traced them exactly */
+  btg_trace_stats stats;  /* beam counters (engine-independent) */
+  btg_trace_stats rayStats;
+  /* with profileKernels: summed launch durations and launch counts per kernel
+   * (0 admit, 1 trace, 2 split, 3 primary rays, 4 other) */
+  double kernelSeconds[5];
+  int kernelLaunches[5];
+} btg_run_info;
+
+/* GpuRenderer::softShadows (cuda/gpu_render.h:36): exact soft shadows of a
+ * W x H image. Outputs (W*H each, row-major, caller-allocated; any may be
+ * NULL): tri = primary hit (-1 background), vis = visible fraction of the
+ * light (-1 background, 0 light not visible), E = exact integral of
+ * cos cos / r^2 over the visible light if exact != 0, else area * G0 * vis,
+ * Eu = area * G0. */
+int btg_soft(btg_ctx* ctx, const btg_camera* cam, const btg_area_light* light, int exact, int32_t* tri, float* vis,
+             float* E, float* Eu, btg_run_info* info);
+
+/* A polygon in image-plane coordinates (OutBeam, src/beam/beam_types.h:68-73):
+ * corners past n repeat the last one. 44 bytes. */
+typedef struct {
+  float x[4], y[4];
+  int32_t n;
+  int32_t tri;
+  float area;
+} btg_poly;
+
+typedef struct btg_polys btg_polys;
+int64_t btg_polys_count(const btg_polys* p);
+const btg_poly* btg_polys_data(const btg_polys* p);
+void btg_polys_free(btg_polys* p);
+
+/* GpuRenderer::primary (cuda/gpu_render.h:41): hit beams (tri >= 0) of one
+ * root per tile x tile pixels. canonical != 0 sorts the list into a
+ * canonical order (tri, n, corners, area); otherwise the order is the
+ * device's. *out receives a new list (free with btg_polys_free). */
+int btg_primary(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, int canonical, btg_polys** out,
+                btg_run_info* info);
+
+/* GpuRenderer::pointShadows (cuda/gpu_render.h:45-46): shadowed polygons
+ * (image plane) for the given primary hit polygons and point light; a
+ * receiver facing away from the light yields its whole primary polygon. */
+int btg_point(btg_ctx* ctx, const btg_camera* cam, const btg_poly* prim, int64_t numPrim, const float* light3,
+              int canonical, btg_polys** out, btg_run_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -310,3 +310,13 @@ def canon(p: dict) -> np.ndarray:
     a["area"] = np.asarray(p["area"], np.float32)
     keys = [a["area"]] + [xy[:, k] for k in range(7, -1, -1)] + [a["n"], a["tri"]]
     return a[np.lexsort(keys)]
+
+
+def _oracle_light_vis(self, x3, ns3, tri, light9, offset):
+    G, st = np.zeros(1, np.float32), np.zeros(14)
+    V = self.L.orc_light_vis(self.h, _f(x3), _f(ns3), int(tri), _f(light9), float(offset),
+                             G.ctypes.data_as(fp), st.ctypes.data_as(dp))
+    return float(V), float(G[0]), stats_dict(st)
+
+
+Oracle.light_vis = _oracle_light_vis

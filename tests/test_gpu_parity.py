@@ -1,0 +1,178 @@
+"""GPU path (lib/libbtg.so, gfx950 kernels) against the oracle.
+
+Bar: primary hit ids, split topology (polygon multisets) and point shadows
+bit-exact; soft-shadow visible fraction within 1e-5 absolute (in practice
+bit-exact: the engine sums piece areas exactly); E (exact Lambert integral)
+within 1e-5 relative, since the reference itself sums it in float in an
+order that depends on the traversal; Eu bit-exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_scene
+from refpy import Oracle, canon
+
+pytestmark = pytest.mark.gpu
+
+SCENES = ["analytic", "room", "random_tris", "sphere", "plant"]
+VIS_TOL = 1e-5   # absolute, visible fraction
+E_TOL = 1e-5     # relative, exact integral
+
+_renderers, _oracles = {}, {}
+
+
+def renderer(name):
+    from beamtrace2007_amd.gpu import GpuRenderer
+    if name not in _renderers:
+        s, a, k, sp = load_scene(name)
+        _renderers[name] = GpuRenderer(s)
+    return _renderers[name]
+
+
+def oracle(name):
+    if name not in _oracles:
+        s, a, k, sp = load_scene(name)
+        _oracles[name] = Oracle(a, k)
+    return _oracles[name]
+
+
+def check_soft(g, o):
+    assert np.array_equal(g["tri"], o["tri"])
+    assert np.array_equal(g["Eu"], o["Eu"])
+    dv = np.abs(g["vis"].astype(np.float64) - o["vis"])
+    assert dv.max() <= VIS_TOL
+    assert np.mean(g["vis"] == o["vis"]) > 0.9999
+    rel = np.abs(g["E"].astype(np.float64) - o["E"]) / np.maximum(np.abs(o["E"]), 1e-30)
+    assert np.all((rel <= E_TOL) | (np.abs(o["E"]) < 1e-30))
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_soft_shadows_match_oracle(require_gpu, name):
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene(name)
+    W = H = 80
+    cam = camera(sp["view"], W, H)
+    print(f"soft {name} {W}x{H}", flush=True)
+    g = renderer(name).soft_shadows(cam, W, H, sp["area_light"], exact=True)
+    o = oracle(name).soft(cam, W, H, sp["area_light"], exact=True)
+    check_soft(g, o)
+    if g["info"]["killed_roots"] == 0:  # counters are engine-independent when no root is traced twice
+        for key in ("kdSteps", "leafVisits", "triTests", "hits", "splits", "beams", "rootBeams", "presplitBeams",
+                    "fiveSplits", "climbs"):
+            assert g["info"]["stats"][key] == o["stats"][key], key
+
+
+@pytest.mark.parametrize("name", SCENES)
+@pytest.mark.parametrize("tile", [8, 16])
+def test_primary_polygons_match_oracle(require_gpu, name, tile):
+    from beamtrace2007_amd.gpu import polys_to_dict
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene(name)
+    W, H = 96, 72
+    cam = camera(sp["view"], W, H)
+    print(f"primary {name} tile {tile}", flush=True)
+    gp, info = renderer(name).primary(cam, W, H, tile=tile)
+    op = oracle(name).primary(cam, W, H, tile=tile)
+    assert np.array_equal(canon(polys_to_dict(gp)), canon(op))
+    if info["killed_roots"] == 0:
+        for key in ("kdSteps", "leafVisits", "triTests", "splits", "beams", "fiveSplits", "climbs"):
+            assert info["stats"][key] == op["stats"][key], key
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_point_shadows_and_masks_match_oracle(require_gpu, name):
+    from beamtrace2007_amd.gpu import polys_to_dict
+    from beamtrace2007_amd.host import camera, rasterize
+    s, a, k, sp = load_scene(name)
+    W, H = 96, 72
+    cam = camera(sp["view"], W, H)
+    print(f"point {name}", flush=True)
+    gp, _ = renderer(name).primary(cam, W, H, tile=8)
+    gs, _ = renderer(name).point_shadows(cam, W, H, gp, sp["point_light"])
+    prim = polys_to_dict(gp)
+    os_ = oracle(name).point(cam, W, H, prim, sp["point_light"])
+    assert np.array_equal(canon(polys_to_dict(gs)), canon(os_))
+    # per-pixel ids and shadow mask (polygons rasterized in canonical order)
+    ids_g = rasterize(cam, W, H, prim)
+    ids_o = rasterize(cam, W, H, oracle(name).primary(cam, W, H, tile=8))
+    assert np.array_equal(ids_g, ids_o)
+    assert np.array_equal(rasterize(cam, W, H, polys_to_dict(gs), shadow=True),
+                          rasterize(cam, W, H, os_, shadow=True))
+
+
+def test_odd_image_sizes_and_partial_tiles(require_gpu):
+    from beamtrace2007_amd.gpu import polys_to_dict
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene("room")
+    for W, H in [(37, 23), (1, 1), (9, 130)]:
+        cam = camera(sp["view"], W, H)
+        print(f"odd size {W}x{H}", flush=True)
+        check_soft(renderer("room").soft_shadows(cam, W, H, sp["area_light"]), oracle("room").soft(cam, W, H, sp["area_light"]))
+        gp, _ = renderer("room").primary(cam, W, H, tile=16)
+        assert np.array_equal(canon(polys_to_dict(gp)), canon(oracle("room").primary(cam, W, H, tile=16)))
+
+
+def test_background_and_unlit_pixels(require_gpu):
+    """A camera that sees nothing gives tri = -1 / vis = -1 / E = Eu = 0; a light
+    under the floor leaves every receiver at vis = 0."""
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene("analytic")
+    away = np.array([0, 5, 5, 0, 10, 20, 0, 1, 0, 30], np.float32)
+    cam = camera(away, 24, 24)
+    g = renderer("analytic").soft_shadows(cam, 24, 24, sp["area_light"])
+    assert np.all(g["tri"] == -1) and np.all(g["vis"] == -1) and np.all(g["E"] == 0) and np.all(g["Eu"] == 0)
+    below = np.array([0, -3, 0, 1, 0, 0, 0, 0, 1], np.float32)
+    cam = camera(sp["view"], 32, 32)
+    g = renderer("analytic").soft_shadows(cam, 32, 32, below)
+    o = oracle("analytic").soft(cam, 32, 32, below)
+    check_soft(g, o)
+    assert np.all(g["vis"][g["tri"] >= 0] == 0)
+
+
+def test_tiny_budget_retries_on_the_gpu_and_stays_exact(require_gpu):
+    """Budget 4 and a 4096-record queue force many roots over budget (the
+    reference's tests/cuda case, cuda/test_cuda.cpp:264-302); they are traced
+    again on the GPU with larger budgets and the results stay exact."""
+    from beamtrace2007_amd.gpu import GpuRenderer
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene("plant")
+    r = GpuRenderer(s)
+    r.set_options(budget=4, capacity=4096)
+    W = H = 48
+    cam = camera(sp["view"], W, H)
+    g = r.soft_shadows(cam, W, H, sp["area_light"])
+    assert g["info"]["killed_roots"] > 100 and g["info"]["retry_passes"] >= 1
+    check_soft(g, oracle("plant").soft(cam, W, H, sp["area_light"]))
+    r.set_options(primary_budget=4, budget=4)
+    gp, info = r.primary(cam, W, H, tile=16)
+    assert info["killed_roots"] > 0
+    from beamtrace2007_amd.gpu import polys_to_dict
+    assert np.array_equal(canon(polys_to_dict(gp)), canon(oracle("plant").primary(cam, W, H, tile=16)))
+    r.close()
+
+
+def test_repeated_runs_are_bitwise_identical(require_gpu):
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene("plant")
+    cam = camera(sp["view"], 64, 64)
+    x = renderer("plant").soft_shadows(cam, 64, 64, sp["area_light"])
+    y = renderer("plant").soft_shadows(cam, 64, 64, sp["area_light"])
+    for key in ("tri", "vis", "E", "Eu"):
+        assert np.array_equal(x[key], y[key])
+
+
+def test_invalid_arguments_are_errors_not_crashes(require_gpu):
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene("room")
+    cam = camera(sp["view"], 16, 16)
+    r = renderer("room")
+    with pytest.raises(RuntimeError):
+        r.primary(cam, 16, 16, tile=0)
+    with pytest.raises(RuntimeError):
+        r.soft_shadows(cam, 0, 16, sp["area_light"])
+    bad = np.zeros(1, dtype=__import__("beamtrace2007_amd.gpu", fromlist=["POLY_DTYPE"]).POLY_DTYPE)
+    bad["tri"] = -1
+    with pytest.raises(RuntimeError):
+        r.point_shadows(cam, 16, 16, bad, sp["point_light"])
+    empty, _ = r.point_shadows(cam, 16, 16, bad[:0], sp["point_light"])
+    assert len(empty) == 0
