@@ -45,11 +45,13 @@ def build_host() -> str:
     return out
 
 
-def build_device() -> str:
+def build_device(out: str | None = None, defines: tuple[str, ...] = ()) -> str:
+    """libbtg.so; `defines` builds an instrumented variant (e.g. BTG_PHASE_CYCLES) at `out`."""
     os.makedirs(LIB, exist_ok=True)
-    out = os.path.join(LIB, "libbtg.so")
+    out = out or os.path.join(LIB, "libbtg.so")
     srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
-    _run([HIPCC, *DEVICE_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", *srcs])
+    _run([HIPCC, *DEVICE_FLAGS, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp",
+          *srcs])
     os.replace(out + ".tmp", out)
     return out
 

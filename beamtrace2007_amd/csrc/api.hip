@@ -74,6 +74,7 @@ void fillInfo(const btg::RunInfo& r, double wall, btg_run_info* info) {
     info->kernelSeconds[k] = r.kernelMs[k] * 1e-3;
     info->kernelLaunches[k] = r.launches[k];
   }
+  for (int k = 0; k < 4; ++k) info->phaseCycles[k] = r.phaseCycles[k];
 }
 
 double now() {
@@ -169,13 +170,14 @@ int btg_get_options(const btg_ctx* ctx, btg_options* o) {
   o->capacity = x.capacity;
   o->roundsPerCheck = x.roundsPerCheck;
   o->profileKernels = x.profile ? 1 : 0;
+  o->leafCap = x.leafCap;
   return BTG_OK;
 }
 
 int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   if (!ctx || !o) return BTG_ERR_ARG;
   if (o->budget < 1 || o->primaryBudget < 1 || o->capacity < 1024 || o->capacity > (int64_t(1) << 31) ||
-      o->roundsPerCheck < 1)
+      o->roundsPerCheck < 1 || o->leafCap < 0)
     return failWith(ctx, BTG_ERR_ARG, "invalid options");
   btg::Options& x = btg::engineOptions(ctx->engine);
   x.budget = o->budget;
@@ -183,6 +185,7 @@ int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   x.capacity = o->capacity;
   x.roundsPerCheck = o->roundsPerCheck;
   x.profile = o->profileKernels != 0;
+  x.leafCap = o->leafCap;
   return BTG_OK;
 }
 

@@ -55,13 +55,15 @@ struct RunInfo {
   double dropped = 0;
   double kernelMs[5] = {};
   int launches[5] = {};
+  uint64_t phaseCycles[4] = {};  // trace loop: refill, traverse, test, append (BTG_PHASE_CYCLES builds)
 };
 
 struct Options {
-  int budget = 128;
-  int primaryBudget = 4096;
-  int64_t capacity = int64_t(1) << 24;
+  int budget = 1024;
+  int primaryBudget = 16384;
+  int64_t capacity = int64_t(1) << 25;
   int roundsPerCheck = 8;
+  int leafCap = 8;
   bool profile = false;
 };
 

@@ -116,6 +116,7 @@ struct Params {
   unsigned long long* missFx;
   unsigned long long* gFx;
   unsigned long long* stats;
+  unsigned long long* cycles;  // trace-loop phase clocks (BTG_PHASE_CYCLES builds)
   double* dropped;
   WState* st;
   int W, H;
@@ -133,6 +134,7 @@ struct Params {
   float* E;
   float* Eu;
   int tile, tilesW, cull;
+  int leafCap;  // leaves a sub-beam may visit per trace launch before it yields (0: no cap)
   F3 pointLight;
   const OutPoly* prim;
   OutPoly* out;
@@ -385,40 +387,67 @@ __device__ __forceinline__ void storeGeom(const Beam& b, Rec& r) {
 
 // Triangle setup (computeTriInfo + triSetup, beam_core.h:251-286): edge
 // planes through the apex in canonical vertex order, plane coefficients.
+// The trace computes the pieces lazily - each edge line when its test is
+// reached, the plane only when all edges pass - with exactly the operations
+// of the full setup, so every value it uses is bit-identical.
 struct TriS {
   int tri;
   bool ok;
+  float s;
+  uint32_t vi[3];
   Ln edge[3];
   float A[3], C;
-  F3 r[3], n;
+  F3 r[3], n, m0;
 };
 
-__device__ TriS triSetup(const Ctx& c, const LeafTri& ref) {
-  TriS ti;
+// Vertices relative to the apex, first edge normal, orientation and the
+// degenerate / back-face rejection.
+__device__ __forceinline__ void triBegin(const Ctx& c, const LeafTri& ref, TriS& ti) {
   const F3& O = c.P.O;
   ti.r[0] = f3(ref.v[0], ref.v[1], ref.v[2]) - O;
   ti.r[1] = f3(ref.v[3], ref.v[4], ref.v[5]) - O;
   ti.r[2] = f3(ref.v[6], ref.v[7], ref.v[8]) - O;
   ti.n = f3(ref.n[0], ref.n[1], ref.n[2]);
+  ti.vi[0] = ref.vi[0];
+  ti.vi[1] = ref.vi[1];
+  ti.vi[2] = ref.vi[2];
   ti.tri = ref.tri;
   ti.ok = false;
-  const F3 m0 = ref.vi[0] < ref.vi[1] ? cross(ti.r[0], ti.r[1]) : -cross(ti.r[1], ti.r[0]);
-  const F3 m1 = ref.vi[1] < ref.vi[2] ? cross(ti.r[1], ti.r[2]) : -cross(ti.r[2], ti.r[1]);
-  const F3 m2 = ref.vi[2] < ref.vi[0] ? cross(ti.r[2], ti.r[0]) : -cross(ti.r[0], ti.r[2]);
-  const float D = dot(ti.r[2], m0);
-  const float scale = length(m0) * length(ti.r[2]);
-  if (!(selAbs(D) > kDegRel * scale)) return ti;
-  if ((c.flags & fCull) && D > 0) return ti;
-  const float s = D > 0 ? 1.0f : -1.0f;
+  ti.m0 = ref.vi[0] < ref.vi[1] ? cross(ti.r[0], ti.r[1]) : -cross(ti.r[1], ti.r[0]);
+  const float D = dot(ti.r[2], ti.m0);
+  const float scale = length(ti.m0) * length(ti.r[2]);
+  if (!(selAbs(D) > kDegRel * scale)) return;  // apex in the triangle's plane
+  if ((c.flags & fCull) && D > 0) return;      // back-facing
+  ti.s = D > 0 ? 1.0f : -1.0f;
+  ti.ok = true;
+}
+
+// Edge line k (k = 0: v0 v1, 1: v1 v2, 2: v2 v0).
+template <int k>
+__device__ __forceinline__ Ln triEdge(const Ctx& c, const TriS& ti) {
+  constexpr int i = k, j = (k + 1) % 3;
+  const F3 m = k == 0 ? ti.m0 : (ti.vi[i] < ti.vi[j] ? cross(ti.r[i], ti.r[j]) : -cross(ti.r[j], ti.r[i]));
   const Plane& P = c.P;
-  ti.edge[0] = makeLine(s * dot(m0, P.u), s * dot(m0, P.v), s * dot(m0, P.d), c.ext);
-  ti.edge[1] = makeLine(s * dot(m1, P.u), s * dot(m1, P.v), s * dot(m1, P.d), c.ext);
-  ti.edge[2] = makeLine(s * dot(m2, P.u), s * dot(m2, P.v), s * dot(m2, P.d), c.ext);
+  return makeLine(ti.s * dot(m, P.u), ti.s * dot(m, P.v), ti.s * dot(m, P.d), c.ext);
+}
+
+__device__ __forceinline__ void triPlane(const Ctx& c, TriS& ti) {
+  const Plane& P = c.P;
   ti.A[0] = dot(ti.n, P.u);
   ti.A[1] = dot(ti.n, P.v);
   ti.A[2] = dot(ti.n, P.d);
   ti.C = dot(ti.n, ti.r[0]);
-  ti.ok = true;
+}
+
+// The complete setup (the split kernel needs every line).
+__device__ __forceinline__ TriS triSetup(const Ctx& c, const LeafTri& ref) {
+  TriS ti;
+  triBegin(c, ref, ti);
+  if (!ti.ok) return ti;
+  ti.edge[0] = triEdge<0>(c, ti);
+  ti.edge[1] = triEdge<1>(c, ti);
+  ti.edge[2] = triEdge<2>(c, ti);
+  triPlane(c, ti);
   return ti;
 }
 
@@ -510,6 +539,37 @@ __device__ int classifyTri(const Ctx& c, const Work& w, const TriS& ti, Ln* L, i
     allIn = allIn && r == 0;
     if (L) L[nl] = cl;
     ++nl;
+  }
+  if (allIn) return kAllIn;
+  if (separated(c, B, ti)) return kKeep;
+  return kClip;
+}
+
+// classifyTri for the trace kernel with the lazy setup: the same decisions as
+// classifyTri (beam_core.h:374-409), computing each line only when reached.
+__device__ int classifyLazy(const Ctx& c, const Work& w, TriS& ti) {
+  const Beam& B = w.b;
+  bool allIn = true;
+  int r = lineTest(c, B, triEdge<0>(c, ti));
+  if (r < 0) return kKeep;
+  allIn = r == 0;
+  r = lineTest(c, B, triEdge<1>(c, ti));
+  if (r < 0) return kKeep;
+  allIn = allIn && r == 0;
+  r = lineTest(c, B, triEdge<2>(c, ti));
+  if (r < 0) return kKeep;
+  allIn = allIn && r == 0;
+  triPlane(c, ti);
+  if (!c.anyHit() && B.hit >= 0) {
+    r = lineTest(c, B, closerLine(c, ti, w.hitA, w.hitC, w.hitN, B.hit));
+    if (r < 0) return kKeep;
+    allIn = allIn && r == 0;
+  }
+  if (c.farIsPlane()) {
+    const float farA[3] = {0, 0, c.farCq};
+    r = lineTest(c, B, closerLine(c, ti, farA, c.farCq, c.P.n, -1));
+    if (r < 0) return kKeep;
+    allIn = allIn && r == 0;
   }
   if (allIn) return kAllIn;
   if (separated(c, B, ti)) return kKeep;
@@ -753,12 +813,7 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
       ss.phase = kTest;
     }
     if (ss.phase == kTest) {
-      const KdNodeD leaf = sv.nodes[r.node];
-      const uint32_t count = leaf.data >> 2;
-      uint32_t i = uint32_t(r.cursor);
-      while (i < count && sv.refs[leaf.offset + i].tri == c.exclude) ++i;
-      r.cursor = int32_t(i);
-      if (i < count) return kContinue;
+      if (uint32_t(r.cursor) < (sv.nodes[r.node].data >> 2)) return kContinue;
       ss.phase = kClimb;
     }
     for (;;) {
@@ -790,19 +845,22 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
   }
 }
 
-// wfTestOne (wavefront_core.h:236-268).
+// wfTestOne (wavefront_core.h:236-268). The excluded triangle (the shading
+// point's own) is skipped here, uncounted, as the reference skips it before
+// testing (wavefront_core.h:198, 276).
 __device__ int testOne(const Params& p, const Ctx& c, Rec& r, Work& w, Cnt& cnt) {
   const KdNodeD leaf = p.sv.nodes[r.node];
   const uint32_t i = uint32_t(r.cursor);
   r.cursor = int32_t(i + 1);
   const LeafTri ref = p.sv.refs[leaf.offset + i];
-  const TriS ti = triSetup(c, ref);
+  if (ref.tri == c.exclude) return kContinue;
+  TriS ti;
+  triBegin(c, ref, ti);
   if (!ti.ok) return kContinue;
   cnt.v[sTests]++;
   const int t = ti.tri;
   if (w.b.hit == t) return kContinue;
-  int nl;
-  const int res = classifyTri(c, w, ti, nullptr, nl);
+  const int res = classifyLazy(c, w, ti);
   if (res == kKeep) return kContinue;
   if (res == kAllIn) {
     if (c.anyHit()) {
@@ -823,21 +881,42 @@ __device__ int testOne(const Params& p, const Ctx& c, Rec& r, Work& w, Cnt& cnt)
 }
 
 // wfMoreInLeaf (wavefront_core.h:272-279).
-__device__ __forceinline__ bool moreInLeaf(const Params& p, const Ctx& c, Rec& r) {
-  const KdNodeD leaf = p.sv.nodes[r.node];
-  const uint32_t count = leaf.data >> 2;
-  uint32_t i = uint32_t(r.cursor);
-  while (i < count && p.sv.refs[leaf.offset + i].tri == c.exclude) ++i;
-  r.cursor = int32_t(i);
-  return i < count;
+__device__ __forceinline__ bool moreInLeaf(const Params& p, const Rec& r) {
+  return uint32_t(r.cursor) < (p.sv.nodes[r.node].data >> 2);
 }
+
+// Phase timing of the trace loop (build with -DBTG_PHASE_CYCLES): wave clock
+// cycles spent refilling, traversing (descend / climb), testing triangles and
+// appending, summed into p.cycles[0..3].
+#ifdef BTG_PHASE_CYCLES
+#define PHASE_INIT()                      \
+  uint64_t phT[4] = {0, 0, 0, 0};         \
+  uint64_t phPrev = __builtin_amdgcn_s_memtime()
+#define PHASE_MARK(k)                                  \
+  do {                                                 \
+    const uint64_t t_ = __builtin_amdgcn_s_memtime();  \
+    phT[k] += t_ - phPrev;                             \
+    phPrev = t_;                                       \
+  } while (0)
+#define PHASE_FLUSH()                                              \
+  do {                                                             \
+    if (lane() == 0 && p.cycles)                                   \
+      for (int k_ = 0; k_ < 4; ++k_) atomicAdd(&p.cycles[k_], phT[k_]); \
+  } while (0)
+#else
+#define PHASE_INIT()
+#define PHASE_MARK(k)
+#define PHASE_FLUSH()
+#endif
 
 __global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
   Cnt cnt;
   cnt.zero();
+  PHASE_INIT();
   const int n = int(min((unsigned long long)p.st->count[cur], p.capacity));
   const Rec* q = p.tq[cur];
   bool active = false, exhausted = false;
+  int leaves = 0;
   Rec r;
   Work w;
   Ctx c;
@@ -866,6 +945,7 @@ __global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
               ss.phase = kTest;
             }
             active = true;
+            leaves = 0;
           }
         }
       }
@@ -874,15 +954,38 @@ __global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
       if (exhausted) break;
       continue;
     }
+    PHASE_MARK(0);
     int res = kContinue;
+    if (active) res = advance(p, c, r, w, ss, cnt);
+    PHASE_MARK(1);
     if (active) {
-      res = advance(p, c, r, w, ss, cnt);
       while (res == kContinue) {
         res = testOne(p, c, r, w, cnt);
-        if (res != kContinue || !moreInLeaf(p, c, r)) break;
+        if (res != kContinue || !moreInLeaf(p, r)) break;
       }
+    }
+    PHASE_MARK(2);
+    if (active) {
       if (res == kFinished) liveSub(p, r.root, 1);
       if (res != kContinue) active = false;
+    }
+    // Straggler cap: a sub-beam that has visited leafCap leaves this launch
+    // continues in the next round's queue (its record is a full continuation).
+    const bool yield = active && p.leafCap > 0 && ++leaves >= p.leafCap;
+    const uint64_t yields = __ballot(yield);
+    if (yields) {
+      int base = 0;
+      if (lane() == 0) base = atomicAdd(&p.st->count[cur ^ 1], __popcll(yields));
+      base = __shfl(base, 0, kWave);
+      if (yield) {
+        storeGeom(w.b, r);
+        const int slot = base + rankBelow(yields);
+        if ((unsigned long long)slot < p.capacity)
+          p.tq[cur ^ 1][slot] = r;
+        else
+          p.st->overflow = 1;
+        active = false;
+      }
     }
     const uint64_t splits = __ballot(res == kSplit);
     if (splits) {
@@ -894,8 +997,10 @@ __global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
         p.sq[base + rankBelow(splits)] = r;
       }
     }
+    PHASE_MARK(3);
   }
   flushCnt(p, cnt);
+  PHASE_FLUSH();
 }
 
 // ------------------------------------------------------------- the split
@@ -1593,8 +1698,9 @@ class Engine {
   void* sceneMem[5] = {};
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
-  DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry;
+  DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry, cycles;
   WState* hostState = nullptr;  // pinned
+  uint64_t phaseCycles[4] = {};
   // per-launch timing (Options::profile): event pairs and the kernel kind
   std::vector<hipEvent_t> evPool;
   std::vector<int> evKind;
@@ -1642,7 +1748,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixN, &vis, &E, &Eu, &prim, &retry};
+                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -1667,7 +1773,9 @@ int residentGrid(const Engine& e, K kernel) {
 
 // Prepares engine buffers for R roots and the common parameters.
 int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) {
-  const size_t cap = size_t(e.opt.capacity);
+  // queue records: the configured capacity, but no more than the roots can hold
+  const size_t cap = size_t(std::min<long long>(e.opt.capacity, std::max<long long>(
+                                                     int64_t(1) << 16, (long long)R * (long long)budget)));
   int rc;
   if ((rc = e.ensure(e.tq0, cap * sizeof(Rec), err)) || (rc = e.ensure(e.tq1, cap * sizeof(Rec), err)) ||
       (rc = e.ensure(e.sq, cap * sizeof(Rec), err)) || (rc = e.ensure(e.roots, size_t(R) * sizeof(RootQ), err)) ||
@@ -1675,6 +1783,7 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
       (rc = e.ensure(e.epoch, size_t(R) * 4, err)) || (rc = e.ensure(e.missFx, size_t(R) * 8, err)) ||
       (rc = e.ensure(e.gFx, size_t(R) * 8, err)) || (rc = e.ensure(e.stats, sizeof(uint64_t) * sNum, err)) ||
       (rc = e.ensure(e.dropped, sizeof(double), err)) || (rc = e.ensure(e.st, sizeof(WState), err)) ||
+      (rc = e.ensure(e.cycles, 4 * sizeof(uint64_t), err)) ||
       (rc = e.ensure(e.list, size_t(R) * 4, err)) || (rc = e.ensure(e.listCount, 4, err)))
     return rc;
   p = Params();
@@ -1695,8 +1804,10 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
   p.missFx = static_cast<unsigned long long*>(e.missFx.ptr);
   p.gFx = static_cast<unsigned long long*>(e.gFx.ptr);
   p.stats = static_cast<unsigned long long*>(e.stats.ptr);
+  p.cycles = static_cast<unsigned long long*>(e.cycles.ptr);
   p.dropped = static_cast<double*>(e.dropped.ptr);
   p.st = static_cast<WState*>(e.st.ptr);
+  p.leafCap = e.opt.leafCap;
   return 0;
 }
 
@@ -1756,6 +1867,7 @@ int runPass(Engine& e, Params& p, RunInfo& info, std::string& err) {
 int readStats(Engine& e, uint64_t* dst, double* dropped, std::string& err) {
   uint64_t s[sNum];
   double d = 0;
+  BTG_TRY(hipMemcpyAsync(e.phaseCycles, e.cycles.ptr, sizeof(e.phaseCycles), hipMemcpyDeviceToHost, e.stream));
   BTG_TRY(hipMemcpyAsync(s, e.stats.ptr, sizeof(s), hipMemcpyDeviceToHost, e.stream));
   BTG_TRY(hipMemcpyAsync(&d, e.dropped.ptr, sizeof(d), hipMemcpyDeviceToHost, e.stream));
   BTG_TRY(hipStreamSynchronize(e.stream));
@@ -1768,6 +1880,7 @@ int zeroStats(Engine& e, std::string& err) {
   const uint64_t z[sNum + 1] = {};
   BTG_TRY(hipMemcpyAsync(e.stats.ptr, z, sizeof(uint64_t) * sNum, hipMemcpyHostToDevice, e.stream));
   BTG_TRY(hipMemcpyAsync(e.dropped.ptr, z, sizeof(double), hipMemcpyHostToDevice, e.stream));
+  BTG_TRY(hipMemcpyAsync(e.cycles.ptr, z, 4 * sizeof(uint64_t), hipMemcpyHostToDevice, e.stream));
   return 0;
 }
 
@@ -1790,11 +1903,11 @@ int runAll(Engine& e, Params& p, int baseBudget, RunInfo& info, Finalize finaliz
     BTG_TRY(hipMemcpyAsync(&nk, e.listCount.ptr, 4, hipMemcpyDeviceToHost, e.stream));
     BTG_TRY(hipStreamSynchronize(e.stream));
     if (nk == 0) break;
-    if ((long long)budget >= e.opt.capacity) {
-      err = "a root needs more sub-beams than the queue capacity (" + std::to_string(e.opt.capacity) + ")";
+    if ((unsigned long long)budget >= p.capacity) {
+      err = "a root needs more sub-beams than the queue capacity (" + std::to_string(p.capacity) + ")";
       return BTG_ERR_LIMIT_CODE;
     }
-    budget = int(std::min<long long>((long long)budget * 32, e.opt.capacity));
+    budget = int(std::min<unsigned long long>((unsigned long long)budget * 32, p.capacity));
     // The retry list must not alias the list collectKilled writes next: copy it.
     int rc2;
     Params q = p;
@@ -1963,6 +2076,7 @@ int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, float offset, 
   info.primaryMs += ms0;
   info.traceMs += ms1;
   if ((rc = readStats(*e, info.stats, &info.dropped, err))) return rc;
+  for (int k = 0; k < 4; ++k) info.phaseCycles[k] = e->phaseCycles[k];
   e->collectLaunches(info);
   if (tri) BTG_TRY(hipMemcpyAsync(tri, e->pixTri.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
   if (vis) BTG_TRY(hipMemcpyAsync(vis, e->vis.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));

@@ -35,7 +35,7 @@ class btg_scene_desc(C.Structure):
 
 class btg_options(C.Structure):
     _fields_ = [("budget", C.c_int), ("primaryBudget", C.c_int), ("capacity", C.c_int64),
-                ("roundsPerCheck", C.c_int), ("profileKernels", C.c_int)]
+                ("roundsPerCheck", C.c_int), ("profileKernels", C.c_int), ("leafCap", C.c_int)]
 
 
 class btg_camera(C.Structure):
@@ -59,7 +59,7 @@ class btg_run_info(C.Structure):
     _fields_ = [("primarySeconds", C.c_double), ("traceSeconds", C.c_double), ("totalSeconds", C.c_double),
                 ("rounds", C.c_int), ("maxQueue", C.c_int), ("killedRoots", C.c_int), ("retryPasses", C.c_int),
                 ("stats", btg_trace_stats), ("rayStats", btg_trace_stats), ("kernelSeconds", C.c_double * 5),
-                ("kernelLaunches", C.c_int * 5)]
+                ("kernelLaunches", C.c_int * 5), ("phaseCycles", C.c_uint64 * 4)]
 
 
 POLY_DTYPE = np.dtype([("x", "<f4", (4,)), ("y", "<f4", (4,)), ("n", "<i4"), ("tri", "<i4"), ("area", "<f4")])
@@ -74,7 +74,7 @@ def lib() -> C.CDLL:
     """Loads lib/libbtg.so; raises if it was not built."""
     global _lib
     if _lib is None:
-        path = os.path.join(LIB_DIR, "libbtg.so")
+        path = os.environ.get("BTG_LIBRARY") or os.path.join(LIB_DIR, "libbtg.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is missing: run `python -m beamtrace2007_amd.build` first")
         L = C.CDLL(path)
@@ -127,7 +127,8 @@ def info_dict(ri: btg_run_info) -> dict:
             "rounds": ri.rounds, "max_queue": ri.maxQueue, "killed_roots": ri.killedRoots,
             "retry_passes": ri.retryPasses, "stats": st, "ray_stats": rs,
             "kernel_s": dict(zip(KERNELS, [float(v) for v in ri.kernelSeconds])),
-            "launches": dict(zip(KERNELS, [int(v) for v in ri.kernelLaunches]))}
+            "launches": dict(zip(KERNELS, [int(v) for v in ri.kernelLaunches])),
+            "phase_cycles": dict(zip(["refill", "traverse", "test", "append"], [int(v) for v in ri.phaseCycles]))}
 
 
 KERNELS = ["admit", "trace", "split", "primary_rays", "other"]
@@ -186,13 +187,13 @@ class GpuRenderer:
         o = btg_options()
         lib().btg_get_options(self._h, C.byref(o))
         return {"budget": o.budget, "primary_budget": o.primaryBudget, "capacity": o.capacity,
-                "rounds_per_check": o.roundsPerCheck, "profile": bool(o.profileKernels)}
+                "rounds_per_check": o.roundsPerCheck, "profile": bool(o.profileKernels), "leaf_cap": o.leafCap}
 
     def set_options(self, **kw) -> None:
         cur = self.options
         cur.update(kw)
         o = btg_options(cur["budget"], cur["primary_budget"], cur["capacity"], cur["rounds_per_check"],
-                        int(cur["profile"]))
+                        int(cur["profile"]), int(cur["leaf_cap"]))
         rc = lib().btg_set_options(self._h, C.byref(o))
         if rc:
             self._err("btg_set_options", rc)

@@ -47,9 +47,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plant", choices=sorted(WORKLOADS))
     ap.add_argument("--res", type=int, default=0)
-    ap.add_argument("--cpu-rows", type=int, default=48, help="rows of the frame the CPU baseline traces")
+    ap.add_argument("--cpu-stride", type=int, default=8, help="the CPU baseline traces every k-th row of the frame")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-launch HIP events")
+    ap.add_argument("--budget", type=int, default=0, help="live sub-beams per root (0: library default)")
+    ap.add_argument("--capacity", type=int, default=0, help="sub-beam records per queue (0: library default)")
+    ap.add_argument("--leaf-cap", type=int, default=-1, help="leaves per sub-beam per trace launch (-1: default)")
     return ap.parse_args()
 
 
@@ -77,7 +80,15 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
     r = gpu.GpuRenderer(s)
-    r.set_options(profile=not args.no_profile)
+    kw = {"profile": not args.no_profile}
+    if args.budget > 0:
+        kw["budget"] = args.budget
+    if args.capacity > 0:
+        kw["capacity"] = args.capacity
+    if args.leaf_cap >= 0:
+        kw["leaf_cap"] = args.leaf_cap
+    r.set_options(**kw)
+    opts = r.options
     for _ in range(args.warmup):
         r.soft_shadows(cam, W, H, light, exact=True, download=False)
     infos = []
@@ -113,11 +124,12 @@ def main() -> int:
         "vs_baseline": None, "dtype": "f32 (f64 exact integral)", "data": "synthetic (seeded procedural scene)",
         "config": {"workload": f"{args.workload} {W}x{H}: {desc}", "mode": "plain (one job per call)",
                    "tris": s.num_tris, "kd_nodes": len(s.kd_arrays()["nodes"]), "exact": True,
-                   "budget": 128},
+                   "budget": opts["budget"], "capacity": opts["capacity"], "leaf_cap": opts["leaf_cap"]},
         "device": dev["name"], "arch": dev["arch"],
         "beams_per_s": round(st["beams"] * args.steps / wall, 1),
         "rounds": infos[-1]["rounds"], "killed_roots": infos[-1]["killed_roots"],
         "kernel_ms_per_step": kernel_ms, "alg_bytes_per_step": int(b_job),
+        "trace_phase_cycles": infos[-1]["phase_cycles"],
         "roofline": roofline, "host_kd_build_s": round(build_s, 3),
     }
     out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full))
@@ -133,29 +145,33 @@ def cpu_and_parity(args, s, sp, cam, W, H, light, full) -> dict:
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from refpy import Oracle, Ref, ref_available
 
-    rows = max(1, min(args.cpu_rows, H))
-    y0 = (H - rows) // 2
-    y1 = y0 + rows
+    stride = max(1, args.cpu_stride)
+    y0, y1 = 0, H
+    rows = (H + stride - 1) // stride
     threads = max(1, min(16, os.cpu_count() or 1))
     a = s.arrays()
     if ref_available("f"):
         ref = Ref("f").load(a["pos"], a["tris"])
-        cpu = ref.soft_rows(sp["view"], W, H, light, y0, y1, exact=True, mailbox=True, trail=False, threads=threads)
+        cpu = ref.soft_rows(sp["view"], W, H, light, y0, y1, exact=True, mailbox=True, trail=False, threads=threads,
+                            stride=stride)
         secs = float(cpu["seconds"][0] + cpu["seconds"][1])
         res["cpu_baseline"] = {"value": round(rows * W / secs, 1), "unit": "pixels/s", "cores": threads,
                                "kind": "reference",
-                               "sample": f"rows {y0}-{y1 - 1} of the same {W}x{H} frame ({rows * W} px, "
+                               "sample": f"every {stride}th row of the same {W}x{H} frame ({rows * W} px, "
                                          f"{secs:.2f} s: primary rays + shadow beams, mailbox on, frame stack)"}
         ref.free()
+    # parity band: a few rows through the plant (the oracle is a scalar port, kept short)
+    py0 = H // 2 + H // 8
+    py1 = min(H, py0 + max(1, H // 32))
     orc = Oracle(a, s.kd_arrays())
-    o = orc.soft(cam, W, H, light, exact=True, threads=threads, rows=(y0, y1))
-    gv, ov = full["vis"][y0:y1], o["vis"][y0:y1]
-    par = {"rows": [y0, y1], "tri_bitexact": bool(np.array_equal(full["tri"][y0:y1], o["tri"][y0:y1])),
+    o = orc.soft(cam, W, H, light, exact=True, threads=threads, rows=(py0, py1))
+    gv, ov = full["vis"][py0:py1], o["vis"][py0:py1]
+    par = {"rows": [py0, py1], "tri_bitexact": bool(np.array_equal(full["tri"][py0:py1], o["tri"][py0:py1])),
            "vis_bitexact_frac": float(np.mean(gv == ov)),
            "max_abs_vis_diff": float(np.max(np.abs(gv.astype(np.float64) - ov)))}
     if ref_available("d"):
         refd = Ref("d").load(a["pos"], a["tris"])
-        d = refd.soft_rows(sp["view"], W, H, light, y0, y1, exact=True, mailbox=False, trail=True, threads=threads)
+        d = refd.soft_rows(sp["view"], W, H, light, py0, py1, exact=True, mailbox=False, trail=True, threads=threads)
         m = d["vis"] >= 0
         err = np.abs(gv.astype(np.float64) - d["vis"])[m & (gv >= 0)]
         par["vis_err_vs_fp64"] = {"max": float(err.max()) if err.size else 0.0,

@@ -84,6 +84,7 @@ typedef struct {
   int64_t capacity;
   int roundsPerCheck;
   int profileKernels; /* 1: time every launch with HIP events (btg_run_info.kernelSeconds) */
+  int leafCap;        /* leaves a sub-beam visits per trace launch before it yields to the next round (0: no cap) */
 } btg_options;
 int btg_get_options(const btg_ctx* ctx, btg_options* opt);
 int btg_set_options(btg_ctx* ctx, const btg_options* opt);
@@ -124,6 +125,9 @@ typedef struct {
    * (0 admit, 1 trace, 2 split, 3 primary rays, 4 other) */
   double kernelSeconds[5];
   int kernelLaunches[5];
+  /* builds with -DBTG_PHASE_CYCLES: trace-loop clocks spent refilling,
+   * traversing, testing triangles and appending (soft shadows) */
+  uint64_t phaseCycles[4];
 } btg_run_info;
 
 /* GpuRenderer::softShadows (cuda/gpu_render.h:36): exact soft shadows of a

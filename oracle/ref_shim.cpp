@@ -188,16 +188,18 @@ void ref_soft(void* sh, void* kh, const double* cam, int W, int H, const double*
 }
 
 // The per-pixel work of softShadows (render.cpp:351-421) restricted to image
-// rows [y0, y1): the bounded CPU-baseline sample. Same calls, same threading.
+// rows y0, y0 + stride, ... < y1: the bounded CPU-baseline sample (a strided
+// sample covers every region of the frame). Same calls, same threading.
 void ref_soft_rows(void* sh, void* kh, const double* cam, int W, int H, const double* light, int exact, int mailbox,
-                   int trail, int threads, int y0, int y1, float* vis, double* stats, double* seconds) {
+                   int trail, int threads, int y0, int y1, int stride, float* vis, double* stats, double* seconds) {
   const Scene& s = *static_cast<Scene*>(sh);
   const KdTree& tree = *static_cast<KdTree*>(kh);
   Camera c = makeCam(cam, W, H);
   AreaLight L = makeLight(light);
   const Real offset = length(s.bounds.diag()) * kEpsRel;
   RayTracer rt(s, tree);
-  const int rows = y1 - y0;
+  stride = std::max(1, stride);
+  const int rows = (y1 - y0 + stride - 1) / stride;
   std::vector<int> triv(size_t(W) * size_t(rows), -1);
   std::vector<Vec3> X(triv.size()), N(triv.size());
   threads = std::max(1, threads);
@@ -208,7 +210,7 @@ void ref_soft_rows(void* sh, void* kh, const double* cam, int W, int H, const do
     auto worker = [&](int id) {
       for (int r = next++; r < rows; r = next++)
         for (int x = 0; x < W; ++x) {
-          const int y = y0 + r;
+          const int y = y0 + r * stride;
           size_t k = size_t(r) * size_t(W) + size_t(x);
           Real qx, qy;
           c.pixelToQ(Real(x) + Real(0.5), Real(y) + Real(0.5), qx, qy);
