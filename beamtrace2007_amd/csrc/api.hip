@@ -74,7 +74,7 @@ void fillInfo(const btg::RunInfo& r, double wall, btg_run_info* info) {
     info->kernelSeconds[k] = r.kernelMs[k] * 1e-3;
     info->kernelLaunches[k] = r.launches[k];
   }
-  for (int k = 0; k < 4; ++k) info->phaseCycles[k] = r.phaseCycles[k];
+  for (int k = 0; k < 8; ++k) info->phaseCycles[k] = r.phaseCycles[k];
 }
 
 double now() {

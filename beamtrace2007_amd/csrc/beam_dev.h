@@ -45,7 +45,14 @@ BTG_HD F3 normalize(F3 a) {
   float l = length(a);
   return l > 0 ? a / l : a;
 }
-BTG_HD float comp(F3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+// c ? a : b on values. (With two lvalue operands C++'s ?: is an lvalue: the
+// compiler then selects between their addresses and loads once, which keeps
+// arrays in private memory.)
+template <class T>
+BTG_HD T pick(bool c, T a, T b) {
+  return c ? T(a) : T(b);
+}
+BTG_HD float comp(F3 a, int i) { return pick(i == 0, a.x, pick(i == 1, a.y, a.z)); }
 BTG_HD void setComp(F3& a, int i, float v) {
   if (i == 0)
     a.x = v;
@@ -104,44 +111,106 @@ BTG_HD bool allAbove(const C4& l, const C4& r, float k) {
 
 // ---------------------------------------------------------------- polygons
 // Convex polygon with room for N vertices (PolyN, src/beam/beam_geom.h:16-52).
+//
+// Every access below uses a compile-time slot index: loops run over all N
+// slots with a guard on n, a push writes its slot through selects, and a
+// vertex at a runtime index is read through a select chain. Functions that
+// choose between two slots work on a local copy (choosing between two loads
+// through a reference would be folded into one load through a chosen
+// address). A polygon (and any struct holding one) therefore stays in
+// registers instead of private (scratch) memory, which on a persistent kernel
+// with thousands of lanes per XCD would spill through L2 to HBM. The float
+// operations and their order are exactly those of the straightforward loops.
 template <int N>
 struct Pg {
   float x[N], y[N];
   int n;
-  BTG_HD void clear() { n = 0; }
-  BTG_HD void push(float px, float py) {
-    if (n < N) {
-      x[n] = px;
-      y[n] = py;
-      ++n;
+  Pg() = default;
+  // element-wise copies (a struct memcpy between polygons defeats the
+  // promotion of their storage to registers)
+  BTG_HD Pg(const Pg& o) { assign(o); }
+  BTG_HD Pg& operator=(const Pg& o) {
+    assign(o);
+    return *this;
+  }
+  BTG_HD void assign(const Pg& o) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      x[k] = o.x[k];
+      y[k] = o.y[k];
     }
+    n = o.n;
+  }
+  BTG_HD void clear() { n = 0; }
+  BTG_HD float xAt(int i) const {
+    float v = x[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) v = pick(i == k, x[k], v);
+    return v;
+  }
+  BTG_HD float yAt(int i) const {
+    float v = y[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) v = pick(i == k, y[k], v);
+    return v;
+  }
+  BTG_HD void push(float px, float py) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const bool at = k == n;
+      x[k] = pick(at, px, x[k]);
+      y[k] = pick(at, py, y[k]);
+    }
+    n = n < N ? n + 1 : n;
+  }
+  // Removes vertex i, moving the later ones down.
+  BTG_HD void removeAt(int i) {
+    Pg<N> q = *this;
+#pragma unroll
+    for (int k = 0; k + 1 < N; ++k) {
+      const bool mv = k >= i;
+      q.x[k] = pick(mv, q.x[k + 1], q.x[k]);
+      q.y[k] = pick(mv, q.y[k + 1], q.y[k]);
+    }
+    --q.n;
+    *this = q;
   }
   // Signed area, summed vertex by vertex (positive = counter-clockwise).
   BTG_HD float area() const {
+    const Pg<N> q = *this;
     float a = 0;
-    for (int i = 0; i < n; ++i) {
-      const int j = i + 1 == n ? 0 : i + 1;
-      a += x[i] * y[j] - x[j] * y[i];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (i < q.n) {
+        const bool wrap = !(i + 1 < q.n);
+        const float xj = pick(wrap, q.x[0], q.x[(i + 1) % N]), yj = pick(wrap, q.y[0], q.y[(i + 1) % N]);
+        a += q.x[i] * yj - xj * q.y[i];
+      }
     }
     return a * 0.5f;
   }
   BTG_HD void reverse() {
-    for (int i = 0, j = n - 1; i < j; ++i, --j) {
-      float t = x[i];
-      x[i] = x[j];
-      x[j] = t;
-      t = y[i];
-      y[i] = y[j];
-      y[j] = t;
+    const Pg<N> q = *this;
+    Pg<N> r = q;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      if (k < q.n) {
+        r.x[k] = q.xAt(q.n - 1 - k);
+        r.y[k] = q.yAt(q.n - 1 - k);
+      }
     }
+    *this = r;
   }
   BTG_HD float extent() const {
     float lx = __builtin_inff(), hx = -__builtin_inff(), ly = __builtin_inff(), hy = -__builtin_inff();
-    for (int i = 0; i < n; ++i) {
-      lx = x[i] < lx ? x[i] : lx;
-      hx = x[i] > hx ? x[i] : hx;
-      ly = y[i] < ly ? y[i] : ly;
-      hy = y[i] > hy ? y[i] : hy;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (i < n) {
+        lx = pick(x[i] < lx, x[i], lx);
+        hx = pick(x[i] > hx, x[i], hx);
+        ly = pick(y[i] < ly, y[i], ly);
+        hy = pick(y[i] > hy, y[i], hy);
+      }
     }
     const float ex = hx - lx, ey = hy - ly;
     return ex > ey ? ex : ey;
@@ -149,7 +218,8 @@ struct Pg {
   template <int M>
   BTG_HD void copyFrom(const Pg<M>& o) {
     n = o.n < N ? o.n : N;
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < (M < N ? M : N); ++i) {
       x[i] = o.x[i];
       y[i] = o.y[i];
     }
@@ -158,6 +228,15 @@ struct Pg {
 
 struct Ln {
   float a, b, c;
+  Ln() = default;
+  BTG_HD Ln(float a_, float b_, float c_) : a(a_), b(b_), c(c_) {}
+  BTG_HD Ln(const Ln& o) : a(o.a), b(o.b), c(o.c) {}
+  BTG_HD Ln& operator=(const Ln& o) {
+    a = o.a;
+    b = o.b;
+    c = o.c;
+    return *this;
+  }
   BTG_HD float at(float px, float py) const { return a * px + b * py + c; }
 };
 
@@ -172,67 +251,112 @@ BTG_HD Ln makeLine(float a, float b, float c, float ext) {
 // splitPoly (beam_geom.h:82-114): vertices within eps of the line go to both
 // pieces; crossings are computed from the lexicographically smaller endpoint.
 template <int N>
-BTG_HD bool splitPoly(const Pg<N>& p, const Ln& L, float eps, Pg<N>& in, Pg<N>& out) {
+BTG_HD bool splitPoly(const Pg<N>& p0, const Ln& L, float eps, Pg<N>& in, Pg<N>& out) {
+  const Pg<N> p = p0;
   in.clear();
   out.clear();
   float d[N];
   int cls[N];
   int nin = 0, nout = 0;
-  for (int i = 0; i < p.n; ++i) {
-    d[i] = L.at(p.x[i], p.y[i]);
-    cls[i] = d[i] > eps ? 1 : (d[i] < -eps ? -1 : 0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    d[i] = 0;
+    cls[i] = 0;
+    if (i < p.n) {
+      d[i] = L.at(p.x[i], p.y[i]);
+      cls[i] = d[i] > eps ? 1 : (d[i] < -eps ? -1 : 0);
+    }
   }
-  for (int i = 0; i < p.n; ++i) {
-    const int j = i + 1 == p.n ? 0 : i + 1;
-    nin += (cls[i] >= 0) + (cls[i] * cls[j] < 0);
-    nout += (cls[i] <= 0) + (cls[i] * cls[j] < 0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i < p.n) {
+      const int cj = pick(i + 1 < p.n, cls[(i + 1) % N], cls[0]);
+      nin += (cls[i] >= 0) + (cls[i] * cj < 0);
+      nout += (cls[i] <= 0) + (cls[i] * cj < 0);
+    }
   }
   if (nin > N || nout > N) return false;
-  for (int i = 0; i < p.n; ++i) {
-    const int j = i + 1 == p.n ? 0 : i + 1;
-    if (cls[i] >= 0) in.push(p.x[i], p.y[i]);
-    if (cls[i] <= 0) out.push(p.x[i], p.y[i]);
-    if (cls[i] * cls[j] < 0) {
-      const bool iLow = p.x[i] < p.x[j] || (p.x[i] == p.x[j] && p.y[i] < p.y[j]);
-      const int a = iLow ? i : j, b = iLow ? j : i;
-      const float t = d[a] / (d[a] - d[b]);
-      const float X = p.x[a] + t * (p.x[b] - p.x[a]);
-      const float Y = p.y[a] + t * (p.y[b] - p.y[a]);
-      in.push(X, Y);
-      out.push(X, Y);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i < p.n) {
+      const bool wrap = !(i + 1 < p.n);
+      const int j = (i + 1) % N;
+      const float xi = p.x[i], yi = p.y[i], di = d[i];
+      const float xj = pick(wrap, p.x[0], p.x[j]), yj = pick(wrap, p.y[0], p.y[j]), dj = pick(wrap, d[0], d[j]);
+      const int cj = pick(wrap, cls[0], cls[j]);
+      if (cls[i] >= 0) in.push(xi, yi);
+      if (cls[i] <= 0) out.push(xi, yi);
+      if (cls[i] * cj < 0) {
+        const bool iLow = xi < xj || (xi == xj && yi < yj);
+        const float xa = pick(iLow, xi, xj), ya = pick(iLow, yi, yj), da = pick(iLow, di, dj);
+        const float xb = pick(iLow, xj, xi), yb = pick(iLow, yj, yi), db = pick(iLow, dj, di);
+        const float t = da / (da - db);
+        const float X = xa + t * (xb - xa);
+        const float Y = ya + t * (yb - ya);
+        in.push(X, Y);
+        out.push(X, Y);
+      }
     }
   }
   return true;
 }
 
 // cleanPoly (beam_geom.h:121-150): welds duplicate or collinear vertices at
-// 1e-3 of the split epsilon; fewer than 3 vertices -> n = 0.
+// 1e-3 of the split epsilon; fewer than 3 vertices -> n = 0. The reference
+// repeats passes over the vertices while a pass removed one, re-testing the
+// vertex that moves into a removed slot. The first pass runs with constant
+// indices; only a polygon that loses a vertex continues on the general loop
+// (from the first removed slot, with exactly the reference's sequence).
+BTG_HD bool weldTest(float xh, float yh, float xi, float yi, float xj, float yj, float e) {
+  const float ex = xj - xh, ey = yj - yh;
+  const float dx = xi - xh, dy = yi - yh;
+  const float l = __builtin_sqrtf(ex * ex + ey * ey);
+  const float d2 = dx * dx + dy * dy;
+  const float cr = __builtin_fabsf(ex * dy - ey * dx);
+  return d2 <= e * e || cr <= e * l;
+}
+
 template <int N>
-BTG_HD void cleanPoly(Pg<N>& p, float epsSplit) {
+BTG_HD void cleanPoly(Pg<N>& p0, float epsSplit) {
   const float e = epsSplit * 1e-3f;
+  if (p0.n < 3) {
+    p0.n = 0;
+    return;
+  }
+  Pg<N> p = p0;
+  int first = N;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i < p.n && first == N) {
+      const float xh = i == 0 ? p.xAt(p.n - 1) : float(p.x[(i + N - 1) % N]);
+      const float yh = i == 0 ? p.yAt(p.n - 1) : float(p.y[(i + N - 1) % N]);
+      const bool wrap = !(i + 1 < p.n);
+      const float xj = pick(wrap, p.x[0], p.x[(i + 1) % N]), yj = pick(wrap, p.y[0], p.y[(i + 1) % N]);
+      if (weldTest(xh, yh, p.x[i], p.y[i], xj, yj, e)) first = i;
+    }
+  }
+  if (first == N) return;  // a pass without removals ends the reference's loop
+  p.removeAt(first);
   bool changed = true;
-  while (changed && p.n >= 3) {
-    changed = false;
-    for (int i = 0; i < p.n && p.n >= 3; ++i) {
-      const int h = i == 0 ? p.n - 1 : i - 1;
-      const int j = i + 1 == p.n ? 0 : i + 1;
-      const float ex = p.x[j] - p.x[h], ey = p.y[j] - p.y[h];
-      const float dx = p.x[i] - p.x[h], dy = p.y[i] - p.y[h];
-      const float l = __builtin_sqrtf(ex * ex + ey * ey);
-      const float d2 = dx * dx + dy * dy;
-      const float cr = __builtin_fabsf(ex * dy - ey * dx);
-      if (d2 <= e * e || cr <= e * l) {
-        for (int k = i; k + 1 < p.n; ++k) {
-          p.x[k] = p.x[k + 1];
-          p.y[k] = p.y[k + 1];
-        }
-        --p.n;
-        changed = true;
-        --i;
-      }
+  int i = first;
+  while (p.n >= 3) {
+    if (i >= p.n) {
+      if (!changed) break;
+      changed = false;
+      i = 0;
+      continue;
+    }
+    const int h = i == 0 ? p.n - 1 : i - 1;
+    const int j = i + 1 == p.n ? 0 : i + 1;
+    if (weldTest(p.xAt(h), p.yAt(h), p.xAt(i), p.yAt(i), p.xAt(j), p.yAt(j), e)) {
+      p.removeAt(i);
+      changed = true;
+    } else {
+      ++i;
     }
   }
   if (p.n < 3) p.n = 0;
+  p0 = p;
 }
 
 // classifyPoly (beam_core.h:68-78): bit 0 some vertex strictly inside, bit 1
@@ -240,12 +364,15 @@ BTG_HD void cleanPoly(Pg<N>& p, float epsSplit) {
 template <int N>
 BTG_HD int sidesOf(const Pg<N>& p, const Ln& L, float eps) {
   int c = 0;
-  for (int i = 0; i < p.n; ++i) {
-    const float d = L.at(p.x[i], p.y[i]);
-    if (d > eps)
-      c |= 1;
-    else if (d < -eps)
-      c |= 2;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i < p.n) {
+      const float d = L.at(p.x[i], p.y[i]);
+      if (d > eps)
+        c |= 1;
+      else if (d < -eps)
+        c |= 2;
+    }
   }
   return c;
 }
@@ -253,12 +380,14 @@ BTG_HD int sidesOf(const Pg<N>& p, const Ln& L, float eps) {
 // splitFive (beam_core.h:80-89): pentagon -> quad (0..3) + triangle (0, 3, 4).
 template <int N>
 BTG_HD void splitFive(const Pg<N>& p, Pg<N>& a, Pg<N>& b) {
-  a.clear();
-  b.clear();
-  for (int i = 0; i < 4; ++i) a.push(p.x[i], p.y[i]);
-  b.push(p.x[0], p.y[0]);
-  b.push(p.x[3], p.y[3]);
-  b.push(p.x[4], p.y[4]);
+  a = p;
+  b = p;
+  a.n = 4;
+  b.n = 3;
+  b.x[1] = p.x[3];
+  b.y[1] = p.y[3];
+  b.x[2] = p.x[4];
+  b.y[2] = p.y[4];
 }
 
 // --------------------------------------------------------------- planes

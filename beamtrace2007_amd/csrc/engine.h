@@ -55,7 +55,7 @@ struct RunInfo {
   double dropped = 0;
   double kernelMs[5] = {};
   int launches[5] = {};
-  uint64_t phaseCycles[4] = {};  // trace loop: refill, traverse, test, append (BTG_PHASE_CYCLES builds)
+  uint64_t phaseCycles[8] = {};  // trace loop clocks per region (BTG_PHASE_CYCLES builds)
 };
 
 struct Options {

@@ -226,8 +226,21 @@ struct Beam {
   bool hasFar;
 };
 
-__device__ __forceinline__ int sgnOf(const Beam& b, int a) { return a == 0 ? b.s0 : (a == 1 ? b.s1 : b.s2); }
-__device__ __forceinline__ C4 awOf(const Beam& b, int a) { return a == 0 ? b.aw0 : (a == 1 ? b.aw1 : b.aw2); }
+// Per-axis fields picked by value (a select of loaded values, never of
+// addresses, so the beam stays in registers).
+__device__ __forceinline__ int sgnOf(const Beam& b, int a) {
+  const int s0 = b.s0, s1 = b.s1, s2 = b.s2;
+  return a == 0 ? s0 : (a == 1 ? s1 : s2);
+}
+__device__ __forceinline__ C4 awOf(const Beam& b, int a) {
+  C4 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float x0 = b.aw0.v[i], x1 = b.aw1.v[i], x2 = b.aw2.v[i];
+    r.v[i] = a == 0 ? x0 : (a == 1 ? x1 : x2);
+  }
+  return r;
+}
 
 // Plane of the current hit cached with the beam (WfWork, wavefront_core.h:58-61).
 struct Work {
@@ -303,7 +316,7 @@ __device__ __forceinline__ void setAw(Beam& b, const Ctx& c) {
 // beamGeom (beam_core.h:200-221): corners (lanes past n repeat the last) and
 // |w| per axis; slivers below minArea are dropped.
 template <int N>
-__device__ bool beamGeom(const Ctx& c, const Pg<N>& p, int s0, int s1, int s2, Beam& b, float& dropped) {
+__device__ __forceinline__ bool beamGeom(const Ctx& c, const Pg<N>& p, int s0, int s1, int s2, Beam& b, float& dropped) {
   Pg<N> q = p;
   cleanPoly(q, c.eps);
   const float A = q.n >= 3 ? q.area() : 0.0f;
@@ -311,9 +324,8 @@ __device__ bool beamGeom(const Ctx& c, const Pg<N>& p, int s0, int s1, int s2, B
     dropped = selAbs(p.area());
     return false;
   }
-  const int l3 = q.n > 3 ? 3 : 2;
-  b.qx = c4(q.x[0], q.x[1], q.x[2], q.x[l3]);
-  b.qy = c4(q.y[0], q.y[1], q.y[2], q.y[l3]);
+  b.qx = c4(q.x[0], q.x[1], q.x[2], q.n > 3 ? q.x[3] : q.x[2]);
+  b.qy = c4(q.y[0], q.y[1], q.y[2], q.n > 3 ? q.y[3] : q.y[2]);
   b.n = q.n;
   b.s0 = s0;
   b.s1 = s1;
@@ -356,9 +368,11 @@ __device__ __forceinline__ void hitPlane(const Params& p, const Ctx& c, int t, W
 // wfExpand (wavefront_core.h:92-112).
 __device__ void expand(const Params& p, const Ctx& c, const Rec& r, Work& w) {
   Beam& b = w.b;
-  const int n = r.n, l3 = n > 3 ? 3 : 2;
-  b.qx = c4(r.qx[0], r.qx[1], r.qx[2], r.qx[l3]);
-  b.qy = c4(r.qy[0], r.qy[1], r.qy[2], r.qy[l3]);
+  const int n = r.n;
+  // corners past n repeat the last (selects, not a runtime index, so the
+  // record stays in registers)
+  b.qx = c4(r.qx[0], r.qx[1], r.qx[2], n > 3 ? r.qx[3] : r.qx[2]);
+  b.qy = c4(r.qy[0], r.qy[1], r.qy[2], n > 3 ? r.qy[3] : r.qy[2]);
   b.n = n;
   b.s0 = (r.negBits & 1) ? -1 : 1;
   b.s1 = (r.negBits & 2) ? -1 : 1;
@@ -509,35 +523,39 @@ __device__ __forceinline__ int lineTest(const Ctx& c, const Beam& B, const Ln& L
   return in ? (out ? 1 : 0) : -1;
 }
 
-// classifyTri (beam_core.h:374-409). Lines that the clip needs are returned
-// in L[0..nl) when L is given.
-__device__ int classifyTri(const Ctx& c, const Work& w, const TriS& ti, Ln* L, int& nl) {
+// classifyTri (beam_core.h:374-409) with every line the clip needs: the
+// three edges in L[0..2], then the closer-than-hit line and the far-plane line
+// when they apply (nl of them in all). The slots are fixed so that L stays in
+// registers.
+__device__ __forceinline__ int classifyTri(const Ctx& c, const Work& w, const TriS& ti, Ln (&L)[5], int& nl) {
   const Beam& B = w.b;
   nl = 0;
   bool allIn = true;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
+    L[k] = ti.edge[k];
     const int r = lineTest(c, B, ti.edge[k]);
     if (r < 0) return kKeep;
     allIn = allIn && r == 0;
-    if (L) L[nl] = ti.edge[k];
     ++nl;
   }
-  if (!c.anyHit() && B.hit >= 0) {
-    const Ln cl = closerLine(c, ti, w.hitA, w.hitC, w.hitN, B.hit);
-    const int r = lineTest(c, B, cl);
+  const bool hasCloser = !c.anyHit() && B.hit >= 0;
+  if (hasCloser) {
+    L[3] = closerLine(c, ti, w.hitA, w.hitC, w.hitN, B.hit);
+    const int r = lineTest(c, B, L[3]);
     if (r < 0) return kKeep;
     allIn = allIn && r == 0;
-    if (L) L[nl] = cl;
     ++nl;
   }
   if (c.farIsPlane()) {
     const float farA[3] = {0, 0, c.farCq};
     const Ln cl = closerLine(c, ti, farA, c.farCq, c.P.n, -1);
+    // field-wise selects: an aggregate ?: would store through a chosen slot
+    L[3] = Ln{hasCloser ? L[3].a : cl.a, hasCloser ? L[3].b : cl.b, hasCloser ? L[3].c : cl.c};
+    L[4] = cl;
     const int r = lineTest(c, B, cl);
     if (r < 0) return kKeep;
     allIn = allIn && r == 0;
-    if (L) L[nl] = cl;
     ++nl;
   }
   if (allIn) return kAllIn;
@@ -576,11 +594,25 @@ __device__ int classifyLazy(const Ctx& c, const Work& w, TriS& ti) {
   return kClip;
 }
 
+// Line k of L[0..5) at a runtime index (a select chain, not an array index).
+__device__ __forceinline__ Ln lineAt(const Ln (&L)[5], int k) {
+  Ln r = L[0];
+#pragma unroll
+  for (int i = 1; i < 5; ++i)
+    if (k == i) r = L[i];
+  return r;
+}
+
 // clipTri (beam_core.h:419-479), pieces delivered to sink.push(list, poly):
 // list 0 keeps the beam's status, list 1 takes the triangle. Edges are
-// ordered by their positive corner sums first.
+// ordered by their positive corner sums first. The reference keeps pentagon
+// halves waiting on a stack; its entries always have strictly increasing
+// start lines from bottom to top (a piece started at line s only pushes
+// halves that start after s), so one register slot per start line holds the
+// stack, and popping takes the highest occupied line. One piece-line step per
+// iteration, so the sink has a single call site.
 template <class Sink>
-__device__ void clipTri(const Ctx& c, const Beam& B, Ln* L, int nl, Sink& sink, uint32_t& fiveSplits) {
+__device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5], int nl, Sink& sink, uint32_t& fiveSplits) {
   float score[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -590,82 +622,143 @@ __device__ void clipTri(const Ctx& c, const Beam& B, Ln* L, int nl, Sink& sink, 
     for (int i = 0; i < 4; ++i) ps[i] = dv.v[i] > 0.0f ? dv.v[i] : 0.0f;
     score[k] = ps[0] + ps[1] + ps[2] + (B.n == 4 ? ps[3] : 0.0f);
   }
-  for (int i = 1; i < 3; ++i)
-    for (int j = i; j > 0 && score[j] < score[j - 1]; --j) {
-      const float t = score[j];
-      score[j] = score[j - 1];
-      score[j - 1] = t;
-      const Ln l = L[j];
-      L[j] = L[j - 1];
-      L[j - 1] = l;
+  // insertion sort of the three edges by score (stable, as the reference)
+  if (score[1] < score[0]) {
+    const float t = score[0];
+    score[0] = score[1];
+    score[1] = t;
+    const Ln l = L[0];
+    L[0] = L[1];
+    L[1] = l;
+  }
+  if (score[2] < score[1]) {
+    const float t = score[1];
+    score[1] = score[2];
+    score[2] = t;
+    const Ln l = L[1];
+    L[1] = L[2];
+    L[2] = l;
+    if (score[1] < score[0]) {
+      const float t2 = score[0];
+      score[0] = score[1];
+      score[1] = t2;
+      const Ln l2 = L[0];
+      L[0] = L[1];
+      L[1] = l2;
     }
-  Pg<6> stack[8];
-  int stackLine[8];
-  int sp = 1;
-  stack[0].n = B.n;
+  }
+  Pg<6> p;
+  p.n = B.n;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    stack[0].x[i] = B.qx.v[i];
-    stack[0].y[i] = B.qy.v[i];
+    p.x[i] = B.qx.v[i];
+    p.y[i] = B.qy.v[i];
   }
-  stackLine[0] = 0;
-  while (sp > 0) {
-    --sp;
-    Pg<6> p = stack[sp];
-    int li = stackLine[sp];
-    bool inside = true;
-    for (; li < nl; ++li) {
-      const int cs = sidesOf(p, L[li], c.eps);
-      if (!(cs & 2)) continue;
-      if (!(cs & 1)) {
-        sink.push(0, p);
-        inside = false;
-        break;
-      }
-      Pg<6> in, out;
-      splitPoly(p, L[li], c.eps, in, out);  // pieces of a <= 4-gon have <= 5 corners: never fails
-      cleanPoly(out, c.eps);
-      if (out.n >= 3) sink.push(0, out);
-      cleanPoly(in, c.eps);
-      if (in.n < 3) {
-        inside = false;
-        break;
-      }
-      if (in.n == 5) {
-        Pg<6> a, b;
-        splitFive(in, a, b);
-        ++fiveSplits;
-        stack[sp] = b;  // depth <= 1 + 5 lines < 8
-        stackLine[sp] = li + 1;
-        ++sp;
-        p = a;
+  p.x[4] = p.x[5] = p.y[4] = p.y[5] = 0;
+  // pending triangle halves by start line 1..5 (slot 0 unused)
+  float px[6][3], py[6][3];
+  uint32_t pending = 0;
+  int li = 0;
+  for (;;) {
+    int list = -1;  // the piece handed to the sink in this step
+    Pg<6> q;
+    bool pieceDone = false;
+    if (li >= nl) {  // inside every line: the part of the beam the triangle covers
+      list = 1;
+      q = p;
+      pieceDone = true;
+    } else {
+      const Ln Lk = lineAt(L, li);
+      const int cs = sidesOf(p, Lk, c.eps);
+      if (!(cs & 2)) {
+        ++li;
+      } else if (!(cs & 1)) {
+        list = 0;
+        q = p;
+        pieceDone = true;
       } else {
-        p = in;
+        Pg<6> in, out;
+        splitPoly(p, Lk, c.eps, in, out);  // pieces of a <= 4-gon have <= 5 corners: never fails
+        cleanPoly(out, c.eps);
+        if (out.n >= 3) {
+          list = 0;
+          q = out;
+        }
+        cleanPoly(in, c.eps);
+        if (in.n < 3) {
+          pieceDone = true;
+        } else if (in.n == 5) {  // quad (0..3) goes on, triangle (0, 3, 4) waits at line li + 1
+          ++fiveSplits;
+          const int lv = li + 1;
+#pragma unroll
+          for (int s = 1; s < 6; ++s) {
+            if (s == lv) {
+              px[s][0] = in.x[0];
+              py[s][0] = in.y[0];
+              px[s][1] = in.x[3];
+              py[s][1] = in.y[3];
+              px[s][2] = in.x[4];
+              py[s][2] = in.y[4];
+            }
+          }
+          pending |= 1u << lv;
+          p = in;
+          p.n = 4;
+          li = lv;
+        } else {
+          p = in;
+          ++li;
+        }
       }
     }
-    if (inside) sink.push(1, p);
+    if (list >= 0) sink.push(list, q);
+    if (pieceDone) {
+      if (!pending) break;
+      const int lv = 31 - __builtin_clz(pending);
+      pending &= ~(1u << lv);
+      p.n = 3;
+#pragma unroll
+      for (int s = 1; s < 6; ++s) {
+        if (s == lv) {
+#pragma unroll
+          for (int v = 0; v < 3; ++v) {
+            p.x[v] = px[s][v];
+            p.y[v] = py[s][v];
+          }
+        }
+      }
+      li = lv;
+    }
   }
 }
 
 // Counting pass: records each list produces (pentagons become two) and the
 // total area of the hit pieces (summed in piece order, as the reference).
 struct CountSink {
-  int recs[2] = {0, 0};
+  int recs0 = 0, recs1 = 0;
   float hitArea = 0;
-  __device__ void push(int l, const Pg<6>& q) {
-    recs[l] += q.n == 5 ? 2 : 1;
-    if (l == 1) hitArea += q.area();
+  __device__ __forceinline__ void push(int l, const Pg<6>& q) {
+    const int k = q.n == 5 ? 2 : 1;
+    if (l == 1) {
+      recs1 += k;
+      hitArea += q.area();
+    } else {
+      recs0 += k;
+    }
   }
 };
 
 // -------------------------------------------------------------- outputs
 __device__ __forceinline__ double pow2(int e) { return __builtin_ldexp(1.0, e); }
 
-// polygonFormFactorG (soft_core.h:18-33).
-__device__ float formFactor(F3 x, F3 n, const F3* poly, int count) {
+// polygonFormFactorG (soft_core.h:18-33) of a 3- or 4-gon, unrolled with
+// constant indices so that the corners stay in registers.
+__device__ float formFactor(F3 x, F3 n, const F3 (&poly)[4], int count) {
   double sum = 0;
-  for (int i = 0; i < count; ++i) {
-    const F3 a = poly[i] - x, b = poly[i + 1 == count ? 0 : i + 1] - x;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= count) break;
+    const F3 a = poly[i] - x, b = (i + 1 < count ? poly[(i + 1) & 3] : poly[0]) - x;
     const double la = double(length(a)), lb = double(length(b));
     if (la == 0 || lb == 0) continue;
     double cc = double(dot(a, b)) / (la * lb);
@@ -679,6 +772,16 @@ __device__ float formFactor(F3 x, F3 n, const F3* poly, int count) {
   return float((sum < 0 ? -sum : sum) * 0.5);
 }
 
+// Signed area of a beam's 3- or 4-gon, as PolyN::area (beam_geom.h:27-33)
+// sums it. A triangle's fourth corner repeats its third; the extra term
+// x2*y2 - x2*y2 is exactly +0 and leaves the sum unchanged.
+__device__ __forceinline__ float area4(const C4& x, const C4& y) {
+  float a = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a += x.v[i] * y.v[(i + 1) & 3] - x.v[(i + 1) & 3] * y.v[i];
+  return a * 0.5f;
+}
+
 // Appends an output polygon (primary / point shadows).
 __device__ void appendPoly(const Params& p, int root, const OutPoly& ob) {
   const int k = atomicAdd(&p.st->outCount, 1);
@@ -688,29 +791,36 @@ __device__ void appendPoly(const Params& p, int root, const OutPoly& ob) {
   p.outTag[k] = p.pass;
 }
 
-// pointShadowToImage (point_core.h:41-64).
-__device__ bool pointToImage(const Plane& camPlane, const Plane& sp, const float* sx, const float* sy, int n, int T,
+// pointShadowToImage (point_core.h:41-64) of a 3- or 4-gon on the receiver
+// plane sp: project to the image plane, make it counter-clockwise, repeat the
+// last corner.
+__device__ bool pointToImage(const Plane& camPlane, const Plane& sp, const C4& sx, const C4& sy, int n, int T,
                              OutPoly& ip) {
   ip.n = n;
   ip.tri = T;
   ip.area = 0;
-  for (int i = 0; i < n; ++i)
-    if (!camPlane.project(sp.point(sx[i], sy[i]), ip.x[i], ip.y[i])) return false;
-  Pg<4> q;
-  q.n = 0;
-  for (int i = 0; i < n; ++i) q.push(ip.x[i], ip.y[i]);
-  const float a = q.area();
-  if (!(a > 0 || a < 0)) return false;
-  if (a < 0) {
-    q.reverse();
-    for (int i = 0; i < n; ++i) {
-      ip.x[i] = q.x[i];
-      ip.y[i] = q.y[i];
-    }
+  C4 X, Y;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (!camPlane.project(sp.point(sx.v[i], sy.v[i]), X.v[i], Y.v[i])) return false;
+  if (n > 3) {
+    if (!camPlane.project(sp.point(sx.v[3], sy.v[3]), X.v[3], Y.v[3])) return false;
+  } else {
+    X.v[3] = X.v[2];
+    Y.v[3] = Y.v[2];
   }
-  for (int i = n; i < 4; ++i) {
-    ip.x[i] = ip.x[n - 1];
-    ip.y[i] = ip.y[n - 1];
+  const float a = area4(X, Y);
+  if (!(a > 0 || a < 0)) return false;
+  if (a < 0) {  // reverse the n corners, then repeat the last
+    const C4 rx = n > 3 ? c4(X.v[3], X.v[2], X.v[1], X.v[0]) : c4(X.v[2], X.v[1], X.v[0], X.v[0]);
+    const C4 ry = n > 3 ? c4(Y.v[3], Y.v[2], Y.v[1], Y.v[0]) : c4(Y.v[2], Y.v[1], Y.v[0], Y.v[0]);
+    X = rx;
+    Y = ry;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    ip.x[i] = X.v[i];
+    ip.y[i] = Y.v[i];
   }
   return true;
 }
@@ -718,14 +828,7 @@ __device__ bool pointToImage(const Plane& camPlane, const Plane& sp, const float
 // A finished piece (wfOutput, wavefront_core.h:124-132, and the per-app
 // handling of cuda/wavefront.cu:193-227).
 __device__ void emitOutput(const Params& p, const Ctx& c, const Beam& b, int root, int tri, Cnt& cnt) {
-  Pg<4> q;
-  q.n = b.n;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    q.x[i] = b.qx.v[i];
-    q.y[i] = b.qy.v[i];
-  }
-  const float area = q.area();
+  const float area = area4(b.qx, b.qy);
   if (tri >= 0) cnt.v[sHits]++;
   cnt.v[sBeams]++;
   if (p.app == kSoft) {
@@ -734,8 +837,9 @@ __device__ void emitOutput(const Params& p, const Ctx& c, const Beam& b, int roo
     if (area > 0) atomicAdd(&p.missFx[root], (unsigned long long)(double(area) * pow2(-R.unitExp)));
     if (p.exact) {
       F3 P[4];
-      for (int i = 0; i < q.n; ++i) P[i] = c.P.point(q.x[i], q.y[i]);
-      const float G = formFactor(R.x, R.ns, P, q.n);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) P[i] = c.P.point(b.qx.v[i], b.qy.v[i]);
+      const float G = formFactor(R.x, R.ns, P, b.n);
       atomicAdd(&p.gFx[root], (unsigned long long)rint(double(G) * pow2(60)));
     }
     return;
@@ -743,18 +847,18 @@ __device__ void emitOutput(const Params& p, const Ctx& c, const Beam& b, int roo
   if (tri < 0) return;
   OutPoly ob;
   if (p.app == kPrimary) {
-    ob.n = q.n;
+    // the beam's corners past n already repeat the last (OutBeam's layout)
+    ob.n = b.n;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int k = i < q.n ? i : q.n - 1;
-      ob.x[i] = q.x[k];
-      ob.y[i] = q.y[k];
+      ob.x[i] = b.qx.v[i];
+      ob.y[i] = b.qy.v[i];
     }
     ob.tri = tri;
     ob.area = area;
   } else {
     const RootQ& R = p.roots[root];
-    if (!pointToImage(p.camPlane, c.P, q.x, q.y, q.n, R.tri, ob)) return;
+    if (!pointToImage(p.camPlane, c.P, b.qx, b.qy, b.n, R.tri, ob)) return;
   }
   appendPoly(p, root, ob);
 }
@@ -771,26 +875,105 @@ __device__ __forceinline__ void killRoot(const Params& p, int root) {
 }
 
 // ------------------------------------------------------------- the trace
+// Kd depths whose ancestors a lane keeps in LDS (deeper ones are reached by
+// parent links). The SAH build limits depth to 8 + 1.3 log2(triangles), 36 at
+// 2M triangles.
+constexpr int kPathLevels = 40;
+
 struct Stepper {
   int phase;
-  Box box;
+  Box box;                         // descend: box of r.node
+  uint32_t leafOffset, leafCount;  // the current leaf
+  int validFrom;                   // path levels [validFrom, min(r.depth, kPathLevels)) hold r.node's ancestors
+};
+
+// A lane's column of the LDS ancestor path: level k at col[k * kBlock].
+struct Path {
+  uint32_t* col;
+  __device__ __forceinline__ uint32_t get(int k) const { return col[k * kBlock]; }
+  __device__ __forceinline__ void set(int k, uint32_t v) const { col[k * kBlock] = v; }
 };
 
 __device__ __forceinline__ int highestBit(uint64_t x) { return 63 - __builtin_clzll(x); }
 
+// Phase timing of the trace loop (build with -DBTG_PHASE_CYCLES): wave clock
+// cycles per code region, summed into p.cycles[0..8). PHASE_MARK(k) charges
+// the time since the previous mark to region k: 0 refill, 1 descend, 2 climb
+// (parent walk), 3 climb (child box test), 4 output emit, 5 triangle tests,
+// 6 queue appends, 7 the rest of the traversal loop.
+enum { kPhRefill, kPhDescend, kPhWalk, kPhBox, kPhEmit, kPhTest, kPhAppend, kPhRest, kPhases };
+#ifdef BTG_PHASE_CYCLES
+struct PhaseClock {
+  uint64_t t[kPhases];
+  uint64_t prev;
+};
+#define PHASE_ARG , PhaseClock& ph_
+#define PHASE_PASS , ph_
+#define PHASE_INIT()                                  \
+  PhaseClock ph_;                                     \
+  for (int k_ = 0; k_ < kPhases; ++k_) ph_.t[k_] = 0; \
+  ph_.prev = __builtin_amdgcn_s_memtime()
+#define PHASE_MARK(k)                                 \
+  do {                                                \
+    const uint64_t t_ = __builtin_amdgcn_s_memtime(); \
+    ph_.t[k] += t_ - ph_.prev;                        \
+    ph_.prev = t_;                                    \
+  } while (0)
+#define PHASE_FLUSH()                                                           \
+  do {                                                                          \
+    if (lane() == 0 && p.cycles)                                                \
+      for (int k_ = 0; k_ < kPhases; ++k_) atomicAdd(&p.cycles[k_], ph_.t[k_]); \
+  } while (0)
+#else
+#define PHASE_ARG
+#define PHASE_PASS
+#define PHASE_INIT()
+#define PHASE_MARK(k)
+#define PHASE_FLUSH()
+#endif
+
+// Occupancy of the trace and split kernels: at least N waves per SIMD
+// (-DBTG_TRACE_WAVES_PER_EU=N, -DBTG_SPLIT_WAVES_PER_EU=N) caps their
+// registers at 512 / N. Two waves per SIMD keep the trace kernel in 224
+// registers with nothing spilled; the split kernel spills a little at two
+// waves but runs 1.6x faster than with one.
+#ifndef BTG_TRACE_WAVES_PER_EU
+#define BTG_TRACE_WAVES_PER_EU 2
+#endif
+#ifndef BTG_SPLIT_WAVES_PER_EU
+#define BTG_SPLIT_WAVES_PER_EU 2
+#endif
+#ifdef BTG_TRACE_WAVES_PER_EU
+#define TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(BTG_TRACE_WAVES_PER_EU)))
+#else
+#define TRACE_OCCUPANCY
+#endif
+#ifdef BTG_SPLIT_WAVES_PER_EU
+#define SPLIT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(BTG_SPLIT_WAVES_PER_EU)))
+#else
+#define SPLIT_OCCUPANCY
+#endif
+
 // wfAdvance (wavefront_core.h:157-231): descend to a leaf, skip the excluded
 // triangle, climb to the next pending far child; returns kContinue with a
 // triangle pending at r.cursor, or kFinished after emitting the output.
-__device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& ss, Cnt& cnt) {
+// The descent records each ancestor in the lane's LDS path, so a climb reads
+// the pending child's parent there instead of following parent links; the
+// climb counter still adds one per level climbed, as the reference's walk
+// does. Both children of an inner node are fetched while it is decided.
+__device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& ss, const Path& path,
+                       Cnt& cnt PHASE_ARG) {
   const DevScene& sv = p.sv;
   const Beam& B = w.b;
   for (;;) {
     if (ss.phase == kDescend) {
+      KdNodeD kn = sv.nodes[r.node];
       for (;;) {
-        const KdNodeD kn = sv.nodes[r.node];
         if ((kn.data & 3u) == 3u) break;
+        const KdNodeD kl = sv.nodes[r.node + 1], kr = sv.nodes[kn.data >> 2];
         cnt.v[sKd]++;
         const int d = r.depth++;
+        if (d < kPathLevels) path.set(d, r.node);
         const int a = int(kn.data & 3u);
         const float s = kn.split;
         const int sga = sgnOf(B, a);
@@ -803,6 +986,7 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
         const bool goRight = (dec == kFar) == toRight;
         if (dec == kBoth) r.trail |= uint64_t(1) << d;
         r.node = goRight ? right : left;
+        kn = goRight ? kr : kl;
         if (goRight)
           setComp(ss.box.lo, a, s);
         else
@@ -810,30 +994,53 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
       }
       r.cursor = 0;
       cnt.v[sLeaf]++;
+      ss.leafOffset = kn.offset;
+      ss.leafCount = kn.data >> 2;
       ss.phase = kTest;
+      PHASE_MARK(kPhDescend);
     }
     if (ss.phase == kTest) {
-      if (uint32_t(r.cursor) < (sv.nodes[r.node].data >> 2)) return kContinue;
+      if (uint32_t(r.cursor) < ss.leafCount) return kContinue;
       ss.phase = kClimb;
     }
     for (;;) {
       if (r.trail == 0) {
+        PHASE_MARK(kPhRest);
         emitOutput(p, c, B, r.root, c.anyHit() ? -1 : B.hit, cnt);
+        PHASE_MARK(kPhEmit);
         return kFinished;
       }
       const int d = highestBit(r.trail);
-      uint32_t node = r.node;
-      for (int depth = r.depth; depth > d; --depth) {
-        node = sv.parent[node];
-        cnt.v[sClimbs]++;
+      cnt.v[sClimbs] += uint32_t(r.depth - d);
+      PHASE_MARK(kPhRest);
+      uint32_t node;
+      if (d >= ss.validFrom && d < kPathLevels) {
+        node = path.get(d);
+      } else {
+        // parent links from the shallowest known ancestor, recording the path
+        int depth = r.depth;
+        node = r.node;
+        if (ss.validFrom < depth && ss.validFrom < kPathLevels) {
+          depth = ss.validFrom;
+          node = path.get(depth);
+        }
+        while (depth > d) {
+          node = sv.parent[node];
+          --depth;
+          if (depth < kPathLevels) path.set(depth, node);
+        }
+        if (d < kPathLevels) ss.validFrom = d;
       }
+      PHASE_MARK(kPhWalk);
       r.trail &= (uint64_t(1) << d) - 1;
       const KdNodeD kn = sv.nodes[node];
       const uint32_t cs = sgnOf(B, int(kn.data & 3u)) > 0 ? (kn.data >> 2) : node + 1;
       r.node = node;
       r.depth = int16_t(d);
       const Box cb = sv.nodeBox[cs];
-      if (overlaps(c, B, cb)) {
+      const bool ov = overlaps(c, B, cb);
+      PHASE_MARK(kPhBox);
+      if (ov) {
         r.node = cs;
         r.depth = int16_t(d + 1);
         r.cursor = -1;
@@ -848,11 +1055,10 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
 // wfTestOne (wavefront_core.h:236-268). The excluded triangle (the shading
 // point's own) is skipped here, uncounted, as the reference skips it before
 // testing (wavefront_core.h:198, 276).
-__device__ int testOne(const Params& p, const Ctx& c, Rec& r, Work& w, Cnt& cnt) {
-  const KdNodeD leaf = p.sv.nodes[r.node];
+__device__ int testOne(const Params& p, const Ctx& c, Rec& r, Work& w, const Stepper& ss, Cnt& cnt) {
   const uint32_t i = uint32_t(r.cursor);
   r.cursor = int32_t(i + 1);
-  const LeafTri ref = p.sv.refs[leaf.offset + i];
+  const LeafTri ref = p.sv.refs[ss.leafOffset + i];
   if (ref.tri == c.exclude) return kContinue;
   TriS ti;
   triBegin(c, ref, ti);
@@ -881,35 +1087,11 @@ __device__ int testOne(const Params& p, const Ctx& c, Rec& r, Work& w, Cnt& cnt)
 }
 
 // wfMoreInLeaf (wavefront_core.h:272-279).
-__device__ __forceinline__ bool moreInLeaf(const Params& p, const Rec& r) {
-  return uint32_t(r.cursor) < (p.sv.nodes[r.node].data >> 2);
+__device__ __forceinline__ bool moreInLeaf(const Stepper& ss, const Rec& r) {
+  return uint32_t(r.cursor) < ss.leafCount;
 }
 
-// Phase timing of the trace loop (build with -DBTG_PHASE_CYCLES): wave clock
-// cycles spent refilling, traversing (descend / climb), testing triangles and
-// appending, summed into p.cycles[0..3].
-#ifdef BTG_PHASE_CYCLES
-#define PHASE_INIT()                      \
-  uint64_t phT[4] = {0, 0, 0, 0};         \
-  uint64_t phPrev = __builtin_amdgcn_s_memtime()
-#define PHASE_MARK(k)                                  \
-  do {                                                 \
-    const uint64_t t_ = __builtin_amdgcn_s_memtime();  \
-    phT[k] += t_ - phPrev;                             \
-    phPrev = t_;                                       \
-  } while (0)
-#define PHASE_FLUSH()                                              \
-  do {                                                             \
-    if (lane() == 0 && p.cycles)                                   \
-      for (int k_ = 0; k_ < 4; ++k_) atomicAdd(&p.cycles[k_], phT[k_]); \
-  } while (0)
-#else
-#define PHASE_INIT()
-#define PHASE_MARK(k)
-#define PHASE_FLUSH()
-#endif
-
-__global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
+__global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, int cur) {
   Cnt cnt;
   cnt.zero();
   PHASE_INIT();
@@ -921,6 +1103,8 @@ __global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
   Work w;
   Ctx c;
   Stepper ss;
+  __shared__ uint32_t pathLds[kPathLevels * kBlock];
+  const Path path{pathLds + threadIdx.x};
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const int nIdle = __popcll(idle);
@@ -942,8 +1126,12 @@ __global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
               ss.phase = kDescend;
               ss.box = p.sv.nodeBox[r.node];
             } else {
+              const KdNodeD kn = p.sv.nodes[r.node];
+              ss.leafOffset = kn.offset;
+              ss.leafCount = kn.data >> 2;
               ss.phase = kTest;
             }
+            ss.validFrom = r.depth;
             active = true;
             leaves = 0;
           }
@@ -954,17 +1142,17 @@ __global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
       if (exhausted) break;
       continue;
     }
-    PHASE_MARK(0);
+    PHASE_MARK(kPhRefill);
     int res = kContinue;
-    if (active) res = advance(p, c, r, w, ss, cnt);
-    PHASE_MARK(1);
+    if (active) res = advance(p, c, r, w, ss, path, cnt PHASE_PASS);
+    PHASE_MARK(kPhRest);
     if (active) {
       while (res == kContinue) {
-        res = testOne(p, c, r, w, cnt);
-        if (res != kContinue || !moreInLeaf(p, r)) break;
+        res = testOne(p, c, r, w, ss, cnt);
+        if (res != kContinue || !moreInLeaf(ss, r)) break;
       }
     }
-    PHASE_MARK(2);
+    PHASE_MARK(kPhTest);
     if (active) {
       if (res == kFinished) liveSub(p, r.root, 1);
       if (res != kContinue) active = false;
@@ -997,7 +1185,7 @@ __global__ void __launch_bounds__(kBlock) traceKernel(Params p, int cur) {
         p.sq[base + rankBelow(splits)] = r;
       }
     }
-    PHASE_MARK(3);
+    PHASE_MARK(kPhAppend);
   }
   flushCnt(p, cnt);
   PHASE_FLUSH();
@@ -1013,32 +1201,34 @@ struct EmitSink {
   const Beam* parent;
   const Rec* base;
   Rec* dst;
-  int k[2];
+  int k0, k1;  // next record of list 0 / list 1
   int t;
   Cnt* cnt;
-  __device__ void push(int l, const Pg<6>& q0) {
+  __device__ __forceinline__ void push(int l, const Pg<6>& q0) {
+    if (q0.n == 5) {
+      Pg<6> sub0, sub1;
+      splitFive(q0, sub0, sub1);
+      cnt->v[sFive]++;
+      one(l, sub0);
+      one(l, sub1);
+    } else {
+      one(l, q0);
+    }
+  }
+  __device__ __forceinline__ void one(int l, const Pg<6>& sub) {
     const bool queued = !(l == 1 && c->anyHit());
     const int hit = l == 1 ? t : parent->hit;
-    Pg<6> sub[2];
-    int ns = 1;
-    if (q0.n == 5) {
-      splitFive(q0, sub[0], sub[1]);
-      cnt->v[sFive]++;
-      ns = 2;
-    } else {
-      sub[0] = q0;
-    }
-    for (int j = 0; j < ns; ++j) {
+    {
       Beam nb;
       float dropped = 0;
-      const bool ok = beamGeom(*c, sub[j], parent->s0, parent->s1, parent->s2, nb, dropped);
+      const bool ok = beamGeom(*c, sub, parent->s0, parent->s1, parent->s2, nb, dropped);
       if (!ok) {
         cnt->dropped += double(dropped);
       } else if (!queued) {
         nb.hit = t;
         emitOutput(*p, *c, nb, base->root, t, *cnt);
       }
-      if (!queued) continue;
+      if (!queued) return;
       Rec rr = *base;
       if (ok) {
         storeGeom(nb, rr);
@@ -1046,14 +1236,17 @@ struct EmitSink {
       } else {
         rr.n = 0;
       }
-      dst[k[l]++] = rr;
+      if (l == 1)
+        dst[k1++] = rr;
+      else
+        dst[k0++] = rr;
     }
   }
 };
 
 __device__ void scheduleNext(const Params& p, int cur);
 
-__global__ void __launch_bounds__(kBlock) splitKernel(Params p, int cur) {
+__global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, int cur) {
   Cnt cnt;
   cnt.zero();
   const int n = int(min((unsigned long long)p.st->splitCount, p.capacity));
@@ -1091,11 +1284,12 @@ __global__ void __launch_bounds__(kBlock) splitKernel(Params p, int cur) {
       classifyTri(c, w, ti, L, nl);  // kClip, as found by the trace
       CountSink cs;
       Ln Lc[5];
+#pragma unroll
       for (int k = 0; k < 5; ++k) Lc[k] = L[k];
       clipTri(c, w.b, Lc, nl, cs, cnt.v[sFive]);
       commit = cs.hitArea >= c.minArea;
-      recs0 = cs.recs[0];
-      slots = commit ? cs.recs[0] + (c.anyHit() ? 0 : cs.recs[1]) : 1;
+      recs0 = cs.recs0;
+      slots = commit ? cs.recs0 + (c.anyHit() ? 0 : cs.recs1) : 1;
       const int old = atomicAdd(&p.live[r.root], slots - 1);
       if (old + slots - 1 > p.budget) {
         killRoot(p, r.root);
@@ -1125,8 +1319,8 @@ __global__ void __launch_bounds__(kBlock) splitKernel(Params p, int cur) {
         es.parent = &w.b;
         es.base = &b;
         es.dst = dst;
-        es.k[0] = 0;
-        es.k[1] = recs0;
+        es.k0 = 0;
+        es.k1 = recs0;
         es.t = tri;
         es.cnt = &cnt;
         uint32_t dummyFive = 0;
@@ -1250,67 +1444,79 @@ __device__ bool admitPiece(const Params& p, const Ctx& c, const Pg<8>& qd, const
   return true;
 }
 
+// presplit (beam_core.h:487-546): split the root by the three ray-direction
+// sign lines (axis 0, 1, 2 in turn), then fan each piece into <= 4-gons. The
+// reference walks this depth-3 binary tree with a stack, the inside piece of
+// a split before the outside one; here the walk keeps one pending outside
+// piece per axis level (a register slot, not a runtime-indexed stack), which
+// visits the pieces in the same order.
 __device__ void admitPoly(const Params& p, const Ctx& c, const Pg<8>& rootPoly, int root, int cur, AdmitOut& ao,
                           Cnt& cnt) {
-  struct Item {
-    Pg<8> p;
-    int axis;
-    int sg[3];
-  };
-  Item stack[16];
-  int sp = 0;
-  Pg<8> p0 = rootPoly;
-  if (p0.n < 3) return;
-  if (p0.area() < 0) p0.reverse();
-  stack[0].p = p0;
-  stack[0].axis = 0;
-  stack[0].sg[0] = stack[0].sg[1] = stack[0].sg[2] = 1;
-  sp = 1;
+  Pg<8> q = rootPoly;
+  if (q.n < 3) return;
+  if (q.area() < 0) q.reverse();
   const Plane& P = c.P;
-  while (sp > 0) {
-    Item it = stack[--sp];
-    const Pg<8>& q = it.p;
-    if (q.n < 3) continue;
-    if (it.axis == 3) {
+  Pg<8> pend0, pend1, pend2;  // outside pieces waiting at axis 0, 1, 2
+  uint32_t pending = 0;
+  int axis = 0;
+  int s0 = 1, s1 = 1, s2 = 1;
+  for (;;) {
+    bool leafDone = false;
+    if (q.n < 3) {
+      leafDone = true;
+    } else if (axis == 3) {
+      // fan into quads (0, k, k+1, k+2) and a last triangle
       for (int k = 1; k + 1 < q.n; k += 2) {
         Pg<8> qd;
         qd.n = 0;
         qd.push(q.x[0], q.y[0]);
-        qd.push(q.x[k], q.y[k]);
-        qd.push(q.x[k + 1], q.y[k + 1]);
-        if (k + 2 < q.n) qd.push(q.x[k + 2], q.y[k + 2]);
-        if (!admitPiece(p, c, qd, it.sg, root, cur, ao, cnt)) return;
+        qd.push(q.xAt(k), q.yAt(k));
+        qd.push(q.xAt(k + 1), q.yAt(k + 1));
+        if (k + 2 < q.n) qd.push(q.xAt(k + 2), q.yAt(k + 2));
+        const int sg[3] = {s0, s1, s2};
+        if (!admitPiece(p, c, qd, sg, root, cur, ao, cnt)) return;
       }
-      continue;
-    }
-    const int axis = it.axis;
-    const Ln L = makeLine(comp(P.u, axis), comp(P.v, axis), comp(P.d, axis), c.ext);
-    const int cs = sidesOf(q, L, c.eps);
-    if (cs == 3) {
-      if (sp + 2 > 16) {
-        ao.ok = false;
-        return;
-      }
-      Pg<8> in, out;
-      splitPoly(q, L, c.eps, in, out);
-      cleanPoly(in, c.eps);
-      cleanPoly(out, c.eps);
-      Item a = it, b = it;
-      a.p = in;
-      a.axis = axis + 1;
-      a.sg[axis] = 1;
-      b.p = out;
-      b.axis = axis + 1;
-      b.sg[axis] = -1;
-      stack[sp++] = b;
-      stack[sp++] = a;
-      cnt.v[sPresplit]++;
-    } else if (cs != 0) {
-      it.axis = axis + 1;
-      it.sg[axis] = (cs & 1) ? 1 : -1;
-      stack[sp++] = it;
+      leafDone = true;
     } else {
-      cnt.dropped += double(selAbs(q.area()));
+      const Ln L = makeLine(comp(P.u, axis), comp(P.v, axis), comp(P.d, axis), c.ext);
+      const int cs = sidesOf(q, L, c.eps);
+      if (cs == 3) {
+        Pg<8> in, out;
+        splitPoly(q, L, c.eps, in, out);
+        cleanPoly(in, c.eps);
+        cleanPoly(out, c.eps);
+        cnt.v[sPresplit]++;
+        if (axis == 0) pend0 = out;
+        if (axis == 1) pend1 = out;
+        if (axis == 2) pend2 = out;
+        pending |= 1u << axis;
+        q = in;
+        if (axis == 0) s0 = 1;
+        if (axis == 1) s1 = 1;
+        if (axis == 2) s2 = 1;
+        ++axis;
+      } else if (cs != 0) {
+        const int sv = (cs & 1) ? 1 : -1;
+        if (axis == 0) s0 = sv;
+        if (axis == 1) s1 = sv;
+        if (axis == 2) s2 = sv;
+        ++axis;
+      } else {
+        cnt.dropped += double(selAbs(q.area()));
+        leafDone = true;
+      }
+    }
+    if (leafDone) {
+      if (!pending) return;
+      const int lv = 31 - __builtin_clz(pending);  // the deepest pending outside piece
+      pending &= ~(1u << lv);
+      if (lv == 0) q = pend0;
+      if (lv == 1) q = pend1;
+      if (lv == 2) q = pend2;
+      if (lv == 0) s0 = -1;
+      if (lv == 1) s1 = -1;
+      if (lv == 2) s2 = -1;
+      axis = lv + 1;
     }
   }
 }
@@ -1700,7 +1906,7 @@ class Engine {
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry, cycles;
   WState* hostState = nullptr;  // pinned
-  uint64_t phaseCycles[4] = {};
+  uint64_t phaseCycles[8] = {};
   // per-launch timing (Options::profile): event pairs and the kernel kind
   std::vector<hipEvent_t> evPool;
   std::vector<int> evKind;
@@ -1783,7 +1989,7 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
       (rc = e.ensure(e.epoch, size_t(R) * 4, err)) || (rc = e.ensure(e.missFx, size_t(R) * 8, err)) ||
       (rc = e.ensure(e.gFx, size_t(R) * 8, err)) || (rc = e.ensure(e.stats, sizeof(uint64_t) * sNum, err)) ||
       (rc = e.ensure(e.dropped, sizeof(double), err)) || (rc = e.ensure(e.st, sizeof(WState), err)) ||
-      (rc = e.ensure(e.cycles, 4 * sizeof(uint64_t), err)) ||
+      (rc = e.ensure(e.cycles, 8 * sizeof(uint64_t), err)) ||
       (rc = e.ensure(e.list, size_t(R) * 4, err)) || (rc = e.ensure(e.listCount, 4, err)))
     return rc;
   p = Params();
@@ -1880,7 +2086,7 @@ int zeroStats(Engine& e, std::string& err) {
   const uint64_t z[sNum + 1] = {};
   BTG_TRY(hipMemcpyAsync(e.stats.ptr, z, sizeof(uint64_t) * sNum, hipMemcpyHostToDevice, e.stream));
   BTG_TRY(hipMemcpyAsync(e.dropped.ptr, z, sizeof(double), hipMemcpyHostToDevice, e.stream));
-  BTG_TRY(hipMemcpyAsync(e.cycles.ptr, z, 4 * sizeof(uint64_t), hipMemcpyHostToDevice, e.stream));
+  BTG_TRY(hipMemcpyAsync(e.cycles.ptr, z, 8 * sizeof(uint64_t), hipMemcpyHostToDevice, e.stream));
   return 0;
 }
 
@@ -2076,7 +2282,7 @@ int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, float offset, 
   info.primaryMs += ms0;
   info.traceMs += ms1;
   if ((rc = readStats(*e, info.stats, &info.dropped, err))) return rc;
-  for (int k = 0; k < 4; ++k) info.phaseCycles[k] = e->phaseCycles[k];
+  for (int k = 0; k < 8; ++k) info.phaseCycles[k] = e->phaseCycles[k];
   e->collectLaunches(info);
   if (tri) BTG_TRY(hipMemcpyAsync(tri, e->pixTri.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
   if (vis) BTG_TRY(hipMemcpyAsync(vis, e->vis.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));

@@ -59,7 +59,7 @@ class btg_run_info(C.Structure):
     _fields_ = [("primarySeconds", C.c_double), ("traceSeconds", C.c_double), ("totalSeconds", C.c_double),
                 ("rounds", C.c_int), ("maxQueue", C.c_int), ("killedRoots", C.c_int), ("retryPasses", C.c_int),
                 ("stats", btg_trace_stats), ("rayStats", btg_trace_stats), ("kernelSeconds", C.c_double * 5),
-                ("kernelLaunches", C.c_int * 5), ("phaseCycles", C.c_uint64 * 4)]
+                ("kernelLaunches", C.c_int * 5), ("phaseCycles", C.c_uint64 * 8)]
 
 
 POLY_DTYPE = np.dtype([("x", "<f4", (4,)), ("y", "<f4", (4,)), ("n", "<i4"), ("tri", "<i4"), ("area", "<f4")])
@@ -128,10 +128,12 @@ def info_dict(ri: btg_run_info) -> dict:
             "retry_passes": ri.retryPasses, "stats": st, "ray_stats": rs,
             "kernel_s": dict(zip(KERNELS, [float(v) for v in ri.kernelSeconds])),
             "launches": dict(zip(KERNELS, [int(v) for v in ri.kernelLaunches])),
-            "phase_cycles": dict(zip(["refill", "traverse", "test", "append"], [int(v) for v in ri.phaseCycles]))}
+            "phase_cycles": dict(zip(PHASES, [int(v) for v in ri.phaseCycles]))}
 
 
 KERNELS = ["admit", "trace", "split", "primary_rays", "other"]
+# trace-loop regions timed by BTG_PHASE_CYCLES builds (btg_run_info.phaseCycles)
+PHASES = ["refill", "descend", "climb_walk", "climb_box", "emit", "test", "append", "advance_rest"]
 
 
 def polys_to_dict(arr: np.ndarray) -> dict:

@@ -53,11 +53,33 @@ def parse():
     ap.add_argument("--budget", type=int, default=0, help="live sub-beams per root (0: library default)")
     ap.add_argument("--capacity", type=int, default=0, help="sub-beam records per queue (0: library default)")
     ap.add_argument("--leaf-cap", type=int, default=-1, help="leaves per sub-beam per trace launch (-1: default)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 10 device-to-device copies of MIB MiB (the known-size copy that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes) and print their size")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int) -> int:
+    """A plain copy kernel of known size, for calibrating the HBM counters:
+    profiled with rocprofv3 --pmc FETCH_SIZE (or WRITE_SIZE), each copy
+    dispatch should read (write) exactly `mib` MiB."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst, src))
+    print(json.dumps({"copy_bytes": n * 4, "copies": 10, "ok": ok}), flush=True)
+    return 0 if ok else 1
 
 
 def main() -> int:
     args = parse()
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the tracer shards by independent roots and needs no collectives, see DESIGN.md", file=sys.stderr)
@@ -116,7 +138,9 @@ def main() -> int:
     kernel_ms = {k: round(sum(i["kernel_s"][k] for i in infos) / args.steps * 1e3, 3) for k in gpu.KERNELS}
 
     # one downloaded frame for the parity band and error against the double build
+    t1 = time.perf_counter()
     full = r.soft_shadows(cam, W, H, light, exact=True, download=True)
+    pcie_s = time.perf_counter() - t1
     r.close()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
@@ -129,9 +153,11 @@ def main() -> int:
         "beams_per_s": round(st["beams"] * args.steps / wall, 1),
         "rounds": infos[-1]["rounds"], "killed_roots": infos[-1]["killed_roots"],
         "kernel_ms_per_step": kernel_ms, "alg_bytes_per_step": int(b_job),
-        "trace_phase_cycles": infos[-1]["phase_cycles"],
+        "pcie_inclusive_px_per_s": round(px / pcie_s, 1),  # one frame with tri/vis/E/Eu copied to pageable host memory
         "roofline": roofline, "host_kd_build_s": round(build_s, 3),
     }
+    if any(infos[-1]["phase_cycles"].values()):  # instrumented builds only
+        out["trace_phase_cycles"] = infos[-1]["phase_cycles"]
     out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full))
     print(json.dumps(out), flush=True)
     return 0

@@ -126,8 +126,9 @@ typedef struct {
   double kernelSeconds[5];
   int kernelLaunches[5];
   /* builds with -DBTG_PHASE_CYCLES: trace-loop clocks spent refilling,
-   * traversing, testing triangles and appending (soft shadows) */
-  uint64_t phaseCycles[4];
+   * descending, climbing (parent walk, child box test), emitting, testing
+   * triangles, appending and in the rest of the traversal (soft shadows) */
+  uint64_t phaseCycles[8];
 } btg_run_info;
 
 /* GpuRenderer::softShadows (cuda/gpu_render.h:36): exact soft shadows of a
