@@ -94,10 +94,19 @@ struct WState {
   int outCount;
   int blocksDone;
   int round;
-  int pad;
+  int shadeCount;  // soft, exact: unoccluded pieces queued this round for the Lambert integral
   unsigned long long reserved;
   unsigned long long traceItems, splitItems;
 };
+
+// An unoccluded soft-shadow piece waiting for its exact Lambert integral:
+// corners on its root's plane (lanes past n repeat the last), root, count.
+struct ShadeRec {
+  float qx[4], qy[4];
+  int32_t root;
+  int32_t n;
+};
+static_assert(sizeof(ShadeRec) == 40, "shade record layout");
 
 struct Params {
   DevScene sv;
@@ -138,6 +147,7 @@ struct Params {
   F3 pointLight;
   const OutPoly* prim;
   OutPoly* out;
+  struct ShadeRec* shq;  // soft, exact: pieces waiting for their Lambert integral
   int* outRoot;
   int* outTag;
   int outCap;
@@ -162,6 +172,17 @@ __device__ __forceinline__ int waveAppend(int* counter, int n) {
   if (lane() == kWave - 1 && total > 0) base = atomicAdd(counter, total);
   base = __shfl(base, kWave - 1, kWave);
   return base + x - n;
+}
+
+// One slot of `counter` per calling lane, one atomic per wave; callable from
+// divergent code (the active lanes share the atomic of their lowest lane).
+__device__ __forceinline__ int activeAppend(int* counter) {
+  const uint64_t m = __ballot(1);
+  const int leader = __builtin_ctzll(m);
+  int base = 0;
+  if (lane() == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader, kWave);
+  return base + rankBelow(m);
 }
 
 struct Cnt {
@@ -206,6 +227,19 @@ __device__ __forceinline__ Ctx ctxOf(const RootQ& q) {
   c.farCq = q.farCq;
   c.flags = q.flags;
   c.exclude = q.exclude;
+  return c;
+}
+
+// The flags every root of an application has, as compile-time constants:
+// soft and point shadows take any hit and stop at a far plane
+// (lightBeamSetup, soft_core.h:38-73; pointShadowSetup, point_core.h:15-39),
+// primary visibility keeps the nearest hit with optional back-face culling.
+// Kernels are instantiated per application, so each drops the other
+// applications' code and registers.
+template <int App>
+__device__ __forceinline__ Ctx modeCtx(const RootQ& q) {
+  Ctx c = ctxOf(q);
+  c.flags = App == kPrimary ? (q.flags & fCull) : (fAnyHit | fFarIsPlane);
   return c;
 }
 
@@ -827,26 +861,36 @@ __device__ bool pointToImage(const Plane& camPlane, const Plane& sp, const C4& s
 
 // A finished piece (wfOutput, wavefront_core.h:124-132, and the per-app
 // handling of cuda/wavefront.cu:193-227).
-__device__ void emitOutput(const Params& p, const Ctx& c, const Beam& b, int root, int tri, Cnt& cnt) {
+template <int App>
+__device__ __forceinline__ void emitOutput(const Params& p, const Ctx& c, const Beam& b, int root, int tri, Cnt& cnt) {
   const float area = area4(b.qx, b.qy);
   if (tri >= 0) cnt.v[sHits]++;
   cnt.v[sBeams]++;
-  if (p.app == kSoft) {
+  if (App == kSoft) {
     if (tri >= 0) return;  // occluded
     const RootQ& R = p.roots[root];
     if (area > 0) atomicAdd(&p.missFx[root], (unsigned long long)(double(area) * pow2(-R.unitExp)));
-    if (p.exact) {
-      F3 P[4];
+    if (p.exact) {  // the Lambert integral is taken by shadeKernel, after this round's trace
+      const int k = activeAppend(&p.st->shadeCount);
+      if ((unsigned long long)k < p.capacity) {
+        ShadeRec sr;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) P[i] = c.P.point(b.qx.v[i], b.qy.v[i]);
-      const float G = formFactor(R.x, R.ns, P, b.n);
-      atomicAdd(&p.gFx[root], (unsigned long long)rint(double(G) * pow2(60)));
+        for (int i = 0; i < 4; ++i) {
+          sr.qx[i] = b.qx.v[i];
+          sr.qy[i] = b.qy.v[i];
+        }
+        sr.root = root;
+        sr.n = b.n;
+        p.shq[k] = sr;
+      } else {
+        p.st->overflow = 1;
+      }
     }
     return;
   }
   if (tri < 0) return;
   OutPoly ob;
-  if (p.app == kPrimary) {
+  if (App == kPrimary) {
     // the beam's corners past n already repeat the last (OutBeam's layout)
     ob.n = b.n;
 #pragma unroll
@@ -940,11 +984,14 @@ struct PhaseClock {
 #ifndef BTG_TRACE_WAVES_PER_EU
 #define BTG_TRACE_WAVES_PER_EU 2
 #endif
+#ifndef BTG_SOFT_TRACE_WAVES_PER_EU
+#define BTG_SOFT_TRACE_WAVES_PER_EU 3
+#endif
 #ifndef BTG_SPLIT_WAVES_PER_EU
 #define BTG_SPLIT_WAVES_PER_EU 2
 #endif
 #ifdef BTG_TRACE_WAVES_PER_EU
-#define TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(BTG_TRACE_WAVES_PER_EU)))
+#define TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(App == kSoft ? BTG_SOFT_TRACE_WAVES_PER_EU : BTG_TRACE_WAVES_PER_EU)))
 #else
 #define TRACE_OCCUPANCY
 #endif
@@ -961,6 +1008,7 @@ struct PhaseClock {
 // the pending child's parent there instead of following parent links; the
 // climb counter still adds one per level climbed, as the reference's walk
 // does. Both children of an inner node are fetched while it is decided.
+template <int App>
 __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& ss, const Path& path,
                        Cnt& cnt PHASE_ARG) {
   const DevScene& sv = p.sv;
@@ -1006,7 +1054,7 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
     for (;;) {
       if (r.trail == 0) {
         PHASE_MARK(kPhRest);
-        emitOutput(p, c, B, r.root, c.anyHit() ? -1 : B.hit, cnt);
+        emitOutput<App>(p, c, B, r.root, c.anyHit() ? -1 : B.hit, cnt);
         PHASE_MARK(kPhEmit);
         return kFinished;
       }
@@ -1055,6 +1103,7 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
 // wfTestOne (wavefront_core.h:236-268). The excluded triangle (the shading
 // point's own) is skipped here, uncounted, as the reference skips it before
 // testing (wavefront_core.h:198, 276).
+template <int App>
 __device__ int testOne(const Params& p, const Ctx& c, Rec& r, Work& w, const Stepper& ss, Cnt& cnt) {
   const uint32_t i = uint32_t(r.cursor);
   r.cursor = int32_t(i + 1);
@@ -1070,7 +1119,7 @@ __device__ int testOne(const Params& p, const Ctx& c, Rec& r, Work& w, const Ste
   if (res == kKeep) return kContinue;
   if (res == kAllIn) {
     if (c.anyHit()) {
-      emitOutput(p, c, w.b, r.root, t, cnt);
+      emitOutput<App>(p, c, w.b, r.root, t, cnt);
       return kFinished;
     }
     setStatus(c, w.b, t, ti.A, ti.C);
@@ -1091,6 +1140,7 @@ __device__ __forceinline__ bool moreInLeaf(const Stepper& ss, const Rec& r) {
   return uint32_t(r.cursor) < ss.leafCount;
 }
 
+template <int App>
 __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, int cur) {
   Cnt cnt;
   cnt.zero();
@@ -1120,7 +1170,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
           if (r.n == 0 || p.state[r.root] == kKilled) {  // a dropped sliver, or an abandoned root
             liveSub(p, r.root, 1);
           } else {
-            c = ctxOf(p.roots[r.root]);
+            c = modeCtx<App>(p.roots[r.root]);
             expand(p, c, r, w);
             if (r.cursor < 0) {
               ss.phase = kDescend;
@@ -1144,11 +1194,11 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
     }
     PHASE_MARK(kPhRefill);
     int res = kContinue;
-    if (active) res = advance(p, c, r, w, ss, path, cnt PHASE_PASS);
+    if (active) res = advance<App>(p, c, r, w, ss, path, cnt PHASE_PASS);
     PHASE_MARK(kPhRest);
     if (active) {
       while (res == kContinue) {
-        res = testOne(p, c, r, w, ss, cnt);
+        res = testOne<App>(p, c, r, w, ss, cnt);
         if (res != kContinue || !moreInLeaf(ss, r)) break;
       }
     }
@@ -1195,6 +1245,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
 // Emitting pass: writes records (or outputs) exactly as wfSplitEmit
 // (wavefront_core.h:371-422) into dst[0..c0) for list 0 and dst[c0..) for
 // list 1.
+template <int App>
 struct EmitSink {
   const Params* p;
   const Ctx* c;
@@ -1225,8 +1276,13 @@ struct EmitSink {
       if (!ok) {
         cnt->dropped += double(dropped);
       } else if (!queued) {
-        nb.hit = t;
-        emitOutput(*p, *c, nb, base->root, t, *cnt);
+        if (App == kSoft) {  // occluded by t: counted, nothing to accumulate
+          cnt->v[sHits]++;
+          cnt->v[sBeams]++;
+        } else {
+          nb.hit = t;
+          emitOutput<App>(*p, *c, nb, base->root, t, *cnt);
+        }
       }
       if (!queued) return;
       Rec rr = *base;
@@ -1246,6 +1302,7 @@ struct EmitSink {
 
 __device__ void scheduleNext(const Params& p, int cur);
 
+template <int App>
 __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, int cur) {
   Cnt cnt;
   cnt.zero();
@@ -1271,7 +1328,7 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
       if (p.state[r.root] == kKilled) {
         liveSub(p, r.root, 1);
       } else {
-        c = ctxOf(p.roots[r.root]);
+        c = modeCtx<App>(p.roots[r.root]);
         expand(p, c, r, w);
         have = true;
       }
@@ -1313,7 +1370,7 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
         dst[0] = b;
       } else {
         cnt.v[sSplits]++;
-        EmitSink es;
+        EmitSink<App> es;
         es.p = &p;
         es.c = &c;
         es.parent = &w.b;
@@ -1357,6 +1414,7 @@ __device__ void scheduleNext(const Params& p, int cur) {
   s->splitItems += (unsigned long long)vs->splitCount;
   s->count[cur] = 0;
   s->splitCount = 0;
+  s->shadeCount = 0;
   s->head[0] = s->head[1] = s->head[2] = 0;
   s->blocksDone = 0;
   const unsigned long long reserved = vs->reserved;
@@ -1381,6 +1439,7 @@ __global__ void initKernel(Params p) {
     s->next = 0;
     s->count[0] = s->count[1] = 0;
     s->splitCount = 0;
+    s->shadeCount = 0;
     s->head[0] = s->head[1] = s->head[2] = 0;
     s->done = 0;
     s->killed = 0;
@@ -1411,6 +1470,7 @@ struct AdmitOut {
   bool ok;
 };
 
+template <int App>
 __device__ bool admitPiece(const Params& p, const Ctx& c, const Pg<8>& qd, const int* sg, int root, int cur,
                            AdmitOut& ao, Cnt& cnt) {
   Beam b;
@@ -1422,7 +1482,7 @@ __device__ bool admitPiece(const Params& p, const Ctx& c, const Pg<8>& qd, const
   setStatus(c, b, -1, nullptr, 0);
   cnt.v[sRoots]++;
   if (!overlaps(c, b, p.sv.bounds)) {
-    emitOutput(p, c, b, root, -1, cnt);
+    emitOutput<App>(p, c, b, root, -1, cnt);
     return true;
   }
   Rec r;
@@ -1450,6 +1510,7 @@ __device__ bool admitPiece(const Params& p, const Ctx& c, const Pg<8>& qd, const
 // a split before the outside one; here the walk keeps one pending outside
 // piece per axis level (a register slot, not a runtime-indexed stack), which
 // visits the pieces in the same order.
+template <int App>
 __device__ void admitPoly(const Params& p, const Ctx& c, const Pg<8>& rootPoly, int root, int cur, AdmitOut& ao,
                           Cnt& cnt) {
   Pg<8> q = rootPoly;
@@ -1474,7 +1535,7 @@ __device__ void admitPoly(const Params& p, const Ctx& c, const Pg<8>& rootPoly, 
         qd.push(q.xAt(k + 1), q.yAt(k + 1));
         if (k + 2 < q.n) qd.push(q.xAt(k + 2), q.yAt(k + 2));
         const int sg[3] = {s0, s1, s2};
-        if (!admitPiece(p, c, qd, sg, root, cur, ao, cnt)) return;
+        if (!admitPiece<App>(p, c, qd, sg, root, cur, ao, cnt)) return;
       }
       leafDone = true;
     } else {
@@ -1593,6 +1654,7 @@ __device__ __forceinline__ bool softPixel(const Params& p, int k, int& pix) {
   return x < p.W && y < p.H;
 }
 
+template <int App>
 __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
   p.live[k] = 0;
   p.state[k] = kNoBeam;
@@ -1606,7 +1668,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
   R.x = R.ns = f3(0, 0, 0);
   Pg<8> poly;
   poly.n = 0;
-  if (p.app == kSoft) {
+  if (App == kSoft) {
     p.missFx[k] = 0;
     p.gFx[k] = 0;
     int pix;
@@ -1633,7 +1695,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
     }
     R.x = x;
     R.ns = ns;
-  } else if (p.app == kPrimary) {
+  } else if (App == kPrimary) {
     const int tx = k % p.tilesW, ty = k / p.tilesW;
     const int x0 = tx * p.tile, y0 = ty * p.tile;
     const int x1 = min(p.W, x0 + p.tile), y1 = min(p.H, y0 + p.tile);
@@ -1657,7 +1719,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
     }
   }
   setupQuery(R, poly.extent());
-  if (p.app == kSoft) {
+  if (App == kSoft) {
     // Unit of the exact area sum: ulp(minArea); every output area is a
     // multiple of it (areas >= minArea).
     int e = 0;
@@ -1666,14 +1728,15 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
   }
   p.roots[k] = R;
   p.state[k] = kTraced;
-  const Ctx c = ctxOf(R);
+  const Ctx c = modeCtx<App>(R);
   AdmitOut ao{0, true};
-  admitPoly(p, c, poly, k, cur, ao, cnt);
+  admitPoly<App>(p, c, poly, k, cur, ao, cnt);
   p.live[k] = ao.pushed;
   if (!ao.ok || ao.pushed > p.budget) killRoot(p, k);
   if (ao.pushed == 0) rootDone(p);
 }
 
+template <int App>
 __global__ void __launch_bounds__(kBlock) admitKernel(Params p, int cur) {
   Cnt cnt;
   cnt.zero();
@@ -1686,10 +1749,27 @@ __global__ void __launch_bounds__(kBlock) admitKernel(Params p, int cur) {
     const int i = base + lane();
     if (i < n) {
       const int slot = begin + i;
-      admitRoot(p, p.rootList ? p.rootList[slot] : slot, cur, cnt);
+      admitRoot<App>(p, p.rootList ? p.rootList[slot] : slot, cur, cnt);
     }
   }
   flushCnt(p, cnt);
+}
+
+// The exact Lambert integral of the unoccluded pieces queued this round
+// (polygonFormFactorG, soft_core.h:18-33), summed per root in 2^-60 fixed
+// point. Kept out of the trace kernel so that its double-precision code does
+// not raise the trace kernel's register count.
+__global__ void __launch_bounds__(kBlock) shadeKernel(Params p) {
+  const int n = int(min((unsigned long long)p.st->shadeCount, p.capacity));
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const ShadeRec sr = p.shq[i];
+    const RootQ& R = p.roots[sr.root];
+    F3 P[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) P[k] = R.plane.point(sr.qx[k], sr.qy[k]);
+    const float G = formFactor(R.x, R.ns, P, sr.n);
+    atomicAdd(&p.gFx[sr.root], (unsigned long long)rint(double(G) * pow2(60)));
+  }
 }
 
 // finalizeSoftKernel (cuda/wavefront.cu:539-561) with the exact sums:
@@ -1904,7 +1984,7 @@ class Engine {
   void* sceneMem[5] = {};
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
-  DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry, cycles;
+  DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry, cycles, shq;
   WState* hostState = nullptr;  // pinned
   uint64_t phaseCycles[8] = {};
   // per-launch timing (Options::profile): event pairs and the kernel kind
@@ -1954,7 +2034,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles};
+                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -2033,8 +2113,11 @@ void setCamera(const HostCam& cam, Params& p) {
 
 
 // Runs the rounds of one pass until the state reports done.
-int runPass(Engine& e, Params& p, RunInfo& info, std::string& err) {
-  const int gA = residentGrid(e, admitKernel), gT = residentGrid(e, traceKernel), gS = residentGrid(e, splitKernel);
+template <int App>
+int runPassApp(Engine& e, Params& p, RunInfo& info, std::string& err) {
+  const int gA = residentGrid(e, admitKernel<App>), gT = residentGrid(e, traceKernel<App>),
+            gS = residentGrid(e, splitKernel<App>), gH = residentGrid(e, shadeKernel);
+  const bool shade = App == kSoft && p.exact;
   initKernel<<<1, 64, 0, e.stream>>>(p);
   BTG_TRY(hipGetLastError());
   int rounds = 0;
@@ -2043,13 +2126,18 @@ int runPass(Engine& e, Params& p, RunInfo& info, std::string& err) {
     for (int i = 0; i < k; ++i, ++rounds) {
       const int cur = rounds & 1;
       int ls = e.beginLaunch(0);
-      admitKernel<<<gA, kBlock, 0, e.stream>>>(p, cur);
+      admitKernel<App><<<gA, kBlock, 0, e.stream>>>(p, cur);
       e.endLaunch(ls);
       ls = e.beginLaunch(1);
-      traceKernel<<<gT, kBlock, 0, e.stream>>>(p, cur);
+      traceKernel<App><<<gT, kBlock, 0, e.stream>>>(p, cur);
       e.endLaunch(ls);
+      if (shade) {
+        ls = e.beginLaunch(4);
+        shadeKernel<<<gH, kBlock, 0, e.stream>>>(p);
+        e.endLaunch(ls);
+      }
       ls = e.beginLaunch(2);
-      splitKernel<<<gS, kBlock, 0, e.stream>>>(p, cur);
+      splitKernel<App><<<gS, kBlock, 0, e.stream>>>(p, cur);
       e.endLaunch(ls);
     }
     BTG_TRY(hipGetLastError());
@@ -2068,6 +2156,12 @@ int runPass(Engine& e, Params& p, RunInfo& info, std::string& err) {
   info.rounds += e.hostState->round;
   info.maxQueue = std::max(info.maxQueue, e.hostState->maxQueue);
   return 0;
+}
+
+int runPass(Engine& e, Params& p, RunInfo& info, std::string& err) {
+  if (p.app == kSoft) return runPassApp<kSoft>(e, p, info, err);
+  if (p.app == kPrimary) return runPassApp<kPrimary>(e, p, info, err);
+  return runPassApp<kPoint>(e, p, info, err);
 }
 
 int readStats(Engine& e, uint64_t* dst, double* dropped, std::string& err) {
@@ -2250,6 +2344,10 @@ int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, float offset, 
   p.lV = L.V;
   p.offset = offset;
   p.exact = exact ? 1 : 0;
+  if (exact) {
+    if ((rc = e->ensure(e->shq, size_t(p.capacity) * sizeof(ShadeRec), err))) return rc;
+    p.shq = static_cast<ShadeRec*>(e->shq.ptr);
+  }
   p.pixTri = static_cast<const int*>(e->pixTri.ptr);
   p.pixX = static_cast<const float*>(e->pixX.ptr);
   p.pixN = static_cast<const float*>(e->pixN.ptr);

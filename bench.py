@@ -38,6 +38,10 @@ WORKLOADS = {
     "room": ("room", 512, "exact soft shadows, seeded 10x8x3 m room (888 tris), 1x1 m ceiling light"),
 }
 METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request latency"
+# Measured HBM bytes per trace-kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+# scaled by a 256 MiB copy: profiles/trace_plant512_hbm_traffic.txt), valid
+# for the default options on that workload; other runs report null.
+TRAFFIC = {("plant", 512): 196594220}
 
 
 def parse():
@@ -130,8 +134,10 @@ def main() -> int:
     roofline = None
     if trace_s > 0:
         achieved = b_trace * args.steps / trace_s / 1e9
+        defaults = args.budget <= 0 and args.capacity <= 0 and args.leaf_cap < 0
+        traffic = TRAFFIC.get((args.workload, res)) if defaults else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                     "kernel": "traceKernel", "launches_per_step": trace_launches / args.steps,
                     "avg_launch_ms": round(trace_s / max(1, trace_launches) * 1e3, 4),
                     "alg_bytes_per_step": int(b_trace)}
