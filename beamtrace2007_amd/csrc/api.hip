@@ -111,49 +111,111 @@ int btg_device_query(btg_device_info* info) {
 
 const char* btg_last_error(const btg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
 
-int btg_create(btg_ctx** out, const btg_scene_desc* d) {
-  if (!out || !d) return failWith(nullptr, BTG_ERR_ARG, "null argument");
-  *out = nullptr;
+namespace {
+
+// Validates a scene description on the host: the kernels trust its indices.
+bool validScene(const btg_scene_desc* d, std::string& why) {
   if (d->numTris <= 0 || d->numNodes <= 0 || !d->pos || !d->tri3 || !d->triN || !d->nodes || !d->parent ||
-      !d->nodeBox || (d->numRefs > 0 && !d->triIndices))
-    return failWith(nullptr, BTG_ERR_ARG, "scene description is incomplete (no triangles or no kd-tree)");
-  if (d->numTris >= (1LL << 31) || d->numNodes >= (1LL << 30) || d->numRefs >= (1LL << 32))
-    return failWith(nullptr, BTG_ERR_ARG, "scene too large for 32-bit indices");
-  // Validate indices on the host: the kernels trust them.
+      !d->nodeBox || (d->numRefs > 0 && !d->triIndices)) {
+    why = "scene description is incomplete (no triangles or no kd-tree)";
+    return false;
+  }
+  if (d->numTris >= (1LL << 31) || d->numNodes >= (1LL << 30) || d->numRefs >= (1LL << 32)) {
+    why = "scene too large for 32-bit indices";
+    return false;
+  }
   for (int64_t t = 0; t < 3 * d->numTris; ++t)
-    if (d->tri3[t] >= uint64_t(d->numVerts)) return failWith(nullptr, BTG_ERR_ARG, "vertex index out of range");
+    if (d->tri3[t] >= uint64_t(d->numVerts)) {
+      why = "vertex index out of range";
+      return false;
+    }
   for (int64_t i = 0; i < d->numRefs; ++i)
-    if (d->triIndices[i] >= uint64_t(d->numTris)) return failWith(nullptr, BTG_ERR_ARG, "leaf triangle out of range");
+    if (d->triIndices[i] >= uint64_t(d->numTris)) {
+      why = "leaf triangle out of range";
+      return false;
+    }
   const btg::KdNodeD* nodes = static_cast<const btg::KdNodeD*>(d->nodes);
   for (int64_t i = 0; i < d->numNodes; ++i) {
     const uint32_t data = nodes[i].data;
     if ((data & 3u) == 3u) {
-      if (uint64_t(nodes[i].offset) + (data >> 2) > uint64_t(d->numRefs))
-        return failWith(nullptr, BTG_ERR_ARG, "kd leaf references out of range");
+      if (uint64_t(nodes[i].offset) + (data >> 2) > uint64_t(d->numRefs)) {
+        why = "kd leaf references out of range";
+        return false;
+      }
     } else if ((data >> 2) <= uint64_t(i) + 1 || (data >> 2) >= uint64_t(d->numNodes)) {
-      return failWith(nullptr, BTG_ERR_ARG, "kd child index out of range");
+      why = "kd child index out of range";
+      return false;
     }
-    if (i > 0 && d->parent[i] >= uint64_t(d->numNodes)) return failWith(nullptr, BTG_ERR_ARG, "kd parent out of range");
+    if (i > 0 && d->parent[i] >= uint64_t(d->numNodes)) {
+      why = "kd parent out of range";
+      return false;
+    }
   }
+  return true;
+}
+
+// Apex offset of the soft-shadow beams: |scene diagonal| * kEpsRel
+// (cuda/gpu_render.cu:574).
+float sceneOffset(const btg_scene_desc* d) {
+  const btg::F3 diag = btg::f3(d->sceneBounds[3] - d->sceneBounds[0], d->sceneBounds[4] - d->sceneBounds[1],
+                               d->sceneBounds[5] - d->sceneBounds[2]);
+  return btg::length(diag) * btg::kEpsRel;
+}
+
+btg::HostLight lightOf(const btg_area_light& l) {
+  btg::HostLight L;
+  L.c = btg::f3(l.c[0], l.c[1], l.c[2]);
+  L.U = btg::f3(l.U[0], l.U[1], l.U[2]);
+  L.V = btg::f3(l.V[0], l.V[1], l.V[2]);
+  return L;
+}
+
+}  // namespace
+
+int btg_create(btg_ctx** out, const btg_scene_desc* d) {
+  if (!out || !d) return failWith(nullptr, BTG_ERR_ARG, "null argument");
+  *out = nullptr;
+  std::string why;
+  if (!validScene(d, why)) return failWith(nullptr, BTG_ERR_ARG, why);
   if (!btg_device_query(nullptr)) return failWith(nullptr, BTG_ERR_DEVICE, "no HIP device available");
   try {
     btg_ctx* c = new btg_ctx();
     std::string err;
-    c->engine = btg::engineCreate(*d, err);
+    c->offset = sceneOffset(d);
+    c->engine = btg::engineCreate(*d, c->offset, err);
     if (!c->engine) {
       delete c;
       return failWith(nullptr, BTG_ERR_DEVICE, err);
     }
     c->desc = *d;
-    // apex offset of the soft-shadow beams: |scene diagonal| * kEpsRel (cuda/gpu_render.cu:574)
-    const btg::F3 diag = btg::f3(d->sceneBounds[3] - d->sceneBounds[0], d->sceneBounds[4] - d->sceneBounds[1],
-                                 d->sceneBounds[5] - d->sceneBounds[2]);
-    c->offset = btg::length(diag) * btg::kEpsRel;
     *out = c;
     return BTG_OK;
   } catch (const std::bad_alloc&) {
     return failWith(nullptr, BTG_ERR_MEMORY, "host allocation failed");
   }
+}
+
+int btg_scene_add(btg_ctx* ctx, const btg_scene_desc* d, int* sceneId) {
+  if (!ctx || !d || !sceneId) return failWith(ctx, BTG_ERR_ARG, "null argument");
+  std::string why;
+  if (!validScene(d, why)) return failWith(ctx, BTG_ERR_ARG, why);
+  try {
+    const int id = btg::engineAddScene(ctx->engine, *d, sceneOffset(d), ctx->err);
+    if (id < 0) return id;
+    *sceneId = id;
+    return BTG_OK;
+  } catch (const std::bad_alloc&) {
+    return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
+  }
+}
+
+int btg_scene_remove(btg_ctx* ctx, int sceneId) {
+  if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");
+  return btg::engineRemoveScene(ctx->engine, sceneId, ctx->err);
+}
+
+int64_t btg_scene_bytes(const btg_ctx* ctx, int sceneId) {
+  return ctx ? btg::engineSceneBytes(ctx->engine, sceneId) : -1;
 }
 
 void btg_destroy(btg_ctx* ctx) {
@@ -195,17 +257,106 @@ int btg_soft(btg_ctx* ctx, const btg_camera* cam, const btg_area_light* light, i
   if (!validCam(cam) || !light) return failWith(ctx, BTG_ERR_ARG, "invalid camera or light");
   const double t0 = now();
   btg::RunInfo ri;
-  btg::HostLight L;
-  L.c = btg::f3(light->c[0], light->c[1], light->c[2]);
-  L.U = btg::f3(light->U[0], light->U[1], light->U[2]);
-  L.V = btg::f3(light->V[0], light->V[1], light->V[2]);
   try {
-    const int rc = btg::engineSoft(ctx->engine, camOf(*cam), L, ctx->offset, exact != 0, tri, vis, E, Eu, ri, ctx->err);
+    const int rc = btg::engineSoft(ctx->engine, camOf(*cam), lightOf(*light), exact != 0, tri, vis, E, Eu, ri, ctx->err);
     if (rc) return rc;
   } catch (const std::bad_alloc&) {
     return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
   }
   fillInfo(ri, now() - t0, info);
+  return BTG_OK;
+}
+
+int btg_soft_batch(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact, btg_run_info* info) {
+  if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");
+  if (!jobs || numJobs <= 0) return failWith(ctx, BTG_ERR_ARG, "a batch needs at least one job");
+  const double t0 = now();
+  btg::RunInfo ri;
+  try {
+    std::vector<btg::SoftJobH> h(static_cast<size_t>(numJobs));
+    for (int j = 0; j < numJobs; ++j) {
+      if (!validCam(&jobs[j].cam)) return failWith(ctx, BTG_ERR_ARG, "job " + std::to_string(j) + ": invalid camera");
+      h[size_t(j)] = btg::SoftJobH{jobs[j].scene, camOf(jobs[j].cam), lightOf(jobs[j].light), jobs[j].tri,
+                                   jobs[j].vis, jobs[j].E, jobs[j].Eu};
+    }
+    const int rc = btg::engineSoftBatch(ctx->engine, h.data(), numJobs, exact != 0, ri, ctx->err);
+    if (rc) return rc;
+  } catch (const std::bad_alloc&) {
+    return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
+  }
+  fillInfo(ri, now() - t0, info);
+  return BTG_OK;
+}
+
+int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact, double maxSeconds,
+                 btg_options* best, btg_autotune_trial* trials, int maxTrials, int* numTrials) {
+  if (!ctx || !best) return failWith(ctx, BTG_ERR_ARG, "null argument");
+  if (!jobs || numJobs <= 0) return failWith(ctx, BTG_ERR_ARG, "the tuning sample needs at least one job");
+  btg::Options& opt = btg::engineOptions(ctx->engine);
+  const btg::Options start = opt;
+  // Coordinate descent over the schedule knobs, one knob at a time, each
+  // value timed on the sample (GPU time of the beam kernels and primary
+  // rays); the best value of a knob is kept before the next knob is tuned.
+  struct Knob {
+    int btg::Options::*field;
+    std::vector<int> values;
+  };
+  const Knob knobs[] = {{&btg::Options::leafCap, {2, 4, 8, 16, 32, 0}},
+                        {&btg::Options::budget, {256, 1024, 4096}},
+                        {&btg::Options::roundsPerCheck, {4, 8, 16, 32}}};
+  std::vector<btg::SoftJobH> h(static_cast<size_t>(numJobs));
+  for (int j = 0; j < numJobs; ++j) {
+    if (!validCam(&jobs[j].cam)) return failWith(ctx, BTG_ERR_ARG, "job " + std::to_string(j) + ": invalid camera");
+    h[size_t(j)] = btg::SoftJobH{jobs[j].scene, camOf(jobs[j].cam), lightOf(jobs[j].light), nullptr, nullptr,
+                                 nullptr, nullptr};
+  }
+  btg::Options cur = start;
+  cur.profile = false;
+  auto timeOf = [&](const btg::Options& o, double& sec) -> int {
+    opt = o;
+    btg::RunInfo ri;
+    const int rc = btg::engineSoftBatch(ctx->engine, h.data(), numJobs, exact != 0, ri, ctx->err);
+    sec = (ri.primaryMs + ri.traceMs) * 1e-3;
+    return rc;
+  };
+  const double t0 = now();
+  int n = 0;
+  double bestSec = 0;
+  int rc = timeOf(cur, bestSec);  // also warms the code and buffers up
+  if (!rc) rc = timeOf(cur, bestSec);
+  auto record = [&](const btg::Options& o, double sec) {
+    if (trials && n < maxTrials) trials[n] = btg_autotune_trial{o.leafCap, o.budget, o.roundsPerCheck, sec};
+    ++n;
+  };
+  if (!rc) record(cur, bestSec);
+  for (const Knob& k : knobs) {
+    if (rc) break;
+    const int keep = cur.*k.field;
+    int bestVal = keep;
+    for (int v : k.values) {
+      if (v == keep) continue;
+      if (now() - t0 > maxSeconds) break;
+      btg::Options o = cur;
+      o.*k.field = v;
+      double sec = 0;
+      if ((rc = timeOf(o, sec))) break;
+      record(o, sec);
+      if (sec < bestSec) {
+        bestSec = sec;
+        bestVal = v;
+      }
+    }
+    cur.*k.field = bestVal;
+  }
+  opt = start;  // the caller applies the result with btg_set_options
+  if (rc) return rc;
+  if (numTrials) *numTrials = n;
+  best->budget = cur.budget;
+  best->primaryBudget = start.primaryBudget;
+  best->capacity = start.capacity;
+  best->roundsPerCheck = cur.roundsPerCheck;
+  best->profileKernels = start.profile ? 1 : 0;
+  best->leafCap = cur.leafCap;
   return BTG_OK;
 }
 

@@ -86,14 +86,36 @@ static_assert(sizeof(OutPoly) == 44, "polygon layout");
 
 class Engine;
 
-// Creates the engine's device state for a scene (copies the arrays).
-Engine* engineCreate(const btg_scene_desc& scene, std::string& err);
+// Creates the engine's device state for a scene (copies the arrays); the
+// scene becomes resident scene 0. `offset` is the scene's soft-shadow apex
+// offset (scene extent x the split epsilon).
+Engine* engineCreate(const btg_scene_desc& scene, float offset, std::string& err);
 void engineDestroy(Engine* e);
 Options& engineOptions(Engine* e);
 
-// Soft shadows: host outputs (any may be null), W*H each.
-int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, float offset, bool exact, int32_t* tri,
-               float* vis, float* E, float* Eu, RunInfo& info, std::string& err);
+// Adds a resident scene (copies the arrays); returns its id >= 1, or a
+// negative error code. Removing frees the id for reuse.
+int engineAddScene(Engine* e, const btg_scene_desc& scene, float offset, std::string& err);
+int engineRemoveScene(Engine* e, int scene, std::string& err);
+// Device bytes a resident scene occupies (-1: no such scene).
+int64_t engineSceneBytes(const Engine* e, int scene);
+
+// A soft-shadow job: resident scene, camera, light, host outputs (any may be
+// null: the image stays in device memory), W*H each.
+struct SoftJobH {
+  int scene;
+  HostCam cam;
+  HostLight light;
+  int32_t* tri;
+  float* vis;
+  float* E;
+  float* Eu;
+};
+// Many jobs in one persistent wavefront pass over all their roots.
+int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo& info, std::string& err);
+// One job on scene 0.
+int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, bool exact, int32_t* tri, float* vis, float* E,
+               float* Eu, RunInfo& info, std::string& err);
 // Primary visibility, one root per tile: hit polygons.
 int enginePrimary(Engine* e, const HostCam& cam, bool cull, int tile, std::vector<OutPoly>& out, RunInfo& info,
                   std::string& err);

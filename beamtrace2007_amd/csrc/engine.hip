@@ -99,6 +99,21 @@ struct WState {
   unsigned long long traceItems, splitItems;
 };
 
+// One soft-shadow job of a batch (a single call is a batch of one). Every
+// job's scene lives in the context's scene arena: its kd root is node node0,
+// its triangles are numbered from triBase. Its pixels are numbered from
+// pixBase in the batch's per-pixel arrays, its roots (pixels in 8 x 8 tiles)
+// from rootBase, its 16 x 16 primary-ray tiles from tileBase.
+struct SoftJob {
+  int W, H, tilesX;
+  uint32_t node0;
+  int triBase, pixBase, rootBase, tileBase;
+  Box bounds;  // the kd root box of the job's scene
+  float offset, halfW, halfH;
+  F3 eye, fwd, right, down;
+  F3 lc, lU, lV;
+};
+
 // An unoccluded soft-shadow piece waiting for its exact Lambert integral:
 // corners on its root's plane (lanes past n repeat the last), root, count.
 struct ShadeRec {
@@ -110,6 +125,10 @@ static_assert(sizeof(ShadeRec) == 40, "shade record layout");
 
 struct Params {
   DevScene sv;
+  uint32_t node0;  // primary / point: kd root and box of the traced scene
+  Box rootBox;
+  const SoftJob* jobs;  // soft shadows: the batch's jobs, by increasing rootBase
+  int numJobs;
   int app;
   int R;
   const int* rootList;
@@ -1468,11 +1487,14 @@ __global__ void initKernel(Params p) {
 struct AdmitOut {
   int pushed;
   bool ok;
+  uint32_t node0;  // kd root and box of the root's scene
+  Box bounds;
 };
 
 template <int App>
 __device__ bool admitPiece(const Params& p, const Ctx& c, const Pg<8>& qd, const int* sg, int root, int cur,
                            AdmitOut& ao, Cnt& cnt) {
+  const uint32_t node0 = ao.node0;
   Beam b;
   float dropped = 0;
   if (!beamGeom(c, qd, sg[0], sg[1], sg[2], b, dropped)) {
@@ -1481,14 +1503,14 @@ __device__ bool admitPiece(const Params& p, const Ctx& c, const Pg<8>& qd, const
   }
   setStatus(c, b, -1, nullptr, 0);
   cnt.v[sRoots]++;
-  if (!overlaps(c, b, p.sv.bounds)) {
+  if (!overlaps(c, b, ao.bounds)) {
     emitOutput<App>(p, c, b, root, -1, cnt);
     return true;
   }
   Rec r;
   storeGeom(b, r);
   r.trail = 0;
-  r.node = 0;
+  r.node = node0;
   r.root = root;
   r.hit = -1;
   r.cursor = -1;
@@ -1583,7 +1605,7 @@ __device__ void admitPoly(const Params& p, const Ctx& c, const Pg<8>& rootPoly, 
 }
 
 // lightBeamSetup (soft_core.h:38-73).
-__device__ bool lightSetup(const Params& p, F3 x, F3 ns, int tri, RootQ& q, Pg<8>& root) {
+__device__ bool lightSetup(const SoftJob& p, F3 x, F3 ns, int tri, RootQ& q, Pg<8>& root) {
   const F3 nL = normalize(cross(p.lU, p.lV));
   if (!(dot(x - p.lc, nL) > 0)) return false;
   const F3 apex = x + ns * p.offset;
@@ -1615,7 +1637,7 @@ __device__ bool lightSetup(const Params& p, F3 x, F3 ns, int tri, RootQ& q, Pg<8
 }
 
 // lightCenterG (soft_core.h:76-84).
-__device__ float centerG(const Params& p, F3 x, F3 ns) {
+__device__ float centerG(const SoftJob& p, F3 x, F3 ns) {
   const F3 nL = normalize(cross(p.lU, p.lV));
   if (!(dot(x - p.lc, nL) > 0)) return 0;
   const F3 w = p.lc - x;
@@ -1646,12 +1668,27 @@ __device__ bool pointSetup(const Params& p, F3 nT, F3 P0, int T, const float* px
   return true;
 }
 
-// Soft-shadow roots are pixels in 8 x 8 tiles: a wave's 64 lanes trace one tile.
-__device__ __forceinline__ bool softPixel(const Params& p, int k, int& pix) {
-  const int t = k >> 6, l = k & 63;
-  const int x = (t % p.tilesX) * 8 + (l & 7), y = (t / p.tilesX) * 8 + (l >> 3);
-  pix = y * p.W + x;
-  return x < p.W && y < p.H;
+// The job of batch root k (jobs are ordered by rootBase).
+__device__ __forceinline__ int jobOfRoot(const Params& p, int k) {
+  int lo = 0, hi = p.numJobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.jobs[mid].rootBase <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Soft-shadow roots are pixels in 8 x 8 tiles: a wave's 64 lanes trace one
+// tile. Root k of job J -> its pixel in the batch's per-pixel arrays.
+__device__ __forceinline__ bool softPixel(const SoftJob& J, int k, int& pix) {
+  const int kl = k - J.rootBase;
+  const int t = kl >> 6, l = kl & 63;
+  const int x = (t % J.tilesX) * 8 + (l & 7), y = (t / J.tilesX) * 8 + (l >> 3);
+  pix = J.pixBase + y * J.W + x;
+  return x < J.W && y < J.H;
 }
 
 template <int App>
@@ -1668,11 +1705,16 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
   R.x = R.ns = f3(0, 0, 0);
   Pg<8> poly;
   poly.n = 0;
+  uint32_t node0 = p.node0;
+  Box bounds = p.rootBox;
   if (App == kSoft) {
     p.missFx[k] = 0;
     p.gFx[k] = 0;
+    const SoftJob& J = p.jobs[jobOfRoot(p, k)];
+    node0 = J.node0;
+    bounds = J.bounds;
     int pix;
-    if (!softPixel(p, k, pix)) {
+    if (!softPixel(J, k, pix)) {
       rootDone(p);
       return;
     }
@@ -1686,8 +1728,8 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
     }
     const F3 x = f3(p.pixX[3 * pix], p.pixX[3 * pix + 1], p.pixX[3 * pix + 2]);
     const F3 ns = f3(p.pixN[3 * pix], p.pixN[3 * pix + 1], p.pixN[3 * pix + 2]);
-    p.Eu[pix] = length(cross(p.lU, p.lV)) * centerG(p, x, ns);
-    if (!lightSetup(p, x, ns, T, R, poly)) {
+    p.Eu[pix] = length(cross(J.lU, J.lV)) * centerG(J, x, ns);
+    if (!lightSetup(J, x, ns, T, R, poly)) {
       p.vis[pix] = 0;
       p.E[pix] = 0;
       rootDone(p);
@@ -1729,7 +1771,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
   p.roots[k] = R;
   p.state[k] = kTraced;
   const Ctx c = modeCtx<App>(R);
-  AdmitOut ao{0, true};
+  AdmitOut ao{0, true, node0, bounds};
   admitPoly<App>(p, c, poly, k, cur, ao, cnt);
   p.live[k] = ao.pushed;
   if (!ao.ok || ao.pushed > p.budget) killRoot(p, k);
@@ -1780,7 +1822,7 @@ __global__ void finalizeSoftKernel(Params p) {
   if (i >= p.R) return;
   const int k = p.rootList ? p.rootList[i] : i;
   int pix;
-  if (!softPixel(p, k, pix)) return;
+  if (!softPixel(p.jobs[jobOfRoot(p, k)], k, pix)) return;
   const int s = p.state[k];
   if (s != kTraced) return;  // no beam (written at admission) or killed (traced again)
   const RootQ& R = p.roots[k];
@@ -1824,13 +1866,14 @@ __device__ __forceinline__ float rayTri(const LeafTri& t, F3 o, F3 d) {
 
 // rayTraverse<false> (ray_core.h:42-132): nearest hit in (0, inf), coplanar
 // ties to the lower id within 4 ulp.
-__device__ bool rayNearest(const DevScene& sv, F3 o, F3 d, int& hitTri, float& hitT, Cnt& cnt) {
+__device__ bool rayNearest(const DevScene& sv, uint32_t node0, const Box& bounds, F3 o, F3 d, int& hitTri, float& hitT,
+                           Cnt& cnt) {
   cnt.v[sRays]++;
   float t0 = 0, t1 = __builtin_inff();
   float inv[3];
   const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
-  const float blo[3] = {sv.bounds.lo.x, sv.bounds.lo.y, sv.bounds.lo.z};
-  const float bhi[3] = {sv.bounds.hi.x, sv.bounds.hi.y, sv.bounds.hi.z};
+  const float blo[3] = {bounds.lo.x, bounds.lo.y, bounds.lo.z};
+  const float bhi[3] = {bounds.hi.x, bounds.hi.y, bounds.hi.z};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     inv[a] = 1 / dd[a];
@@ -1852,7 +1895,7 @@ __device__ bool rayNearest(const DevScene& sv, F3 o, F3 d, int& hitTri, float& h
   uint32_t stNode[64];
   float stT0[64], stT1[64];
   int sp = 0;
-  uint32_t node = 0;
+  uint32_t node = node0;
   float best = __builtin_inff();
   int bestTri = -1;
   for (;;) {
@@ -1919,22 +1962,34 @@ __device__ bool rayNearest(const DevScene& sv, F3 o, F3 d, int& hitTri, float& h
 __global__ void __launch_bounds__(kBlock) primaryRayKernel(Params p) {
   Cnt cnt;
   cnt.zero();
-  const int x = int(blockIdx.x) * 16 + int(threadIdx.x & 15), y = int(blockIdx.y) * 16 + int(threadIdx.x >> 4);
-  if (x < p.W && y < p.H) {
-    const size_t k = size_t(y) * size_t(p.W) + size_t(x);
-    const float qx = (2 * (float(x) + 0.5f) / float(p.W) - 1) * p.halfW;
-    const float qy = (2 * (float(y) + 0.5f) / float(p.H) - 1) * p.halfH;
-    const F3 d = p.fwd + p.right * qx + p.down * qy;
-    int t = -1;
+  // block -> (job, 16 x 16 tile): the job with the largest tileBase <= block
+  const int b = int(blockIdx.x);
+  int lo = 0, hi = p.numJobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.jobs[mid].tileBase <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SoftJob& J = p.jobs[lo];
+  const int t = b - J.tileBase, tx = (J.W + 15) / 16;
+  const int x = (t % tx) * 16 + int(threadIdx.x & 15), y = (t / tx) * 16 + int(threadIdx.x >> 4);
+  if (x < J.W && y < J.H) {
+    const size_t k = size_t(J.pixBase) + size_t(y) * size_t(J.W) + size_t(x);
+    const float qx = (2 * (float(x) + 0.5f) / float(J.W) - 1) * J.halfW;
+    const float qy = (2 * (float(y) + 0.5f) / float(J.H) - 1) * J.halfH;
+    const F3 d = J.fwd + J.right * qx + J.down * qy;
+    int th_tri = -1;
     float th = 0;
     int* triOut = const_cast<int*>(p.pixTri);
     float* X = const_cast<float*>(p.pixX);
     float* N = const_cast<float*>(p.pixN);
     triOut[k] = -1;
-    if (rayNearest(p.sv, p.eye, d, t, th, cnt)) {
-      triOut[k] = t;
-      const F3 hp = p.eye + d * th;
-      const float4 nn = p.sv.triNV[2 * size_t(t)];
+    if (rayNearest(p.sv, J.node0, J.bounds, J.eye, d, th_tri, th, cnt)) {
+      triOut[k] = th_tri;
+      const F3 hp = J.eye + d * th;
+      const float4 nn = p.sv.triNV[2 * size_t(th_tri)];
       F3 nrm = f3(nn.x, nn.y, nn.z);
       if (dot(nrm, d) > 0) nrm = -nrm;
       X[3 * k] = hp.x;
@@ -1973,9 +2028,26 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// A resident scene in device format, kept on the host so that the arena can
+// be rebuilt when scenes are added or removed.
+struct SceneRec {
+  bool live = false;
+  std::vector<float4> triNV;
+  std::vector<KdNodeD> nodes;
+  std::vector<Box> boxes;
+  std::vector<uint32_t> parent;
+  std::vector<LeafTri> refs;
+  Box bounds;
+  int numTris = 0;
+  uint32_t numVerts = 0;
+  float offset = 0;
+  uint32_t nodeBase = 0, refBase = 0, triBase = 0, vertBase = 0;  // placement in the arena
+};
+
 class Engine {
  public:
-  DevScene sv;
+  DevScene sv;  // the arena: every resident scene, concatenated
+  std::vector<SceneRec> scenes;
   Options opt;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {};
@@ -1984,7 +2056,7 @@ class Engine {
   void* sceneMem[5] = {};
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
-  DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry, cycles, shq;
+  DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
   WState* hostState = nullptr;  // pinned
   uint64_t phaseCycles[8] = {};
   // per-launch timing (Options::profile): event pairs and the kernel kind
@@ -2034,7 +2106,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq};
+                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -2074,6 +2146,8 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
     return rc;
   p = Params();
   p.sv = e.sv;
+  p.node0 = 0;  // primary and point shadows trace scene 0
+  p.rootBox = e.scenes[0].bounds;
   p.app = app;
   p.R = R;
   p.rootList = nullptr;
@@ -2249,7 +2323,118 @@ int upload(Engine& e, int slot, const T* src, size_t n, const T*& dst, std::stri
 
 // GpuRenderer::GpuRenderer (cuda/gpu_render.cu:489-528): the scene, kd-tree
 // and packed leaf triangles go to device memory once.
-Engine* engineCreate(const btg_scene_desc& d, std::string& err) {
+namespace {
+
+// Device format of a scene: per-triangle plane data, kd nodes, boxes, parent
+// links, and the leaf triangles in kd reference order (buildTriRefs,
+// beam_types.h:100-112).
+SceneRec makeSceneRec(const btg_scene_desc& d, float offset) {
+  SceneRec sr;
+  sr.live = true;
+  sr.triNV.resize(size_t(d.numTris) * 2);
+  for (int64_t t = 0; t < d.numTris; ++t) {
+    const uint32_t v0 = d.tri3[3 * t];
+    sr.triNV[size_t(2 * t)] = make_float4(d.triN[3 * t], d.triN[3 * t + 1], d.triN[3 * t + 2], 0.0f);
+    sr.triNV[size_t(2 * t + 1)] = make_float4(d.pos[3 * v0], d.pos[3 * v0 + 1], d.pos[3 * v0 + 2], 0.0f);
+  }
+  sr.refs.resize(static_cast<size_t>(d.numRefs));
+  for (int64_t i = 0; i < d.numRefs; ++i) {
+    const uint32_t t = d.triIndices[i];
+    LeafTri& r = sr.refs[size_t(i)];
+    for (int k = 0; k < 3; ++k) {
+      r.vi[k] = d.tri3[3 * size_t(t) + k];
+      for (int a = 0; a < 3; ++a) r.v[3 * k + a] = d.pos[3 * size_t(r.vi[k]) + a];
+      r.n[k] = d.triN[3 * size_t(t) + k];
+    }
+    r.tri = int32_t(t);
+  }
+  const KdNodeD* nodes = static_cast<const KdNodeD*>(d.nodes);
+  sr.nodes.assign(nodes, nodes + d.numNodes);
+  sr.boxes.resize(static_cast<size_t>(d.numNodes));
+  for (int64_t i = 0; i < d.numNodes; ++i) {
+    const float* b = d.nodeBox + 6 * i;
+    sr.boxes[size_t(i)] = Box{f3(b[0], b[1], b[2]), f3(b[3], b[4], b[5])};
+  }
+  sr.parent.assign(d.parent, d.parent + d.numNodes);
+  sr.bounds = Box{f3(d.kdBounds[0], d.kdBounds[1], d.kdBounds[2]), f3(d.kdBounds[3], d.kdBounds[4], d.kdBounds[5])};
+  sr.numTris = int(d.numTris);
+  sr.numVerts = uint32_t(d.numVerts);
+  sr.offset = offset;
+  return sr;
+}
+
+// Concatenates the live scenes into the device arena. Node, leaf-reference,
+// triangle and vertex numbers of each scene are shifted by its base; the
+// order of a triangle's vertex numbers (which orients shared edges) is kept.
+int rebuildArena(Engine& e, std::string& err) {
+  uint32_t nodes = 0, refs = 0, tris = 0, verts = 0;
+  for (SceneRec& sr : e.scenes) {
+    if (!sr.live) continue;
+    sr.nodeBase = nodes;
+    sr.refBase = refs;
+    sr.triBase = tris;
+    sr.vertBase = verts;
+    const uint64_t n2 = uint64_t(nodes) + sr.nodes.size(), r2 = uint64_t(refs) + sr.refs.size();
+    const uint64_t t2 = uint64_t(tris) + uint64_t(sr.numTris), v2 = uint64_t(verts) + sr.numVerts;
+    if (n2 >= (1ull << 30) || r2 >= (1ull << 31) || t2 >= (1ull << 31) || v2 >= (1ull << 32)) {
+      err = "resident scenes exceed the arena's 30-bit node / 31-bit triangle numbering";
+      return BTG_ERR_LIMIT_CODE;
+    }
+    nodes = uint32_t(n2);
+    refs = uint32_t(r2);
+    tris = uint32_t(t2);
+    verts = uint32_t(v2);
+  }
+  std::vector<float4> nv;
+  std::vector<KdNodeD> kn;
+  std::vector<Box> bx;
+  std::vector<uint32_t> par;
+  std::vector<LeafTri> lt;
+  nv.reserve(size_t(tris) * 2);
+  kn.reserve(nodes);
+  bx.reserve(nodes);
+  par.reserve(nodes);
+  lt.reserve(refs);
+  for (const SceneRec& sr : e.scenes) {
+    if (!sr.live) continue;
+    nv.insert(nv.end(), sr.triNV.begin(), sr.triNV.end());
+    for (size_t i = 0; i < sr.nodes.size(); ++i) {
+      KdNodeD n = sr.nodes[i];
+      if ((n.data & 3u) != 3u)
+        n.data = (n.data & 3u) | (((n.data >> 2) + sr.nodeBase) << 2);
+      else
+        n.offset += sr.refBase;
+      kn.push_back(n);
+      par.push_back(i == 0 ? sr.nodeBase : sr.parent[i] + sr.nodeBase);
+    }
+    bx.insert(bx.end(), sr.boxes.begin(), sr.boxes.end());
+    for (LeafTri r : sr.refs) {
+      r.tri += int32_t(sr.triBase);
+      for (int k = 0; k < 3; ++k) r.vi[k] += sr.vertBase;
+      lt.push_back(r);
+    }
+  }
+  for (void*& m : e.sceneMem)
+    if (m) {
+      (void)hipFree(m);
+      m = nullptr;
+    }
+  DevScene& sv = e.sv;
+  int rc = upload(e, 0, nv.data(), nv.size(), sv.triNV, err);
+  if (!rc) rc = upload(e, 1, kn.data(), kn.size(), sv.nodes, err);
+  if (!rc) rc = upload(e, 2, bx.data(), bx.size(), sv.nodeBox, err);
+  if (!rc) rc = upload(e, 3, par.data(), par.size(), sv.parent, err);
+  if (!rc) rc = upload(e, 4, lt.data(), lt.size(), sv.refs, err);
+  if (rc) return rc;
+  const SceneRec& s0 = e.scenes[0];
+  sv.bounds = s0.bounds;
+  sv.numTris = s0.numTris;
+  return 0;
+}
+
+}  // namespace
+
+Engine* engineCreate(const btg_scene_desc& d, float offset, std::string& err) {
   Engine* e = new Engine();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -2271,78 +2456,113 @@ Engine* engineCreate(const btg_scene_desc& d, std::string& err) {
       delete e;
       return nullptr;
     }
-  // per-triangle (normal, 0), (vertex 0, 0) for hit planes and shading normals
-  std::vector<float4> nv(size_t(d.numTris) * 2);
-  for (int64_t t = 0; t < d.numTris; ++t) {
-    const uint32_t v0 = d.tri3[3 * t];
-    nv[size_t(2 * t)] = make_float4(d.triN[3 * t], d.triN[3 * t + 1], d.triN[3 * t + 2], 0.0f);
-    nv[size_t(2 * t + 1)] = make_float4(d.pos[3 * v0], d.pos[3 * v0 + 1], d.pos[3 * v0 + 2], 0.0f);
-  }
-  // leaf triangles in kd reference order (buildTriRefs, beam_types.h:100-112)
-  std::vector<LeafTri> refs(static_cast<size_t>(d.numRefs));
-  for (int64_t i = 0; i < d.numRefs; ++i) {
-    const uint32_t t = d.triIndices[i];
-    LeafTri& r = refs[size_t(i)];
-    for (int k = 0; k < 3; ++k) {
-      r.vi[k] = d.tri3[3 * size_t(t) + k];
-      for (int a = 0; a < 3; ++a) r.v[3 * k + a] = d.pos[3 * size_t(r.vi[k]) + a];
-      r.n[k] = d.triN[3 * size_t(t) + k];
-    }
-    r.tri = int32_t(t);
-  }
-  std::vector<Box> boxes(static_cast<size_t>(d.numNodes));
-  for (int64_t i = 0; i < d.numNodes; ++i) {
-    const float* b = d.nodeBox + 6 * i;
-    boxes[size_t(i)] = Box{f3(b[0], b[1], b[2]), f3(b[3], b[4], b[5])};
-  }
-  DevScene& sv = e->sv;
-  int rc = upload(*e, 0, nv.data(), nv.size(), sv.triNV, err);
-  if (!rc) rc = upload(*e, 1, static_cast<const KdNodeD*>(d.nodes), size_t(d.numNodes), sv.nodes, err);
-  if (!rc) rc = upload(*e, 2, boxes.data(), boxes.size(), sv.nodeBox, err);
-  if (!rc) rc = upload(*e, 3, d.parent, size_t(d.numNodes), sv.parent, err);
-  if (!rc) rc = upload(*e, 4, refs.data(), refs.size(), sv.refs, err);
-  if (rc) {
+  e->scenes.push_back(makeSceneRec(d, offset));
+  if (rebuildArena(*e, err)) {
     delete e;
     return nullptr;
   }
-  sv.bounds = Box{f3(d.kdBounds[0], d.kdBounds[1], d.kdBounds[2]), f3(d.kdBounds[3], d.kdBounds[4], d.kdBounds[5])};
-  sv.numTris = int(d.numTris);
   return e;
+}
+
+int engineAddScene(Engine* e, const btg_scene_desc& d, float offset, std::string& err) {
+  int id = -1;
+  for (size_t i = 1; i < e->scenes.size(); ++i)
+    if (!e->scenes[i].live) {
+      id = int(i);
+      break;
+    }
+  if (id < 0) {
+    id = int(e->scenes.size());
+    e->scenes.emplace_back();
+  }
+  e->scenes[size_t(id)] = makeSceneRec(d, offset);
+  BTG_TRY(hipStreamSynchronize(e->stream));
+  if (int rc = rebuildArena(*e, err)) {
+    e->scenes[size_t(id)] = SceneRec();
+    return rc;
+  }
+  return id;
+}
+
+int engineRemoveScene(Engine* e, int scene, std::string& err) {
+  if (scene <= 0 || size_t(scene) >= e->scenes.size() || !e->scenes[size_t(scene)].live) {
+    err = "no resident scene " + std::to_string(scene) + " (scene 0 stays for the context's lifetime)";
+    return BTG_ERR_ARG_CODE;
+  }
+  BTG_TRY(hipStreamSynchronize(e->stream));
+  e->scenes[size_t(scene)] = SceneRec();
+  return rebuildArena(*e, err);
+}
+
+int64_t engineSceneBytes(const Engine* e, int scene) {
+  if (scene < 0 || size_t(scene) >= e->scenes.size() || !e->scenes[size_t(scene)].live) return -1;
+  const SceneRec& sr = e->scenes[size_t(scene)];
+  return int64_t(sr.triNV.size() * sizeof(float4) + sr.nodes.size() * (sizeof(KdNodeD) + sizeof(Box) + 4) +
+                 sr.refs.size() * sizeof(LeafTri));
 }
 
 void engineDestroy(Engine* e) { delete e; }
 
 Options& engineOptions(Engine* e) { return e->opt; }
 
-int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, float offset, bool exact, int32_t* tri, float* vis,
-               float* E, float* Eu, RunInfo& info, std::string& err) {
-  const auto t0 = std::chrono::steady_clock::now();
-  (void)t0;
-  const int W = cam.W, H = cam.H;
-  if (W <= 0 || H <= 0) {
-    err = "image size must be positive";
+int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo& info, std::string& err) {
+  if (n <= 0) {
+    err = "a batch needs at least one job";
     return BTG_ERR_ARG_CODE;
   }
-  const size_t npix = size_t(W) * size_t(H);
-  const int tilesX = (W + 7) / 8, tilesY = (H + 7) / 8;
-  const long long R = (long long)tilesX * tilesY * 64;
-  if (R > (1LL << 30)) {
-    err = "image too large";
-    return BTG_ERR_ARG_CODE;
+  std::vector<SoftJob> jt(static_cast<size_t>(n));
+  long long R = 0, npix = 0, tiles = 0;
+  for (int j = 0; j < n; ++j) {
+    const SoftJobH& h = jobs[j];
+    if (h.scene < 0 || size_t(h.scene) >= e->scenes.size() || !e->scenes[size_t(h.scene)].live) {
+      err = "job " + std::to_string(j) + ": no resident scene " + std::to_string(h.scene);
+      return BTG_ERR_ARG_CODE;
+    }
+    if (h.cam.W <= 0 || h.cam.H <= 0) {
+      err = "job " + std::to_string(j) + ": image size must be positive";
+      return BTG_ERR_ARG_CODE;
+    }
+    const SceneRec& sr = e->scenes[size_t(h.scene)];
+    SoftJob& J = jt[size_t(j)];
+    J.W = h.cam.W;
+    J.H = h.cam.H;
+    J.tilesX = (h.cam.W + 7) / 8;
+    J.node0 = sr.nodeBase;
+    J.triBase = int(sr.triBase);
+    J.bounds = sr.bounds;
+    J.offset = sr.offset;
+    J.halfW = h.cam.halfW;
+    J.halfH = h.cam.halfH;
+    J.eye = h.cam.eye;
+    J.fwd = h.cam.fwd;
+    J.right = h.cam.right;
+    J.down = h.cam.down;
+    J.lc = h.light.c;
+    J.lU = h.light.U;
+    J.lV = h.light.V;
+    J.pixBase = int(npix);
+    J.rootBase = int(R);
+    J.tileBase = int(tiles);
+    npix += (long long)h.cam.W * h.cam.H;
+    R += (long long)J.tilesX * ((h.cam.H + 7) / 8) * 64;
+    tiles += (long long)((h.cam.W + 15) / 16) * ((h.cam.H + 15) / 16);
+    if (R > (1LL << 30) || npix > (1LL << 30)) {
+      err = "batch too large (more than 2^30 pixels)";
+      return BTG_ERR_ARG_CODE;
+    }
   }
   Params p;
   int rc = prepare(*e, kSoft, int(R), e->opt.budget, p, err);
   if (rc) return rc;
-  if ((rc = e->ensure(e->pixTri, npix * 4, err)) || (rc = e->ensure(e->pixX, npix * 12, err)) ||
-      (rc = e->ensure(e->pixN, npix * 12, err)) || (rc = e->ensure(e->vis, npix * 4, err)) ||
-      (rc = e->ensure(e->E, npix * 4, err)) || (rc = e->ensure(e->Eu, npix * 4, err)))
+  const size_t np = size_t(npix);
+  if ((rc = e->ensure(e->pixTri, np * 4, err)) || (rc = e->ensure(e->pixX, np * 12, err)) ||
+      (rc = e->ensure(e->pixN, np * 12, err)) || (rc = e->ensure(e->vis, np * 4, err)) ||
+      (rc = e->ensure(e->E, np * 4, err)) || (rc = e->ensure(e->Eu, np * 4, err)) ||
+      (rc = e->ensure(e->jobs, jt.size() * sizeof(SoftJob), err)))
     return rc;
-  setCamera(cam, p);
-  p.tilesX = tilesX;
-  p.lc = L.c;
-  p.lU = L.U;
-  p.lV = L.V;
-  p.offset = offset;
+  BTG_TRY(hipMemcpyAsync(e->jobs.ptr, jt.data(), jt.size() * sizeof(SoftJob), hipMemcpyHostToDevice, e->stream));
+  p.jobs = static_cast<const SoftJob*>(e->jobs.ptr);
+  p.numJobs = n;
   p.exact = exact ? 1 : 0;
   if (exact) {
     if ((rc = e->ensure(e->shq, size_t(p.capacity) * sizeof(ShadeRec), err))) return rc;
@@ -2356,9 +2576,8 @@ int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, float offset, 
   p.Eu = static_cast<float*>(e->Eu.ptr);
   if ((rc = zeroStats(*e, err))) return rc;
   BTG_TRY(hipEventRecord(e->ev[0], e->stream));
-  dim3 grid((W + 15) / 16, (H + 15) / 16);
   const int lp = e->beginLaunch(3);
-  primaryRayKernel<<<grid, kBlock, 0, e->stream>>>(p);
+  primaryRayKernel<<<unsigned(tiles), kBlock, 0, e->stream>>>(p);
   e->endLaunch(lp);
   BTG_TRY(hipGetLastError());
   BTG_TRY(hipEventRecord(e->ev[1], e->stream));
@@ -2382,12 +2601,31 @@ int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, float offset, 
   if ((rc = readStats(*e, info.stats, &info.dropped, err))) return rc;
   for (int k = 0; k < 8; ++k) info.phaseCycles[k] = e->phaseCycles[k];
   e->collectLaunches(info);
-  if (tri) BTG_TRY(hipMemcpyAsync(tri, e->pixTri.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
-  if (vis) BTG_TRY(hipMemcpyAsync(vis, e->vis.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
-  if (E) BTG_TRY(hipMemcpyAsync(E, e->E.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
-  if (Eu) BTG_TRY(hipMemcpyAsync(Eu, e->Eu.ptr, npix * 4, hipMemcpyDeviceToHost, e->stream));
+  for (int j = 0; j < n; ++j) {
+    const SoftJobH& h = jobs[j];
+    const SoftJob& J = jt[size_t(j)];
+    const size_t off = size_t(J.pixBase), cnt = size_t(J.W) * size_t(J.H);
+    if (h.tri) BTG_TRY(hipMemcpyAsync(h.tri, static_cast<int*>(e->pixTri.ptr) + off, cnt * 4, hipMemcpyDeviceToHost, e->stream));
+    if (h.vis) BTG_TRY(hipMemcpyAsync(h.vis, static_cast<float*>(e->vis.ptr) + off, cnt * 4, hipMemcpyDeviceToHost, e->stream));
+    if (h.E) BTG_TRY(hipMemcpyAsync(h.E, static_cast<float*>(e->E.ptr) + off, cnt * 4, hipMemcpyDeviceToHost, e->stream));
+    if (h.Eu) BTG_TRY(hipMemcpyAsync(h.Eu, static_cast<float*>(e->Eu.ptr) + off, cnt * 4, hipMemcpyDeviceToHost, e->stream));
+  }
   BTG_TRY(hipStreamSynchronize(e->stream));
+  // triangle ids back to each job's scene numbering
+  for (int j = 0; j < n; ++j) {
+    const SoftJobH& h = jobs[j];
+    const int base = jt[size_t(j)].triBase;
+    if (h.tri && base)
+      for (size_t i = 0, c = size_t(h.cam.W) * size_t(h.cam.H); i < c; ++i)
+        if (h.tri[i] >= 0) h.tri[i] -= base;
+  }
   return 0;
+}
+
+int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, bool exact, int32_t* tri, float* vis, float* E,
+               float* Eu, RunInfo& info, std::string& err) {
+  SoftJobH j{0, cam, L, tri, vis, E, Eu};
+  return engineSoftBatch(e, &j, 1, exact, info, err);
 }
 
 namespace {

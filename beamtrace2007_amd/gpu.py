@@ -47,6 +47,15 @@ class btg_area_light(C.Structure):
     _fields_ = [("c", C.c_float * 3), ("U", C.c_float * 3), ("V", C.c_float * 3)]
 
 
+class btg_autotune_trial(C.Structure):
+    _fields_ = [("leafCap", C.c_int), ("budget", C.c_int), ("roundsPerCheck", C.c_int), ("seconds", C.c_double)]
+
+
+class btg_soft_job(C.Structure):
+    _fields_ = [("scene", C.c_int), ("cam", btg_camera), ("light", btg_area_light), ("tri", C.c_void_p),
+                ("vis", C.c_void_p), ("E", C.c_void_p), ("Eu", C.c_void_p)]
+
+
 STAT_FIELDS = ["kdSteps", "leafVisits", "triTests", "hits", "splits", "beams", "rootBeams", "presplitBeams",
                "mailboxSkips", "fiveSplits", "visibleTris", "rays", "climbs"]
 
@@ -67,7 +76,8 @@ assert POLY_DTYPE.itemsize == 44
 
 # Every entry point include/btg.h declares (checked by the tests).
 EXPORTS = ["btg_device_query", "btg_create", "btg_destroy", "btg_last_error", "btg_get_options", "btg_set_options",
-           "btg_soft", "btg_polys_count", "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point"]
+           "btg_soft", "btg_scene_add", "btg_scene_remove", "btg_scene_bytes", "btg_soft_batch", "btg_autotune", "btg_polys_count",
+           "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point"]
 
 
 def lib() -> C.CDLL:
@@ -88,6 +98,13 @@ def lib() -> C.CDLL:
         L.btg_set_options.argtypes = [vp, C.POINTER(btg_options)]
         L.btg_soft.argtypes = [vp, C.POINTER(btg_camera), C.POINTER(btg_area_light), C.c_int, vp, vp, vp, vp,
                                C.POINTER(btg_run_info)]
+        L.btg_scene_add.argtypes = [vp, C.POINTER(btg_scene_desc), C.POINTER(C.c_int)]
+        L.btg_scene_remove.argtypes = [vp, C.c_int]
+        L.btg_scene_bytes.argtypes = [vp, C.c_int]
+        L.btg_scene_bytes.restype = C.c_int64
+        L.btg_soft_batch.argtypes = [vp, C.POINTER(btg_soft_job), C.c_int, C.c_int, C.POINTER(btg_run_info)]
+        L.btg_autotune.argtypes = [vp, C.POINTER(btg_soft_job), C.c_int, C.c_int, C.c_double, C.POINTER(btg_options),
+                                   C.POINTER(btg_autotune_trial), C.c_int, C.POINTER(C.c_int)]
         L.btg_polys_count.argtypes = [vp]
         L.btg_polys_count.restype = C.c_int64
         L.btg_polys_data.argtypes = [vp]
@@ -148,27 +165,97 @@ def dict_to_polys(p: dict) -> np.ndarray:
     return a
 
 
+def _scene_desc(scene: Scene):
+    a = scene.arrays()
+    k = scene.kd_arrays()
+    d = btg_scene_desc()
+    d.pos, d.numVerts = a["pos"].ctypes.data, len(a["pos"])
+    d.tri3, d.triN, d.numTris = a["tris"].ctypes.data, a["triN"].ctypes.data, len(a["tris"])
+    d.nodes, d.numNodes = k["nodes"].ctypes.data, len(k["nodes"])
+    d.triIndices, d.numRefs = k["triIndices"].ctypes.data, len(k["triIndices"])
+    d.parent, d.nodeBox = k["parent"].ctypes.data, k["nodeBox"].ctypes.data
+    d.kdBounds[:] = [float(v) for v in k["bounds"]]
+    d.sceneBounds[:] = [float(v) for v in a["bounds"]]
+    return d, (a, k)
+
+
+def _light(light9) -> btg_area_light:
+    lt = btg_area_light()
+    l9 = np.asarray(light9, np.float32)
+    lt.c[:], lt.U[:], lt.V[:] = [float(v) for v in l9[0:3]], [float(v) for v in l9[3:6]], [float(v) for v in l9[6:9]]
+    return lt
+
+
 class GpuRenderer:
     """The reference's GpuRenderer over one device context. The scene must
     have its kd-tree built (Scene.build_kd)."""
 
     def __init__(self, scene: Scene):
         L = lib()
-        a = scene.arrays()
-        k = scene.kd_arrays()
-        self._keep = (a, k)
-        d = btg_scene_desc()
-        d.pos, d.numVerts = a["pos"].ctypes.data, len(a["pos"])
-        d.tri3, d.triN, d.numTris = a["tris"].ctypes.data, a["triN"].ctypes.data, len(a["tris"])
-        d.nodes, d.numNodes = k["nodes"].ctypes.data, len(k["nodes"])
-        d.triIndices, d.numRefs = k["triIndices"].ctypes.data, len(k["triIndices"])
-        d.parent, d.nodeBox = k["parent"].ctypes.data, k["nodeBox"].ctypes.data
-        d.kdBounds[:] = [float(v) for v in k["bounds"]]
-        d.sceneBounds[:] = [float(v) for v in a["bounds"]]
+        d, keep = _scene_desc(scene)
         h = C.c_void_p()
         if L.btg_create(C.byref(h), C.byref(d)) != 0:
             raise RuntimeError("btg_create: " + L.btg_last_error(None).decode())
         self._h = h
+
+    def add_scene(self, scene: Scene) -> int:
+        """Makes another scene resident (its kd-tree built); returns its id for soft_batch jobs."""
+        d, keep = _scene_desc(scene)
+        sid = C.c_int(-1)
+        rc = lib().btg_scene_add(self._h, C.byref(d), C.byref(sid))
+        if rc:
+            self._err("btg_scene_add", rc)
+        return sid.value
+
+    def remove_scene(self, scene_id: int) -> None:
+        rc = lib().btg_scene_remove(self._h, int(scene_id))
+        if rc:
+            self._err("btg_scene_remove", rc)
+
+    def scene_bytes(self, scene_id: int) -> int:
+        return int(lib().btg_scene_bytes(self._h, int(scene_id)))
+
+    def _jobs(self, jobs, outs=None):
+        arr = (btg_soft_job * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i].scene = int(j.get("scene", 0))
+            arr[i].cam = make_camera(j["cam"], int(j["W"]), int(j["H"]))
+            arr[i].light = _light(j["light"])
+            for f in ("tri", "vis", "E", "Eu"):
+                setattr(arr[i], f, outs[i][f].ctypes.data if outs else None)
+        return arr
+
+    def autotune(self, jobs, exact: bool = True, max_seconds: float = 20.0):
+        """Tunes leaf cap, budget and check interval on the sample batch `jobs`; returns (options dict, trials).
+        The renderer's options are not changed (apply with set_options)."""
+        arr = self._jobs(jobs)
+        best = btg_options()
+        trials = (btg_autotune_trial * 64)()
+        n = C.c_int(0)
+        rc = lib().btg_autotune(self._h, arr, len(jobs), int(exact), float(max_seconds), C.byref(best), trials, 64,
+                                C.byref(n))
+        if rc:
+            self._err("btg_autotune", rc)
+        opts = {"budget": best.budget, "rounds_per_check": best.roundsPerCheck, "leaf_cap": best.leafCap}
+        tr = [{"leaf_cap": t.leafCap, "budget": t.budget, "rounds_per_check": t.roundsPerCheck,
+               "seconds": t.seconds} for t in trials[:min(n.value, 64)]]
+        return opts, tr
+
+    def soft_batch(self, jobs, exact: bool = True, download: bool = True):
+        """Soft shadows for many jobs in one persistent pass. jobs: dicts with scene (id), cam (14 floats),
+        W, H, light (9 floats). Returns (list of per-job dicts with tri/vis/E/Eu, run info)."""
+        n = len(jobs)
+        outs = []
+        for j in jobs:
+            W, H = int(j["W"]), int(j["H"])
+            outs.append({"tri": np.empty((H, W), np.int32), "vis": np.empty((H, W), np.float32),
+                         "E": np.empty((H, W), np.float32), "Eu": np.empty((H, W), np.float32)} if download else {})
+        arr = self._jobs(jobs, outs if download else None)
+        ri = btg_run_info()
+        rc = lib().btg_soft_batch(self._h, arr, n, int(exact), C.byref(ri))
+        if rc:
+            self._err("btg_soft_batch", rc)
+        return outs, info_dict(ri)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -206,9 +293,7 @@ class GpuRenderer:
         n = W * H if download else 0
         tri, vis = np.empty(n, np.int32), np.empty(n, np.float32)
         E, Eu = np.empty(n, np.float32), np.empty(n, np.float32)
-        lt = btg_area_light()
-        l9 = np.asarray(light9, np.float32)
-        lt.c[:], lt.U[:], lt.V[:] = [float(v) for v in l9[0:3]], [float(v) for v in l9[3:6]], [float(v) for v in l9[6:9]]
+        lt = _light(light9)
         ri = btg_run_info()
         ptr = (lambda a: a.ctypes.data) if download else (lambda a: None)
         rc = lib().btg_soft(self._h, C.byref(make_camera(cam14, W, H)), C.byref(lt), int(exact), ptr(tri), ptr(vis),

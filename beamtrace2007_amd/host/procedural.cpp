@@ -10,10 +10,15 @@
 //                          (closed-form soft shadows)
 //   random_tris            interpenetrating random triangles (tests)
 //   sphere                 one UV sphere (crack tests)
+//   blob      (~345K tris) noisy level-7 icosphere with limbs on a ground quad
+//   atrium    (~66K tris)  two-storey arcades around a courtyard, cloth strips
+//   conference (~125K tris) 20 x 12 m hall, 8 tables, 120 chairs
 //
 // The generators are this project's own; only their triangle counts follow
 // the benchmark descriptions. Random numbers come from PCG32 (O'Neill 2014,
 // pcg32_srandom / pcg32_random as published at pcg-random.org).
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <map>
 
@@ -254,6 +259,271 @@ void buildSphere(Scene& s, SceneSpec& spec, int stacks) {
   spec.lightV = V3(0, 0, 1);
 }
 
+
+// ---- blob: batched-config scene A (~345K tris) ------------------------------
+// An icosphere subdivided 7 times (327,680 triangles) pushed in and out by
+// seeded multi-octave radial noise (15% amplitude), with tapered limbs
+// extruded from it, standing on a ground quad.
+struct SphereNoise {
+  struct Wave {
+    V3 d;
+    double f, ph, a;
+  };
+  std::vector<Wave> w;
+  SphereNoise(Pcg32& r, int octaves, int perOctave, double amp) {
+    double a = 1.0, norm = 0;
+    for (int o = 0; o < octaves; ++o, a *= 0.5) norm += a * perOctave;
+    a = 1.0;
+    for (int o = 0; o < octaves; ++o, a *= 0.5)
+      for (int k = 0; k < perOctave; ++k) {
+        const V3 d = unit3(V3(float(r.range(-1, 1)), float(r.range(-1, 1)), float(r.range(-1, 1))));
+        w.push_back({d, 2.0 * double(1 << o) * r.range(0.8, 1.25), r.range(0, 2 * M_PI), amp * a / norm});
+      }
+  }
+  double at(V3 p) const {
+    double v = 0;
+    for (const Wave& x : w) v += x.a * std::sin(x.f * double(dot3(p, x.d)) + x.ph);
+    return v;
+  }
+};
+
+void buildBlob(Scene& s, SceneSpec& spec, uint64_t seed) {
+  Pcg32 r(seed, 11);
+  const float Rb = 0.8f;
+  const V3 centre(0, 1.0f, 0);
+  // icosahedron
+  const float t = float((1.0 + std::sqrt(5.0)) / 2.0);
+  std::vector<V3> v = {V3(-1, t, 0), V3(1, t, 0), V3(-1, -t, 0), V3(1, -t, 0), V3(0, -1, t), V3(0, 1, t),
+                       V3(0, -1, -t), V3(0, 1, -t), V3(t, 0, -1), V3(t, 0, 1), V3(-t, 0, -1), V3(-t, 0, 1)};
+  for (V3& x : v) x = unit3(x);
+  std::vector<std::array<uint32_t, 3>> f = {{0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11},
+                                             {1, 5, 9},  {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8},
+                                             {3, 9, 4},  {3, 4, 2},  {3, 2, 6},   {3, 6, 8},  {3, 8, 9},
+                                             {4, 9, 5},  {2, 4, 11}, {6, 2, 10},  {8, 6, 7},  {9, 8, 1}};
+  for (int level = 0; level < 7; ++level) {
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> mid;
+    auto midpoint = [&](uint32_t a, uint32_t b) {
+      const auto key = std::make_pair(std::min(a, b), std::max(a, b));
+      auto it = mid.find(key);
+      if (it != mid.end()) return it->second;
+      v.push_back(unit3((v[a] + v[b]) * 0.5f));
+      const uint32_t id = uint32_t(v.size() - 1);
+      mid.emplace(key, id);
+      return id;
+    };
+    std::vector<std::array<uint32_t, 3>> g;
+    g.reserve(f.size() * 4);
+    for (const auto& tr : f) {
+      const uint32_t a = midpoint(tr[0], tr[1]), b = midpoint(tr[1], tr[2]), c = midpoint(tr[2], tr[0]);
+      g.push_back({tr[0], a, c});
+      g.push_back({tr[1], b, a});
+      g.push_back({tr[2], c, b});
+      g.push_back({a, b, c});
+    }
+    f.swap(g);
+  }
+  const SphereNoise noise(r, 4, 5, 0.15);
+  const uint32_t base = uint32_t(s.pos.size());
+  for (const V3& d : v) s.vert(centre + d * (Rb * float(1.0 + noise.at(d))));
+  for (const auto& tr : f) s.tri(base + tr[0], base + tr[1], base + tr[2]);
+  // limbs: tapered tubes (24 sides x 60 rings) leaving the surface, capped
+  const int limbs = 6, sides = 24, rings = 60;
+  for (int l = 0; l < limbs; ++l) {
+    V3 d = unit3(V3(float(r.range(-1, 1)), float(r.range(-0.3, 1)), float(r.range(-1, 1))));
+    const V3 bend = unit3(V3(float(r.range(-1, 1)), float(r.range(-1, 1)), float(r.range(-1, 1))));
+    const float len = float(r.range(0.5, 0.9)), rad0 = float(r.range(0.07, 0.12));
+    const V3 start = centre + d * (Rb * float(1.0 + noise.at(d)) * 0.9f);
+    const uint32_t lb = uint32_t(s.pos.size());
+    V3 p = start, dir = d;
+    for (int i = 0; i <= rings; ++i) {
+      const float u = float(i) / float(rings);
+      const V3 e1 = orthoUnit(dir), e2 = cross3(dir, e1);
+      const float rad = rad0 * (1.0f - 0.7f * u);
+      for (int k = 0; k < sides; ++k) {
+        const double a = 2 * M_PI * k / sides;
+        s.vert(p + e1 * float(std::cos(a) * rad) + e2 * float(std::sin(a) * rad));
+      }
+      dir = unit3(dir + bend * (1.2f / float(rings)));
+      p = p + dir * (len / float(rings));
+    }
+    for (int i = 0; i < rings; ++i)
+      for (int k = 0; k < sides; ++k) {
+        const uint32_t a = lb + uint32_t(i * sides + k), b = lb + uint32_t(i * sides + (k + 1) % sides);
+        s.tri(a, b, b + sides);
+        s.tri(a, b + sides, a + sides);
+      }
+    const uint32_t tip = s.vert(p), last = lb + uint32_t(rings * sides);
+    for (int k = 0; k < sides; ++k) s.tri(last + uint32_t(k), last + uint32_t((k + 1) % sides), tip);
+  }
+  quad(s, V3(-4, 0, -4), V3(-4, 0, 4), V3(4, 0, 4), V3(4, 0, -4));
+  spec.view.eye = V3(2.6f, 2.0f, 3.0f);
+  spec.view.target = V3(0.0f, 0.8f, 0.0f);
+  spec.view.fovY = 50;
+  spec.hasPoint = true;
+  spec.pointLight = V3(1.5f, 4.0f, 1.0f);
+  spec.hasArea = true;
+  spec.lightC = V3(0.8f, 4.0f, 0.6f);  // 0.25 m square: light scales 0.25-4 give 6 cm to 1 m
+  spec.lightU = V3(0.25f, 0, 0);
+  spec.lightV = V3(0, 0, 0.25f);
+}
+
+// ---- atrium: batched-config scene B (~66K tris) -----------------------------
+// Two stories of arcades (12 arches on each long side of a 26 x 10 m open
+// courtyard) on 24-segment cylindrical columns, gallery floors, back walls,
+// and cloth strips hanging from the upper gallery.
+void arch(Scene& s, V3 a, V3 b, float rise, float depth, float thick, int seg) {
+  // a semicircular arch band from a to b (horizontal), extruded along z by depth
+  const V3 c = (a + b) * 0.5f, half = (b - a) * 0.5f;
+  const float r0 = len3(half), r1 = r0 + thick;
+  const V3 ex = half / r0, ey(0, 1, 0), ez(0, 0, depth);
+  const uint32_t b0 = uint32_t(s.pos.size());
+  for (int i = 0; i <= seg; ++i) {
+    const double th = M_PI * i / seg;
+    const V3 dir = ex * float(std::cos(th)) + ey * float(std::sin(th) * (rise / r0));
+    s.vert(c + dir * r0);
+    s.vert(c + dir * r1);
+    s.vert(c + dir * r0 + ez);
+    s.vert(c + dir * r1 + ez);
+  }
+  for (int i = 0; i < seg; ++i) {
+    const uint32_t p = b0 + 4 * uint32_t(i), q = p + 4;
+    s.tri(p, q, q + 2);  // intrados
+    s.tri(p, q + 2, p + 2);
+    s.tri(p + 1, p + 3, q + 3);  // extrados
+    s.tri(p + 1, q + 3, q + 1);
+    s.tri(p, p + 1, q + 1);  // front face
+    s.tri(p, q + 1, q);
+    s.tri(p + 2, q + 2, q + 3);  // back face
+    s.tri(p + 2, q + 3, p + 3);
+  }
+}
+
+void buildAtrium(Scene& s, SceneSpec& spec, uint64_t seed) {
+  Pcg32 r(seed, 13);
+  const float L = 26, Wc = 10, gal = 3.0f, storey = 4.5f, colR = 0.25f;
+  const int bays = 12;
+  const float bay = L / float(bays);
+  // ground: courtyard + galleries
+  gridQuad(s, V3(0, 0, -gal), V3(0, 0, Wc + 2 * gal), V3(L, 0, 0), 16, 52);
+  for (int st = 0; st < 2; ++st) {
+    const float y0 = float(st) * storey;
+    for (int side = 0; side < 2; ++side) {
+      const float z = side == 0 ? 0.0f : Wc;
+      for (int i = 0; i <= bays; ++i) {
+        // column: 24 segments, 10 rings
+        const V3 bse(float(i) * bay, y0, z);
+        const int rings = 10;
+        const float h = storey - 1.6f;
+        for (int k = 0; k < rings; ++k)
+          cylinder(s, bse + V3(0, h * float(k) / float(rings), 0), V3(0, h / float(rings), 0), colR, 24, false);
+      }
+      for (int i = 0; i < bays; ++i) {
+        const float x0 = float(i) * bay + colR, x1 = float(i + 1) * bay - colR;
+        const float zz = side == 0 ? -0.3f : Wc - 0.3f;
+        arch(s, V3(x0, y0 + storey - 1.6f, zz), V3(x1, y0 + storey - 1.6f, zz), 1.2f, 0.6f, 0.25f, 24);
+      }
+      // gallery ceiling / upper floor and back wall
+      const float zb = side == 0 ? -gal : Wc + gal;
+      gridQuad(s, V3(0, y0 + storey, side == 0 ? -gal : Wc), V3(L, 0, 0), V3(0, 0, gal), 26, 6);
+      gridQuad(s, V3(0, y0, zb), V3(0, storey, 0), V3(L, 0, 0), 9, 26);
+    }
+  }
+  // cloth strips hanging from the upper gallery edges, gently waving
+  const int strips = 84, cu = 4, cv = 40;
+  for (int k = 0; k < strips; ++k) {
+    const int side = k % 2;
+    const float x = float(r.range(0.5, L - 1.2)), w = float(r.range(0.4, 0.9)), hgt = float(r.range(2.5, 4.0));
+    const float z = side == 0 ? 0.35f : Wc - 0.35f, amp = float(r.range(0.05, 0.25)), ph = float(r.range(0, 6.28));
+    const uint32_t b0 = uint32_t(s.pos.size());
+    for (int j = 0; j <= cv; ++j)
+      for (int i = 0; i <= cu; ++i) {
+        const float u = float(i) / float(cu), v = float(j) / float(cv);
+        const float dz = amp * std::sin(3.0f * v * 3.14159f + ph) * (side == 0 ? 1.0f : -1.0f);
+        s.vert(V3(x + u * w, 2 * storey - 0.1f - v * hgt, z + dz));
+      }
+    for (int j = 0; j < cv; ++j)
+      for (int i = 0; i < cu; ++i) {
+        const uint32_t a = b0 + uint32_t(j * (cu + 1) + i);
+        s.tri(a, a + 1, a + uint32_t(cu + 2));
+        s.tri(a, a + uint32_t(cu + 2), a + uint32_t(cu + 1));
+      }
+  }
+  spec.view.eye = V3(-2.0f, 2.2f, Wc / 2 + 1.0f);
+  spec.view.target = V3(L / 2, 2.8f, Wc / 2);
+  spec.view.fovY = 60;
+  spec.hasPoint = true;
+  spec.pointLight = V3(L / 2, 7.5f, Wc / 2);
+  spec.hasArea = true;
+  spec.lightC = V3(L / 2, 14.0f, Wc / 2 - 2.0f);
+  spec.lightU = V3(3.0f, 0, 0);
+  spec.lightV = V3(0, 0, 3.0f);
+}
+
+// ---- conference: the served-stream scene (~125K tris) -----------------------
+// A 20 x 12 m hall with 8 long tables and 120 instanced chairs (908
+// triangles each) at seeded jittered positions.
+void chair(Scene& s, V3 o, float yaw) {
+  const float c = std::cos(yaw), sn = std::sin(yaw);
+  auto X = [&](V3 p) { return o + V3(c * p.x + sn * p.z, p.y, -sn * p.x + c * p.z); };
+  auto slab = [&](V3 lo, V3 hi, int nu, int nv) {
+    // a box whose top and front faces are finely tessellated (cushions)
+    const V3 d = hi - lo;
+    const uint32_t before = uint32_t(s.pos.size());
+    gridQuad(s, lo + V3(0, d.y, 0), V3(0, 0, d.z), V3(d.x, 0, 0), nv, nu);  // top (+y)
+    gridQuad(s, lo, V3(d.x, 0, 0), V3(0, 0, d.z), 1, 1);                  // bottom
+    gridQuad(s, lo, V3(0, d.y, 0), V3(d.x, 0, 0), 2, nu);                 // front (-z)
+    gridQuad(s, lo + V3(0, 0, d.z), V3(d.x, 0, 0), V3(0, d.y, 0), nu, 2); // back
+    gridQuad(s, lo, V3(0, 0, d.z), V3(0, d.y, 0), nv, 2);                 // left
+    gridQuad(s, lo + V3(d.x, 0, 0), V3(0, d.y, 0), V3(0, 0, d.z), 2, nv); // right
+    for (size_t i = before; i < s.pos.size(); ++i) s.pos[i] = X(s.pos[i]);
+  };
+  slab(V3(-0.23f, 0.42f, -0.22f), V3(0.23f, 0.48f, 0.22f), 10, 10);  // seat
+  slab(V3(-0.23f, 0.48f, 0.18f), V3(0.23f, 0.95f, 0.23f), 12, 2);    // back rest
+  for (int k = 0; k < 4; ++k) {
+    const V3 leg((k & 1) ? 0.19f : -0.19f, 0, (k & 2) ? 0.18f : -0.18f);
+    const uint32_t before = uint32_t(s.pos.size());
+    for (int ring = 0; ring < 4; ++ring)
+      cylinder(s, leg + V3(0, 0.105f * float(ring), 0), V3(0, 0.105f, 0), 0.018f, 12, false);
+    for (size_t i = before; i < s.pos.size(); ++i) s.pos[i] = X(s.pos[i]);
+  }
+}
+
+void buildConference(Scene& s, SceneSpec& spec, uint64_t seed) {
+  Pcg32 r(seed, 17);
+  const float X = 20, Y = 4, Z = 12;
+  gridQuad(s, V3(0, 0, 0), V3(0, 0, Z), V3(X, 0, 0), 48, 60);
+  gridQuad(s, V3(0, Y, 0), V3(X, 0, 0), V3(0, 0, Z), 60, 48);
+  gridQuad(s, V3(0, 0, 0), V3(X, 0, 0), V3(0, Y, 0), 40, 8);
+  gridQuad(s, V3(0, 0, Z), V3(0, Y, 0), V3(X, 0, 0), 8, 40);
+  gridQuad(s, V3(0, 0, 0), V3(0, Y, 0), V3(0, 0, Z), 8, 24);
+  gridQuad(s, V3(X, 0, 0), V3(0, 0, Z), V3(0, Y, 0), 24, 8);
+  // 8 long tables in two columns of four rows, 15 chairs each
+  for (int t = 0; t < 8; ++t) {
+    const float tx = t < 4 ? 2.0f : 11.0f, tz = 1.6f + 2.6f * float(t % 4), tl = 7.0f, tw = 0.9f;
+    gridQuad(s, V3(tx, 0.75f, tz), V3(0, 0, tw), V3(tl, 0, 0), 4, 28);  // top
+    box(s, V3(tx, 0.70f, tz), V3(tx + tl, 0.75f, tz + tw));
+    for (int k = 0; k < 4; ++k) {
+      const float lx = tx + ((k & 1) ? tl - 0.1f : 0.05f), lz = tz + ((k & 2) ? tw - 0.1f : 0.05f);
+      box(s, V3(lx, 0, lz), V3(lx + 0.05f, 0.70f, lz + 0.05f));
+    }
+    for (int c = 0; c < 15; ++c) {
+      const bool front = c < 8;
+      const float cx = tx + 0.5f + float(front ? c : c - 8) * (front ? 0.85f : 0.95f) + float(r.range(-0.12, 0.12));
+      const float cz = front ? tz - 0.45f + float(r.range(-0.1, 0.1)) : tz + tw + 0.45f + float(r.range(-0.1, 0.1));
+      const float yaw = (front ? 3.14159f : 0.0f) + float(r.range(-0.3, 0.3));
+      chair(s, V3(cx, 0, cz), yaw);
+    }
+  }
+  spec.view.eye = V3(0.8f, 1.7f, 0.8f);
+  spec.view.target = V3(10.0f, 0.7f, 6.5f);
+  spec.view.fovY = 65;
+  spec.hasPoint = true;
+  spec.pointLight = V3(10.0f, 3.6f, 6.0f);
+  spec.hasArea = true;
+  spec.lightC = V3(10.0f, 3.95f, 6.0f);
+  spec.lightU = V3(2.0f, 0, 0);
+  spec.lightV = V3(0, 0, 1.0f);
+}
 }  // namespace
 
 bool generate(const std::string& name, uint64_t seed, Scene& s, SceneSpec& spec, std::string& err) {
@@ -273,6 +543,12 @@ bool generate(const std::string& name, uint64_t seed, Scene& s, SceneSpec& spec,
     buildRandomTris(s, spec, seed, 150);
   else if (name == "sphere")
     buildSphere(s, spec, 24);
+  else if (name == "blob")
+    buildBlob(s, spec, seed);
+  else if (name == "atrium")
+    buildAtrium(s, spec, seed);
+  else if (name == "conference")
+    buildConference(s, spec, seed);
   else {
     err = "unknown scene '" + name + "'";
     return false;

@@ -36,6 +36,9 @@ WORKLOADS = {
     "plant": ("plant", 512, "exact soft shadows, seeded procedural plant (curved stem, 292 bent 3x3-grid leaves "
                             "2-6 cm, ground quad; 5378 tris), 0.5x0.5 m area light 1 m above, light-scale 1"),
     "room": ("room", 512, "exact soft shadows, seeded 10x8x3 m room (888 tris), 1x1 m ceiling light"),
+    "batch64": (None, 0, "exact soft shadows, 64 jobs drawn with seed 64: scene A (noisy level-7 icosphere blob with "
+                         "limbs, 345106 tris) or B (two-storey atrium with arches, columns and cloth, 65840 tris) by "
+                         "coin, square side uniform 128..1024, light scale log-uniform 0.25..4"),
 }
 METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request latency"
 # Measured HBM bytes per trace-kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
@@ -88,6 +91,8 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the tracer shards by independent roots and needs no collectives, see DESIGN.md", file=sys.stderr)
         return 2
+    if args.workload == "batch64":
+        return bench_batch(args)
     from beamtrace2007_amd import gpu
     from beamtrace2007_amd.host import Scene, camera
 
@@ -165,6 +170,118 @@ def main() -> int:
     if any(infos[-1]["phase_cycles"].values()):  # instrumented builds only
         out["trace_phase_cycles"] = infos[-1]["phase_cycles"]
     out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_batch(args) -> int:
+    """The batched form: 64 uneven jobs on two resident scenes in one
+    persistent pass per step, against the plain form (one softShadows call per
+    job, one context per scene) on the same jobs."""
+    from beamtrace2007_amd import gpu
+    from beamtrace2007_amd.host import Scene, camera
+    from beamtrace2007_amd.workloads import batch_draw, scaled_light
+
+    dev = gpu.device_info()
+    if dev is None:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    t0 = time.perf_counter()
+    scenes = {}
+    for name in ("blob", "atrium"):
+        sc = Scene.generate(name)
+        sc.build_kd(0.4)
+        scenes[name] = sc
+    build_s = time.perf_counter() - t0
+    draw = batch_draw(64, seed=64)
+    r = gpu.GpuRenderer(scenes["blob"])
+    ids = {"blob": 0, "atrium": r.add_scene(scenes["atrium"])}
+    kw = {"profile": not args.no_profile}
+    if args.budget > 0:
+        kw["budget"] = args.budget
+    if args.capacity > 0:
+        kw["capacity"] = args.capacity
+    if args.leaf_cap >= 0:
+        kw["leaf_cap"] = args.leaf_cap
+    r.set_options(**kw)
+    jobs = []
+    for d in draw:
+        sp = scenes[d["scene"]].spec()
+        jobs.append({"name": d["scene"], "scene": ids[d["scene"]], "cam": camera(sp["view"], d["side"], d["side"]),
+                     "W": d["side"], "H": d["side"], "light": scaled_light(sp["area_light"], d["light_scale"])})
+    px = sum(j["W"] * j["H"] for j in jobs)
+    # warm-up: one small job per scene in both forms
+    small = [dict(j, W=64, H=64, cam=camera(scenes[j["name"]].spec()["view"], 64, 64)) for j in
+             [next(j for j in jobs if j["name"] == n) for n in ("blob", "atrium")]]
+    plain = {n: gpu.GpuRenderer(scenes[n]) for n in scenes}
+    for p_ in plain.values():
+        p_.set_options(**kw)
+    for _ in range(max(1, args.warmup)):
+        r.soft_batch(small, exact=True, download=False)
+        for j in small:
+            plain[j["name"]].soft_shadows(j["cam"], j["W"], j["H"], j["light"], exact=True, download=False)
+    # the schedule for the batched form comes from the autotuner, timed on the
+    # same 64 jobs at a quarter of their side (outside the timed region)
+    sample = [dict(j, W=max(32, j["W"] // 4), H=max(32, j["H"] // 4),
+                   cam=camera(scenes[j["name"]].spec()["view"], max(32, j["W"] // 4), max(32, j["H"] // 4)))
+              for j in jobs]
+    t1 = time.perf_counter()
+    tuned, trials = r.autotune(sample, exact=True, max_seconds=30.0)
+    tune_s = time.perf_counter() - t1
+    print(f"bench.py: autotuned in {tune_s:.1f} s: {tuned}", file=sys.stderr, flush=True)
+    r.set_options(**tuned)
+    # plain form: one call per job, library defaults
+    t0 = time.perf_counter()
+    plain_beams = 0
+    for j in jobs:
+        info = plain[j["name"]].soft_shadows(j["cam"], j["W"], j["H"], j["light"], exact=True, download=False)["info"]
+        plain_beams += info["stats"]["beams"]
+    plain_s = time.perf_counter() - t0
+    for p_ in plain.values():
+        p_.close()
+    print(f"bench.py: plain form {plain_s:.2f} s", file=sys.stderr, flush=True)
+    # batched form: the timed steps
+    infos = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        infos.append(r.soft_batch(jobs, exact=True, download=False)[1])
+        print(f"bench.py: batched step {k + 1}: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    r.close()
+    st = infos[-1]["stats"]
+    b_trace = 12 * (st["kdSteps"] + st["leafVisits"]) + 64 * st["triTests"] + 4 * st["climbs"]
+    trace_s = sum(i["kernel_s"]["trace"] for i in infos)
+    trace_launches = sum(i["launches"]["trace"] for i in infos)
+    roofline = None
+    if trace_s > 0:
+        achieved = b_trace * args.steps / trace_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "traceKernel",
+                    "launches_per_step": trace_launches / args.steps,
+                    "avg_launch_ms": round(trace_s / max(1, trace_launches) * 1e3, 4), "alg_bytes_per_step": int(b_trace)}
+    value = px * args.steps / wall
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 exact integral)",
+        "data": "synthetic (seeded procedural scenes, seeded job draw)",
+        "config": {"workload": f"batch64: {WORKLOADS['batch64'][2]}", "mode": "batched (64 jobs per call)",
+                   "jobs": len(jobs), "pixels_per_step": px, "tris": {n: scenes[n].num_tris for n in scenes},
+                   "draw_seed": 64, "scene_seed": 2007},
+        "device": dev["name"], "arch": dev["arch"],
+        "plain": {"value": round(px / plain_s, 1), "unit": "pixels/s", "seconds": round(plain_s, 3),
+                  "form": "one softShadows call per job"},
+        "batched_over_plain": round(plain_s / (wall / args.steps), 3),
+        "beams_per_s": round(st["beams"] * args.steps / wall, 1),
+        "rounds": infos[-1]["rounds"], "killed_roots": infos[-1]["killed_roots"],
+        "retry_passes": infos[-1]["retry_passes"],
+        "kernel_ms_per_step": {k: round(sum(i["kernel_s"][k] for i in infos) / args.steps * 1e3, 3)
+                               for k in gpu.KERNELS},
+        "roofline": roofline, "host_kd_build_s": round(build_s, 3), "cpu_baseline": None, "parity": None,
+        "autotune": {"chosen": tuned, "seconds": round(tune_s, 2), "trials": trials,
+                     "sample": "the 64 jobs at a quarter of their side"},
+        "draw": [[d["scene"][0], d["side"], round(d["light_scale"], 4)] for d in draw],
+    }
     print(json.dumps(out), flush=True)
     return 0
 

@@ -140,6 +140,51 @@ typedef struct {
 int btg_soft(btg_ctx* ctx, const btg_camera* cam, const btg_area_light* light, int exact, int32_t* tri, float* vis,
              float* E, float* Eu, btg_run_info* info);
 
+/* Resident scenes (no counterpart in the reference, whose GpuRenderer holds
+ * one scene; the batched and served forms of the soft-shadow path need
+ * several resident at once). The scene given to btg_create is scene 0 for
+ * the context's lifetime; btg_scene_add copies another one to the device and
+ * returns its id (>= 1); btg_scene_remove frees it (the id may be reused).
+ * btg_scene_bytes: device bytes the scene occupies (-1: no such scene). */
+int btg_scene_add(btg_ctx* ctx, const btg_scene_desc* scene, int* sceneId);
+int btg_scene_remove(btg_ctx* ctx, int sceneId);
+int64_t btg_scene_bytes(const btg_ctx* ctx, int sceneId);
+
+/* One job of a soft-shadow batch: resident scene, camera, area light, and
+ * outputs as btg_soft (W*H each, any may be NULL). */
+typedef struct {
+  int scene;
+  btg_camera cam;
+  btg_area_light light;
+  int32_t* tri;
+  float* vis;
+  float* E;
+  float* Eu;
+} btg_soft_job;
+
+/* GpuRenderer::softShadows (cuda/gpu_render.h:36) for many jobs at once:
+ * every job's roots share one persistent wavefront pass (one primary-ray
+ * launch, one set of rounds), so small and uneven jobs keep the device full.
+ * Each job's outputs equal those of btg_soft on the same scene, camera and
+ * light. info holds the batch's totals. */
+int btg_soft_batch(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact, btg_run_info* info);
+
+/* Schedule autotuner (no counterpart in the reference, whose knobs are fixed,
+ * cuda/gpu_render.h:56-58 and cuda/gpu_common.cuh:54-58): times the sample
+ * batch `jobs` (typically the real batch at reduced resolution) under the
+ * current options, then tunes one knob at a time - leaf cap, per-root
+ * sub-beam budget, rounds between host checks - keeping each knob's fastest
+ * value, within about maxSeconds. Writes the best options to `best` (the
+ * context's options are left unchanged: apply them with btg_set_options)
+ * and, if `trials` is given, up to maxTrials timed settings. Results do not
+ * depend on these knobs, only speed does. */
+typedef struct {
+  int leafCap, budget, roundsPerCheck;
+  double seconds;
+} btg_autotune_trial;
+int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact, double maxSeconds,
+                 btg_options* best, btg_autotune_trial* trials, int maxTrials, int* numTrials);
+
 /* A polygon in image-plane coordinates (OutBeam, src/beam/beam_types.h:68-73):
  * corners past n repeat the last one. 44 bytes. */
 typedef struct {
