@@ -1,9 +1,14 @@
 // The C ABI of include/btg.h on top of the wavefront engine (engine.hip).
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <exception>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "engine.h"
@@ -432,6 +437,346 @@ int btg_point(btg_ctx* ctx, const btg_camera* cam, const btg_poly* prim, int64_t
   } catch (const std::bad_alloc&) {
     return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
   }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ server
+namespace {
+
+struct SrvReq {
+  uint64_t id = 0;
+  btg_request req{};
+  double tSubmit = 0, tAdmit = 0, tDone = 0;
+  int status = BTG_OK;
+  bool done = false;
+  btg_polys* hits = nullptr;
+  btg_polys* polys = nullptr;
+};
+
+}  // namespace
+
+struct btg_server {
+  btg_ctx* ctx = nullptr;
+  int rounds = 8;
+  int64_t rootCap = 0, pixCap = 0;
+  int jobsPerPass = 0;
+  btg::OpenPass* pass[2] = {nullptr, nullptr};
+  std::unordered_map<int, SrvReq*> slots[2];  // pass slot -> request
+  int active = 0;                             // the pass that takes new requests
+  btg::Engine* pointEngine = nullptr;
+  btg::Engine* fallback = nullptr;            // re-runs a request whose roots went over budget
+  std::mutex m;
+  std::condition_variable cvSoft, cvPoint, cvDone;
+  std::deque<SrvReq*> softQ, pointQ;
+  std::unordered_map<uint64_t, SrvReq*> reqs;
+  uint64_t nextId = 1;
+  bool stop = false;
+  std::string err;
+  std::thread softThread, pointThread;
+};
+
+namespace {
+
+// Marks a request finished (the server's lock is held by the caller).
+void finishLocked(btg_server* s, SrvReq* r, int status, const std::string& why) {
+  r->status = status;
+  r->done = true;
+  r->tDone = now();
+  if (status != BTG_OK) s->err = why;
+  s->cvDone.notify_all();
+}
+
+btg::SoftJobH jobOf(const btg_request& q) {
+  return btg::SoftJobH{q.scene, camOf(q.cam), lightOf(q.light), q.tri, q.vis, q.E, q.Eu};
+}
+
+// Fails every request of pass g and starts the pass afresh.
+void failPass(btg_server* s, int g, int code, const std::string& why) {
+  std::string e2;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    for (auto& kv : s->slots[g]) finishLocked(s, kv.second, code, why);
+    s->slots[g].clear();
+  }
+  btg::openDestroy(s->pass[g]);
+  s->pass[g] = nullptr;
+}
+
+void softLoop(btg_server* s) {
+  std::vector<int> completed, killed;
+  std::string err;
+  for (;;) {
+    bool stepped[2] = {false, false};
+    {
+      std::unique_lock<std::mutex> lk(s->m);
+      // admission: requests join the accepting pass in arrival order while
+      // their roots and pixels fit; a full pass hands over to the other one
+      // once that one is idle
+      while (!s->softQ.empty()) {
+        SrvReq* r = s->softQ.front();
+        btg::OpenPass* acc = s->pass[s->active];
+        if (!acc) break;
+        if (btg::openIdle(acc) && btg::openJobs(acc) > 0) {
+          if (btg::openReset(acc, err)) break;
+          s->slots[s->active].clear();
+        }
+        int slot = -1;
+        const int rc = btg::openTryAdd(acc, jobOf(r->req), &slot, err);
+        if (rc == 1) {
+          s->slots[s->active][slot] = r;
+          r->tAdmit = now();
+          s->softQ.pop_front();
+          continue;
+        }
+        if (rc < 0) {
+          finishLocked(s, r, rc, err);
+          s->softQ.pop_front();
+          continue;
+        }
+        const int o = 1 - s->active;
+        if (s->pass[o] && btg::openIdle(s->pass[o])) {
+          if (btg::openJobs(s->pass[o]) > 0 && btg::openReset(s->pass[o], err)) break;
+          s->slots[o].clear();
+          s->active = o;
+          continue;
+        }
+        break;  // both passes hold work: wait until one drains
+      }
+      const bool busy0 = s->pass[0] && !btg::openIdle(s->pass[0]);
+      const bool busy1 = s->pass[1] && !btg::openIdle(s->pass[1]);
+      if (!busy0 && !busy1) {
+        if (s->stop && s->softQ.empty()) return;
+        s->cvSoft.wait(lk);
+        continue;
+      }
+    }
+    for (int g = 0; g < 2; ++g)
+      if (s->pass[g] && !btg::openIdle(s->pass[g])) {
+        if (int rc = btg::openStep(s->pass[g], s->rounds, err)) {
+          failPass(s, g, rc, err);
+          continue;
+        }
+        stepped[g] = true;
+      }
+    for (int g = 0; g < 2; ++g) {
+      if (!stepped[g]) continue;
+      if (int rc = btg::openCollect(s->pass[g], completed, killed, err)) {
+        failPass(s, g, rc, err);
+        continue;
+      }
+      for (size_t i = 0; i < completed.size(); ++i) {
+        SrvReq* r;
+        {
+          std::lock_guard<std::mutex> lk(s->m);
+          r = s->slots[g][completed[i]];
+          s->slots[g].erase(completed[i]);
+        }
+        int rc = BTG_OK;
+        if (killed[i] > 0) {  // over budget: trace the request again alone, with GPU retries
+          btg::SoftJobH h = jobOf(r->req);
+          btg::RunInfo ri;
+          rc = btg::engineSoftBatch(s->fallback, &h, 1, true, ri, err);
+        }
+        std::lock_guard<std::mutex> lk(s->m);
+        finishLocked(s, r, rc, err);
+      }
+    }
+    // a failed pass is rebuilt before it takes requests again
+    for (int g = 0; g < 2; ++g)
+      if (!s->pass[g]) {
+        btg::OpenPass* op = btg::openCreate(s->ctx->engine, s->rootCap, s->pixCap, s->jobsPerPass, true, err);
+        std::lock_guard<std::mutex> lk(s->m);
+        s->pass[g] = op;
+        if (!op) {  // no pass left to run on: fail what waits
+          for (SrvReq* r : s->softQ) finishLocked(s, r, BTG_ERR_DEVICE, err);
+          s->softQ.clear();
+          return;
+        }
+      }
+  }
+}
+
+void pointLoop(btg_server* s) {
+  for (;;) {
+    SrvReq* r;
+    {
+      std::unique_lock<std::mutex> lk(s->m);
+      s->cvPoint.wait(lk, [&] { return s->stop || !s->pointQ.empty(); });
+      if (s->pointQ.empty()) return;
+      r = s->pointQ.front();
+      s->pointQ.pop_front();
+      r->tAdmit = now();
+    }
+    std::string err;
+    btg_polys* hits = nullptr;
+    btg_polys* polys = nullptr;
+    int rc = BTG_OK;
+    try {
+      hits = new btg_polys();
+      polys = new btg_polys();
+      btg::RunInfo ri, ri2;
+      const btg::HostCam cam = camOf(r->req.cam);
+      rc = btg::enginePrimary(s->pointEngine, cam, false, 16, hits->v, ri, err);
+      if (!rc) {
+        // the point pass takes the primary polygons in the order the reference's
+        // renderer hands them over: canonical
+        std::sort(hits->v.begin(), hits->v.end(), polyLess);
+        rc = btg::enginePoint(s->pointEngine, cam, hits->v.data(), int64_t(hits->v.size()),
+                              btg::f3(r->req.pointLight[0], r->req.pointLight[1], r->req.pointLight[2]), polys->v,
+                              ri2, err);
+      }
+      if (!rc) std::sort(polys->v.begin(), polys->v.end(), polyLess);
+    } catch (const std::bad_alloc&) {
+      rc = BTG_ERR_MEMORY;
+      err = "host allocation failed";
+    }
+    std::lock_guard<std::mutex> lk(s->m);
+    if (rc) {
+      delete hits;
+      delete polys;
+    } else {
+      r->hits = hits;
+      r->polys = polys;
+    }
+    finishLocked(s, r, rc, err);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** out) {
+  if (!ctx || !out) return failWith(ctx, BTG_ERR_ARG, "null argument");
+  *out = nullptr;
+  btg_server_config c{};
+  if (cfg) c = *cfg;
+  const int64_t hbm = c.hbmBytes > 0 ? c.hbmBytes : (int64_t(32) << 30);
+  const int jobs = c.jobsPerPass > 0 ? c.jobsPerPass : 512;
+  const int rounds = c.roundsPerStep > 0 ? c.roundsPerStep : 8;
+  const btg::Options& opt = btg::engineOptions(ctx->engine);
+  // per pass: three sub-beam queues (64 B records) and the shade queue (40 B),
+  // then per pixel 40 B of images and per root ~164 B of state
+  const int64_t queues = int64_t(opt.capacity) * (3 * 64 + 40);
+  const int64_t perPass = hbm / 2 - queues;
+  if (perPass < (int64_t(1) << 26))
+    return failWith(ctx, BTG_ERR_ARG, "hbmBytes too small for two passes at the configured queue capacity");
+  const int64_t pixCap = std::min<int64_t>(int64_t(1) << 28, perPass / (40 + 164));
+  const int64_t rootCap = std::min<int64_t>(int64_t(1) << 30, pixCap + int64_t(jobs) * 16448);
+  try {
+    btg_server* s = new btg_server();
+    s->ctx = ctx;
+    s->rounds = rounds;
+    s->rootCap = rootCap;
+    s->pixCap = pixCap;
+    s->jobsPerPass = jobs;
+    std::string err;
+    for (int g = 0; g < 2 && err.empty(); ++g) {
+      s->pass[g] = btg::openCreate(ctx->engine, rootCap, pixCap, jobs, true, err);
+      if (!s->pass[g] && err.empty()) err = "open pass creation failed";
+    }
+    if (err.empty()) s->pointEngine = btg::engineShare(ctx->engine, err);
+    if (err.empty()) s->fallback = btg::engineShare(ctx->engine, err);
+    if (!err.empty()) {
+      for (auto* op : s->pass) btg::openDestroy(op);
+      btg::engineDestroy(s->pointEngine);
+      btg::engineDestroy(s->fallback);
+      delete s;
+      return failWith(ctx, BTG_ERR_MEMORY, err);
+    }
+    btg::engineOptions(s->pointEngine) = opt;
+    btg::engineOptions(s->fallback) = opt;
+    s->softThread = std::thread(softLoop, s);
+    s->pointThread = std::thread(pointLoop, s);
+    *out = s;
+    return BTG_OK;
+  } catch (const std::exception& ex) {
+    return failWith(ctx, BTG_ERR_MEMORY, std::string("server creation failed: ") + ex.what());
+  }
+}
+
+int btg_server_submit(btg_server* s, const btg_request* req, uint64_t* ticket) {
+  if (!s || !req || !ticket) return BTG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->m);
+  if (s->stop) {
+    s->err = "server is shutting down";
+    return BTG_ERR_ARG;
+  }
+  if ((req->kind != BTG_REQ_SOFT && req->kind != BTG_REQ_POINT) || !validCam(&req->cam) ||
+      (req->kind == BTG_REQ_SOFT && req->exact != 1)) {
+    s->err = "invalid request (kind, camera, or a soft request without exact = 1)";
+    return BTG_ERR_ARG;
+  }
+  try {
+    SrvReq* r = new SrvReq();
+    r->id = s->nextId++;
+    r->req = *req;
+    r->tSubmit = now();
+    s->reqs[r->id] = r;
+    if (req->kind == BTG_REQ_SOFT) {
+      s->softQ.push_back(r);
+      s->cvSoft.notify_one();
+    } else {
+      s->pointQ.push_back(r);
+      s->cvPoint.notify_one();
+    }
+    *ticket = r->id;
+    return BTG_OK;
+  } catch (const std::bad_alloc&) {
+    s->err = "host allocation failed";
+    return BTG_ERR_MEMORY;
+  }
+}
+
+int btg_server_wait(btg_server* s, uint64_t ticket, double timeoutSeconds, btg_request_status* st) {
+  if (!s || !st) return BTG_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->m);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) {
+    s->err = "unknown ticket";
+    return BTG_ERR_ARG;
+  }
+  SrvReq* r = it->second;
+  auto ready = [&] { return r->done; };
+  if (timeoutSeconds < 0)
+    s->cvDone.wait(lk, ready);
+  else
+    s->cvDone.wait_for(lk, std::chrono::duration<double>(timeoutSeconds), ready);
+  std::memset(st, 0, sizeof(*st));
+  if (!r->done) return BTG_OK;
+  st->done = 1;
+  st->status = r->status;
+  st->queueSeconds = (r->tAdmit > 0 ? r->tAdmit : r->tDone) - r->tSubmit;
+  st->latencySeconds = r->tDone - r->tSubmit;
+  st->hits = r->hits;
+  st->polys = r->polys;
+  s->reqs.erase(it);
+  delete r;
+  return BTG_OK;
+}
+
+const char* btg_server_error(btg_server* s) { return s ? s->err.c_str() : ""; }
+
+void btg_server_destroy(btg_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    s->stop = true;
+    s->cvSoft.notify_all();
+    s->cvPoint.notify_all();
+  }
+  if (s->softThread.joinable()) s->softThread.join();
+  if (s->pointThread.joinable()) s->pointThread.join();
+  for (auto* op : s->pass) btg::openDestroy(op);
+  btg::engineDestroy(s->pointEngine);
+  btg::engineDestroy(s->fallback);
+  for (auto& kv : s->reqs) {
+    delete kv.second->hits;
+    delete kv.second->polys;
+    delete kv.second;
+  }
+  delete s;
 }
 
 }  // extern "C"

@@ -116,6 +116,28 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
 // One job on scene 0.
 int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, bool exact, int32_t* tri, float* vis, float* E,
                float* Eu, RunInfo& info, std::string& err);
+// An engine with its own stream and buffers over another engine's resident
+// scenes (which must outlive it and not change while it runs).
+Engine* engineShare(Engine* base, std::string& err);
+
+// Open soft-shadow passes (continuous batching): a persistent wavefront pass
+// to which jobs are appended while it runs. Capacities: roots, pixels and
+// jobs the pass holds until it is reset (after all its jobs finished).
+struct OpenPass;
+OpenPass* openCreate(Engine* base, int64_t rootCap, int64_t pixCap, int jobCap, bool exact, std::string& err);
+void openDestroy(OpenPass* op);
+// Appends a job (launches its primary rays); 1 = added as *slot, 0 = no room
+// left in this pass, negative = error.
+int openTryAdd(OpenPass* op, const SoftJobH& job, int* slot, std::string& err);
+// Launches `rounds` rounds and an asynchronous read-back of the job counters.
+int openStep(OpenPass* op, int rounds, std::string& err);
+// Waits for the last step; finalizes and downloads the jobs that completed
+// since (their slots, and per slot the roots that went over budget).
+int openCollect(OpenPass* op, std::vector<int>& completed, std::vector<int>& killed, std::string& err);
+int openJobs(const OpenPass* op);
+bool openIdle(const OpenPass* op);
+int openReset(OpenPass* op, std::string& err);
+
 // Primary visibility, one root per tile: hit polygons.
 int enginePrimary(Engine* e, const HostCam& cam, bool cull, int tile, std::vector<OutPoly>& out, RunInfo& info,
                   std::string& err);

@@ -129,6 +129,7 @@ struct Params {
   Box rootBox;
   const SoftJob* jobs;  // soft shadows: the batch's jobs, by increasing rootBase
   int numJobs;
+  int* jobLeft;  // open passes: roots of each job not finished yet (null otherwise)
   int app;
   int R;
   const int* rootList;
@@ -927,11 +928,16 @@ __device__ __forceinline__ void emitOutput(const Params& p, const Ctx& c, const 
 }
 
 // ---------------------------------------------------------- root budgets
-__device__ __forceinline__ void rootDone(const Params& p) {
+__device__ int jobOfRoot(const Params& p, int k);
+
+// A root has no live sub-beam left: release its budget reservation and, in an
+// open pass, count it off its job.
+__device__ __forceinline__ void rootDone(const Params& p, int root) {
   atomicAdd(&p.st->reserved, (unsigned long long)(-(long long)p.budget));
+  if (p.jobLeft) atomicSub(&p.jobLeft[jobOfRoot(p, root)], 1);
 }
 __device__ __forceinline__ void liveSub(const Params& p, int root, int n) {
-  if (atomicSub(&p.live[root], n) == n) rootDone(p);
+  if (atomicSub(&p.live[root], n) == n) rootDone(p, root);
 }
 __device__ __forceinline__ void killRoot(const Params& p, int root) {
   if (atomicExch(&p.state[root], int(kKilled)) != int(kKilled)) atomicAdd(&p.st->killed, 1);
@@ -1373,7 +1379,7 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
         slots = 0;
       } else {
         emit = true;
-        if (slots == 0 && old == 1) rootDone(p);
+        if (slots == 0 && old == 1) rootDone(p, r.root);
       }
     }
     const int slot = waveAppend(&p.st->count[nxt], slots);
@@ -1419,6 +1425,23 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
 
 // scheduleKernel (cuda/wavefront.cu:327-345), run by one thread after a
 // round: resets the round's counters and admits roots while their budgets fit.
+// Admits the next roots while their budgets fit in the free queue space
+// (reserved += n * budget, as the reference's scheduleKernel).
+__device__ void admitNext(const Params& p) {
+  WState* s = p.st;
+  volatile WState* vs = s;
+  const unsigned long long reserved = vs->reserved;
+  const unsigned long long freeSlots = p.capacity > reserved ? p.capacity - reserved : 0;
+  long long nAdmit = (long long)(p.R - vs->next);
+  const long long fit = (long long)(freeSlots / (unsigned long long)p.budget);
+  if (fit < nAdmit) nAdmit = fit;
+  if (nAdmit < 0) nAdmit = 0;
+  s->admitBegin = vs->next;
+  s->admitCount = int(nAdmit);
+  s->next = vs->next + int(nAdmit);
+  s->reserved = reserved + (unsigned long long)nAdmit * (unsigned long long)p.budget;
+}
+
 __device__ void scheduleNext(const Params& p, int cur) {
   WState* s = p.st;
   const int nxt = cur ^ 1;
@@ -1436,19 +1459,22 @@ __device__ void scheduleNext(const Params& p, int cur) {
   s->shadeCount = 0;
   s->head[0] = s->head[1] = s->head[2] = 0;
   s->blocksDone = 0;
-  const unsigned long long reserved = vs->reserved;
-  const unsigned long long freeSlots = p.capacity > reserved ? p.capacity - reserved : 0;
-  long long nAdmit = (long long)(p.R - vs->next);
-  const long long fit = (long long)(freeSlots / (unsigned long long)p.budget);
-  if (fit < nAdmit) nAdmit = fit;
-  s->admitBegin = vs->next;
-  s->admitCount = int(nAdmit);
-  s->next = vs->next + int(nAdmit);
-  s->reserved = reserved + (unsigned long long)nAdmit * (unsigned long long)p.budget;
+  admitNext(p);
   const int q = vs->count[nxt];
   if (q > vs->maxQueue) s->maxQueue = q;
-  s->done = (nAdmit == 0 && vs->next >= p.R && q == 0) ? 1 : 0;
+  s->done = (vs->admitCount == 0 && vs->next >= p.R && q == 0) ? 1 : 0;
   s->round = vs->round + 1;
+  __threadfence();
+}
+
+// An open pass whose roots all finished is reopened after jobs were
+// appended (p.R grew): admission resumes at the first root not yet admitted.
+__global__ void reopenKernel(Params p) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  WState* s = p.st;
+  if (!s->done) return;
+  admitNext(p);
+  s->done = s->admitCount == 0 ? 1 : 0;
   __threadfence();
 }
 
@@ -1669,7 +1695,7 @@ __device__ bool pointSetup(const Params& p, F3 nT, F3 P0, int T, const float* px
 }
 
 // The job of batch root k (jobs are ordered by rootBase).
-__device__ __forceinline__ int jobOfRoot(const Params& p, int k) {
+__device__ int jobOfRoot(const Params& p, int k) {
   int lo = 0, hi = p.numJobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1715,7 +1741,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
     bounds = J.bounds;
     int pix;
     if (!softPixel(J, k, pix)) {
-      rootDone(p);
+      rootDone(p, k);
       return;
     }
     const int T = p.pixTri[pix];
@@ -1723,7 +1749,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
       p.vis[pix] = -1;
       p.E[pix] = 0;
       p.Eu[pix] = 0;
-      rootDone(p);
+      rootDone(p, k);
       return;
     }
     const F3 x = f3(p.pixX[3 * pix], p.pixX[3 * pix + 1], p.pixX[3 * pix + 2]);
@@ -1732,7 +1758,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
     if (!lightSetup(J, x, ns, T, R, poly)) {
       p.vis[pix] = 0;
       p.E[pix] = 0;
-      rootDone(p);
+      rootDone(p, k);
       return;
     }
     R.x = x;
@@ -1756,7 +1782,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
     const float4 nn = p.sv.triNV[2 * size_t(T)], vv = p.sv.triNV[2 * size_t(T) + 1];
     if (!pointSetup(p, f3(nn.x, nn.y, nn.z), f3(vv.x, vv.y, vv.z), T, pb.x, pb.y, pb.n, R, poly)) {
       appendPoly(p, k, pb);  // the receiver faces away from the light: all in shadow
-      rootDone(p);
+      rootDone(p, k);
       return;
     }
   }
@@ -1775,7 +1801,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
   admitPoly<App>(p, c, poly, k, cur, ao, cnt);
   p.live[k] = ao.pushed;
   if (!ao.ok || ao.pushed > p.budget) killRoot(p, k);
-  if (ao.pushed == 0) rootDone(p);
+  if (ao.pushed == 0) rootDone(p, k);
 }
 
 template <int App>
@@ -1817,14 +1843,12 @@ __global__ void __launch_bounds__(kBlock) shadeKernel(Params p) {
 // finalizeSoftKernel (cuda/wavefront.cu:539-561) with the exact sums:
 // vis = float(missArea) / lightArea as the CPU computes it
 // (src/render/render.cpp:291).
-__global__ void finalizeSoftKernel(Params p) {
-  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= p.R) return;
-  const int k = p.rootList ? p.rootList[i] : i;
+// Writes root k's pixel; returns its state.
+__device__ __forceinline__ int finalizeRoot(const Params& p, int k) {
   int pix;
-  if (!softPixel(p.jobs[jobOfRoot(p, k)], k, pix)) return;
+  if (!softPixel(p.jobs[jobOfRoot(p, k)], k, pix)) return kNoBeam;
   const int s = p.state[k];
-  if (s != kTraced) return;  // no beam (written at admission) or killed (traced again)
+  if (s != kTraced) return s;  // no beam (written at admission) or killed (traced again)
   const RootQ& R = p.roots[k];
   const double miss = double(p.missFx[k]) * pow2(R.unitExp);
   const float V0 = float(miss) / R.lightArea;
@@ -1832,6 +1856,21 @@ __global__ void finalizeSoftKernel(Params p) {
   const float V = 0.0f < m ? m : 0.0f;
   p.vis[pix] = V;
   p.E[pix] = p.exact ? float(double(p.gFx[k]) * pow2(-60)) : p.Eu[pix] * V;
+  return s;
+}
+
+__global__ void finalizeSoftKernel(Params p) {
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= p.R) return;
+  finalizeRoot(p, p.rootList ? p.rootList[i] : i);
+}
+
+// finalizeSoftKernel over the roots [begin, begin + count) of one job of an
+// open pass, plus the number of them that were abandoned (over budget).
+__global__ void finalizeRangeKernel(Params p, int begin, int count, int* killed) {
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  if (finalizeRoot(p, begin + i) == kKilled) atomicAdd(killed, 1);
 }
 
 __global__ void clearInt(int* x) {
@@ -2626,6 +2665,296 @@ int engineSoft(Engine* e, const HostCam& cam, const HostLight& L, bool exact, in
                float* Eu, RunInfo& info, std::string& err) {
   SoftJobH j{0, cam, L, tri, vis, E, Eu};
   return engineSoftBatch(e, &j, 1, exact, info, err);
+}
+
+// ------------------------------------------------------------ open passes
+Engine* engineShare(Engine* base, std::string& err) {
+  Engine* e = new Engine();
+  e->cus = base->cus;
+  e->opt = base->opt;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&e->hostState), sizeof(WState), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    err = "stream / pinned allocation failed";
+    delete e;
+    return nullptr;
+  }
+  for (hipEvent_t& ev : e->ev)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      err = "event creation failed";
+      delete e;
+      return nullptr;
+    }
+  // the arena stays owned by `base` (sceneMem stays empty here)
+  e->sv = base->sv;
+  e->scenes.resize(base->scenes.size());
+  for (size_t i = 0; i < base->scenes.size(); ++i) {
+    const SceneRec& b = base->scenes[i];
+    SceneRec& r = e->scenes[i];
+    r.live = b.live;
+    r.bounds = b.bounds;
+    r.numTris = b.numTris;
+    r.numVerts = b.numVerts;
+    r.offset = b.offset;
+    r.nodeBase = b.nodeBase;
+    r.refBase = b.refBase;
+    r.triBase = b.triBase;
+    r.vertBase = b.vertBase;
+  }
+  return e;
+}
+
+struct OpenPass {
+  Engine* e = nullptr;  // its own stream and buffers; the arena is shared
+  Params p;
+  int64_t rootCap = 0, pixCap = 0;
+  int jobCap = 0;
+  int64_t pixUsed = 0;
+  int rounds = 0;  // rounds launched since the last reset (the queue parity)
+  std::vector<SoftJob> jobs;
+  std::vector<SoftJobH> hostJobs;
+  std::vector<int> roots;
+  std::vector<char> finished;
+  int* jobLeftHost = nullptr;  // pinned
+  int* killedHost = nullptr;   // pinned, one per job
+  DevBuf jobLeft, killed;
+  hipEvent_t stepDone = nullptr;
+  bool exact = true;
+};
+
+void openDestroy(OpenPass* op) {
+  if (!op) return;
+  if (op->e) (void)hipStreamSynchronize(op->e->stream);
+  if (op->jobLeftHost) (void)hipHostFree(op->jobLeftHost);
+  if (op->killedHost) (void)hipHostFree(op->killedHost);
+  if (op->jobLeft.ptr) (void)hipFree(op->jobLeft.ptr);
+  if (op->killed.ptr) (void)hipFree(op->killed.ptr);
+  if (op->stepDone) (void)hipEventDestroy(op->stepDone);
+  delete op->e;
+  delete op;
+}
+
+namespace {
+int openInit(OpenPass& op, std::string& err) {
+  Engine& e = *op.e;
+  const int budget = op.p.budget;
+  const unsigned long long cap = op.p.capacity;
+  Params& p = op.p;
+  p.R = 0;
+  p.numJobs = 0;
+  p.pass = 0;
+  p.rootList = nullptr;
+  (void)budget;
+  (void)cap;
+  op.pixUsed = 0;
+  op.rounds = 0;
+  op.jobs.clear();
+  op.hostJobs.clear();
+  op.roots.clear();
+  op.finished.clear();
+  initKernel<<<1, 64, 0, e.stream>>>(p);
+  BTG_TRY(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+OpenPass* openCreate(Engine* base, int64_t rootCap, int64_t pixCap, int jobCap, bool exact, std::string& err) {
+  if (rootCap <= 0 || rootCap > (1LL << 30) || pixCap <= 0 || pixCap > (1LL << 30) || jobCap <= 0) {
+    err = "open pass capacities out of range";
+    return nullptr;
+  }
+  OpenPass* op = new OpenPass();
+  op->e = engineShare(base, err);
+  if (!op->e) {
+    delete op;
+    return nullptr;
+  }
+  Engine& e = *op->e;
+  op->rootCap = rootCap;
+  op->pixCap = pixCap;
+  op->jobCap = jobCap;
+  op->exact = exact;
+  Params& p = op->p;
+  int rc = prepare(e, kSoft, int(rootCap), e.opt.budget, p, err);
+  const size_t np = size_t(pixCap);
+  if (!rc && ((rc = e.ensure(e.pixTri, np * 4, err)) || (rc = e.ensure(e.pixX, np * 12, err)) ||
+              (rc = e.ensure(e.pixN, np * 12, err)) || (rc = e.ensure(e.vis, np * 4, err)) ||
+              (rc = e.ensure(e.E, np * 4, err)) || (rc = e.ensure(e.Eu, np * 4, err)) ||
+              (rc = e.ensure(e.jobs, size_t(jobCap) * sizeof(SoftJob), err)) ||
+              (rc = e.ensure(op->jobLeft, size_t(jobCap) * 4, err)) ||
+              (rc = e.ensure(op->killed, size_t(jobCap) * 4, err))))
+    ;
+  if (!rc && exact) rc = e.ensure(e.shq, size_t(p.capacity) * sizeof(ShadeRec), err);
+  if (!rc && (hipHostMalloc(reinterpret_cast<void**>(&op->jobLeftHost), size_t(jobCap) * 4, hipHostMallocDefault) !=
+                  hipSuccess ||
+              hipHostMalloc(reinterpret_cast<void**>(&op->killedHost), size_t(jobCap) * 4, hipHostMallocDefault) !=
+                  hipSuccess ||
+              hipEventCreate(&op->stepDone) != hipSuccess)) {
+    (void)hipGetLastError();
+    err = "pinned allocation failed";
+    rc = BTG_ERR_MEMORY_CODE;
+  }
+  if (rc) {
+    openDestroy(op);
+    return nullptr;
+  }
+  p.jobs = static_cast<const SoftJob*>(e.jobs.ptr);
+  p.jobLeft = static_cast<int*>(op->jobLeft.ptr);
+  p.exact = exact ? 1 : 0;
+  if (exact) p.shq = static_cast<ShadeRec*>(e.shq.ptr);
+  p.pixTri = static_cast<const int*>(e.pixTri.ptr);
+  p.pixX = static_cast<const float*>(e.pixX.ptr);
+  p.pixN = static_cast<const float*>(e.pixN.ptr);
+  p.vis = static_cast<float*>(e.vis.ptr);
+  p.E = static_cast<float*>(e.E.ptr);
+  p.Eu = static_cast<float*>(e.Eu.ptr);
+  op->jobs.reserve(size_t(jobCap));
+  if (zeroStats(e, err) || openInit(*op, err)) {
+    openDestroy(op);
+    return nullptr;
+  }
+  return op;
+}
+
+int openJobs(const OpenPass* op) { return int(op->jobs.size()); }
+
+bool openIdle(const OpenPass* op) {
+  for (char f : op->finished)
+    if (!f) return false;
+  return true;
+}
+
+int openReset(OpenPass* op, std::string& err) {
+  if (!openIdle(op)) {
+    err = "open pass still has jobs in flight";
+    return BTG_ERR_ARG_CODE;
+  }
+  return openInit(*op, err);
+}
+
+int openTryAdd(OpenPass* op, const SoftJobH& h, int* slot, std::string& err) {
+  Engine& e = *op->e;
+  Params& p = op->p;
+  if (h.scene < 0 || size_t(h.scene) >= e.scenes.size() || !e.scenes[size_t(h.scene)].live) {
+    err = "no resident scene " + std::to_string(h.scene);
+    return BTG_ERR_ARG_CODE;
+  }
+  if (h.cam.W <= 0 || h.cam.H <= 0) {
+    err = "image size must be positive";
+    return BTG_ERR_ARG_CODE;
+  }
+  const int tilesX = (h.cam.W + 7) / 8;
+  const int64_t roots = int64_t(tilesX) * ((h.cam.H + 7) / 8) * 64, pix = int64_t(h.cam.W) * h.cam.H;
+  if (roots > op->rootCap || pix > op->pixCap) {
+    err = "request larger than a whole open pass";
+    return BTG_ERR_LIMIT_CODE;
+  }
+  if (p.R + roots > op->rootCap || op->pixUsed + pix > op->pixCap || int(op->jobs.size()) >= op->jobCap) return 0;
+  const SceneRec& sr = e.scenes[size_t(h.scene)];
+  SoftJob J;
+  J.W = h.cam.W;
+  J.H = h.cam.H;
+  J.tilesX = tilesX;
+  J.node0 = sr.nodeBase;
+  J.triBase = int(sr.triBase);
+  J.bounds = sr.bounds;
+  J.offset = sr.offset;
+  J.halfW = h.cam.halfW;
+  J.halfH = h.cam.halfH;
+  J.eye = h.cam.eye;
+  J.fwd = h.cam.fwd;
+  J.right = h.cam.right;
+  J.down = h.cam.down;
+  J.lc = h.light.c;
+  J.lU = h.light.U;
+  J.lV = h.light.V;
+  J.pixBase = int(op->pixUsed);
+  J.rootBase = p.R;
+  J.tileBase = 0;  // the job's primary rays are launched on their own
+  const int j = int(op->jobs.size());
+  op->jobs.push_back(J);  // capacity reserved: the element stays put until the copy has read it
+  op->hostJobs.push_back(h);
+  op->roots.push_back(int(roots));
+  op->finished.push_back(0);
+  op->jobLeftHost[j] = int(roots);
+  BTG_TRY(hipMemcpyAsync(static_cast<SoftJob*>(e.jobs.ptr) + j, &op->jobs[size_t(j)], sizeof(SoftJob),
+                         hipMemcpyHostToDevice, e.stream));
+  BTG_TRY(hipMemcpyAsync(static_cast<int*>(op->jobLeft.ptr) + j, &op->jobLeftHost[j], 4, hipMemcpyHostToDevice,
+                         e.stream));
+  Params q = p;
+  q.jobs = static_cast<const SoftJob*>(e.jobs.ptr) + j;
+  q.numJobs = 1;
+  const int tiles = ((h.cam.W + 15) / 16) * ((h.cam.H + 15) / 16);
+  primaryRayKernel<<<unsigned(tiles), kBlock, 0, e.stream>>>(q);
+  BTG_TRY(hipGetLastError());
+  p.R += int(roots);
+  p.numJobs = j + 1;
+  op->pixUsed += pix;
+  reopenKernel<<<1, 64, 0, e.stream>>>(p);
+  BTG_TRY(hipGetLastError());
+  *slot = j;
+  return 1;
+}
+
+int openStep(OpenPass* op, int rounds, std::string& err) {
+  Engine& e = *op->e;
+  Params& p = op->p;
+  const int gA = residentGrid(e, admitKernel<kSoft>), gT = residentGrid(e, traceKernel<kSoft>),
+            gS = residentGrid(e, splitKernel<kSoft>), gH = residentGrid(e, shadeKernel);
+  for (int i = 0; i < rounds; ++i, ++op->rounds) {
+    const int cur = op->rounds & 1;
+    admitKernel<kSoft><<<gA, kBlock, 0, e.stream>>>(p, cur);
+    traceKernel<kSoft><<<gT, kBlock, 0, e.stream>>>(p, cur);
+    if (p.exact) shadeKernel<<<gH, kBlock, 0, e.stream>>>(p);
+    splitKernel<kSoft><<<gS, kBlock, 0, e.stream>>>(p, cur);
+  }
+  BTG_TRY(hipGetLastError());
+  BTG_TRY(hipMemcpyAsync(e.hostState, p.st, sizeof(WState), hipMemcpyDeviceToHost, e.stream));
+  if (p.numJobs > 0)
+    BTG_TRY(hipMemcpyAsync(op->jobLeftHost, op->jobLeft.ptr, size_t(p.numJobs) * 4, hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipEventRecord(op->stepDone, e.stream));
+  return 0;
+}
+
+int openCollect(OpenPass* op, std::vector<int>& completed, std::vector<int>& killed, std::string& err) {
+  Engine& e = *op->e;
+  Params& p = op->p;
+  completed.clear();
+  killed.clear();
+  BTG_TRY(hipEventSynchronize(op->stepDone));
+  if (e.hostState->overflow) {
+    err = "sub-beam queue overflow in an open pass";
+    return BTG_ERR_LIMIT_CODE;
+  }
+  for (int j = 0; j < int(op->jobs.size()); ++j)
+    if (!op->finished[size_t(j)] && op->jobLeftHost[j] == 0) completed.push_back(j);
+  if (completed.empty()) return 0;
+  BTG_TRY(hipMemsetAsync(op->killed.ptr, 0, op->jobs.size() * 4, e.stream));
+  for (int j : completed) {
+    const SoftJob& J = op->jobs[size_t(j)];
+    const int cnt = op->roots[size_t(j)];
+    finalizeRangeKernel<<<(cnt + 255) / 256, 256, 0, e.stream>>>(p, J.rootBase, cnt,
+                                                               static_cast<int*>(op->killed.ptr) + j);
+    const SoftJobH& h = op->hostJobs[size_t(j)];
+    const size_t off = size_t(J.pixBase), n = size_t(J.W) * size_t(J.H);
+    if (h.tri) BTG_TRY(hipMemcpyAsync(h.tri, static_cast<int*>(e.pixTri.ptr) + off, n * 4, hipMemcpyDeviceToHost, e.stream));
+    if (h.vis) BTG_TRY(hipMemcpyAsync(h.vis, static_cast<float*>(e.vis.ptr) + off, n * 4, hipMemcpyDeviceToHost, e.stream));
+    if (h.E) BTG_TRY(hipMemcpyAsync(h.E, static_cast<float*>(e.E.ptr) + off, n * 4, hipMemcpyDeviceToHost, e.stream));
+    if (h.Eu) BTG_TRY(hipMemcpyAsync(h.Eu, static_cast<float*>(e.Eu.ptr) + off, n * 4, hipMemcpyDeviceToHost, e.stream));
+  }
+  BTG_TRY(hipGetLastError());
+  BTG_TRY(hipMemcpyAsync(op->killedHost, op->killed.ptr, op->jobs.size() * 4, hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipStreamSynchronize(e.stream));
+  for (int j : completed) {
+    op->finished[size_t(j)] = 1;
+    killed.push_back(op->killedHost[j]);
+    const SoftJobH& h = op->hostJobs[size_t(j)];
+    const int base = op->jobs[size_t(j)].triBase;
+    if (h.tri && base)
+      for (size_t i = 0, c = size_t(h.cam.W) * size_t(h.cam.H); i < c; ++i)
+        if (h.tri[i] >= 0) h.tri[i] -= base;
+  }
+  return 0;
 }
 
 namespace {

@@ -51,6 +51,21 @@ class btg_autotune_trial(C.Structure):
     _fields_ = [("leafCap", C.c_int), ("budget", C.c_int), ("roundsPerCheck", C.c_int), ("seconds", C.c_double)]
 
 
+class btg_server_config(C.Structure):
+    _fields_ = [("hbmBytes", C.c_int64), ("roundsPerStep", C.c_int), ("jobsPerPass", C.c_int)]
+
+
+class btg_request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("scene", C.c_int), ("cam", btg_camera), ("light", btg_area_light),
+                ("pointLight", C.c_float * 3), ("exact", C.c_int), ("tri", C.c_void_p), ("vis", C.c_void_p),
+                ("E", C.c_void_p), ("Eu", C.c_void_p)]
+
+
+class btg_request_status(C.Structure):
+    _fields_ = [("done", C.c_int), ("status", C.c_int), ("queueSeconds", C.c_double),
+                ("latencySeconds", C.c_double), ("hits", C.c_void_p), ("polys", C.c_void_p)]
+
+
 class btg_soft_job(C.Structure):
     _fields_ = [("scene", C.c_int), ("cam", btg_camera), ("light", btg_area_light), ("tri", C.c_void_p),
                 ("vis", C.c_void_p), ("E", C.c_void_p), ("Eu", C.c_void_p)]
@@ -76,7 +91,8 @@ assert POLY_DTYPE.itemsize == 44
 
 # Every entry point include/btg.h declares (checked by the tests).
 EXPORTS = ["btg_device_query", "btg_create", "btg_destroy", "btg_last_error", "btg_get_options", "btg_set_options",
-           "btg_soft", "btg_scene_add", "btg_scene_remove", "btg_scene_bytes", "btg_soft_batch", "btg_autotune", "btg_polys_count",
+           "btg_soft", "btg_scene_add", "btg_scene_remove", "btg_scene_bytes", "btg_soft_batch", "btg_autotune", "btg_server_create", "btg_server_submit", "btg_server_wait",
+           "btg_server_error", "btg_server_destroy", "btg_polys_count",
            "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point"]
 
 
@@ -105,6 +121,12 @@ def lib() -> C.CDLL:
         L.btg_soft_batch.argtypes = [vp, C.POINTER(btg_soft_job), C.c_int, C.c_int, C.POINTER(btg_run_info)]
         L.btg_autotune.argtypes = [vp, C.POINTER(btg_soft_job), C.c_int, C.c_int, C.c_double, C.POINTER(btg_options),
                                    C.POINTER(btg_autotune_trial), C.c_int, C.POINTER(C.c_int)]
+        L.btg_server_create.argtypes = [vp, C.POINTER(btg_server_config), C.POINTER(vp)]
+        L.btg_server_submit.argtypes = [vp, C.POINTER(btg_request), C.POINTER(C.c_uint64)]
+        L.btg_server_wait.argtypes = [vp, C.c_uint64, C.c_double, C.POINTER(btg_request_status)]
+        L.btg_server_error.argtypes = [vp]
+        L.btg_server_error.restype = C.c_char_p
+        L.btg_server_destroy.argtypes = [vp]
         L.btg_polys_count.argtypes = [vp]
         L.btg_polys_count.restype = C.c_int64
         L.btg_polys_data.argtypes = [vp]
@@ -333,3 +355,84 @@ class GpuRenderer:
         if rc:
             self._err("btg_point", rc)
         return self._take(ph), info_dict(ri)
+
+
+def _take_polys(ph) -> np.ndarray:
+    L = lib()
+    n = L.btg_polys_count(ph)
+    out = np.zeros(n, POLY_DTYPE)
+    if n:
+        C.memmove(out.ctypes.data, L.btg_polys_data(ph), n * POLY_DTYPE.itemsize)
+    L.btg_polys_free(ph)
+    return out
+
+
+class Server:
+    """Request server over a renderer's context (btg_server_*): soft-shadow
+    requests join a running persistent pass (continuous batching), point-shadow
+    requests overlap on their own stream. The renderer must not be used while
+    the server exists."""
+
+    def __init__(self, renderer: GpuRenderer, hbm_bytes: int = 0, rounds_per_step: int = 0, jobs_per_pass: int = 0):
+        self._r = renderer
+        cfg = btg_server_config(int(hbm_bytes), int(rounds_per_step), int(jobs_per_pass))
+        h = C.c_void_p()
+        rc = lib().btg_server_create(renderer._h, C.byref(cfg), C.byref(h))
+        if rc:
+            renderer._err("btg_server_create", rc)
+        self._h = h
+        self._outs = {}
+
+    def submit_soft(self, scene: int, cam14, W: int, H: int, light9) -> int:
+        o = {"tri": np.empty((H, W), np.int32), "vis": np.empty((H, W), np.float32),
+             "E": np.empty((H, W), np.float32), "Eu": np.empty((H, W), np.float32)}
+        q = btg_request()
+        q.kind, q.scene, q.exact = 0, int(scene), 1
+        q.cam = make_camera(cam14, W, H)
+        q.light = _light(light9)
+        for f in ("tri", "vis", "E", "Eu"):
+            setattr(q, f, o[f].ctypes.data)
+        return self._submit(q, o)
+
+    def submit_point(self, cam14, W: int, H: int, light3) -> int:
+        q = btg_request()
+        q.kind, q.scene = 1, 0
+        q.cam = make_camera(cam14, W, H)
+        q.pointLight[:] = [float(v) for v in light3]
+        return self._submit(q, {})
+
+    def _submit(self, q, outs) -> int:
+        t = C.c_uint64(0)
+        rc = lib().btg_server_submit(self._h, C.byref(q), C.byref(t))
+        if rc:
+            raise RuntimeError(f"btg_server_submit failed ({rc}): " + lib().btg_server_error(self._h).decode())
+        self._outs[t.value] = outs
+        return t.value
+
+    def wait(self, ticket: int, timeout: float = -1.0) -> dict | None:
+        st = btg_request_status()
+        rc = lib().btg_server_wait(self._h, int(ticket), float(timeout), C.byref(st))
+        if rc:
+            raise RuntimeError(f"btg_server_wait failed ({rc}): " + lib().btg_server_error(self._h).decode())
+        if not st.done:
+            return None
+        res = dict(self._outs.pop(ticket))
+        res.update({"status": st.status, "queue_s": st.queueSeconds, "latency_s": st.latencySeconds})
+        if st.hits:
+            res["hits"] = _take_polys(C.c_void_p(st.hits))
+        if st.polys:
+            res["polys"] = _take_polys(C.c_void_p(st.polys))
+        if st.status:
+            res["error"] = lib().btg_server_error(self._h).decode()
+        return res
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().btg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -212,6 +212,63 @@ int btg_primary(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, int can
 int btg_point(btg_ctx* ctx, const btg_camera* cam, const btg_poly* prim, int64_t numPrim, const float* light3,
               int canonical, btg_polys** out, btg_run_info* info);
 
+/* ---------------------------------------------------------------- serving
+ * A request server over one context (no counterpart in the reference, which
+ * renders one job per call): scenes stay resident, requests are admitted by
+ * the device memory their work needs, and soft-shadow requests join a
+ * persistent wavefront pass that is already running (continuous batching:
+ * a request that arrives while others are on the device is added to the
+ * running pass as soon as its roots and pixels fit, and all in-flight
+ * requests' sub-beams share every round). Two such passes alternate, so a
+ * pass's space is reclaimed once its requests are done. Point-shadow
+ * requests run primary visibility and point shadows on their own stream,
+ * overlapping the soft-shadow work. Results equal those of the one-call
+ * entry points on the same inputs.
+ *
+ * While a server exists, the context must not be used by other calls
+ * (its resident scenes must not change). btg_server_submit is thread-safe;
+ * each ticket is waited on once. */
+typedef struct btg_server btg_server;
+
+typedef struct {
+  int64_t hbmBytes;    /* device memory for work in flight, split between two passes (0: 32 GiB) */
+  int roundsPerStep;   /* wavefront rounds between host checks (0: 8) */
+  int jobsPerPass;     /* request slots of one pass (0: 512) */
+} btg_server_config;
+
+enum { BTG_REQ_SOFT = 0, BTG_REQ_POINT = 1 };
+
+typedef struct {
+  int kind;                /* BTG_REQ_SOFT or BTG_REQ_POINT */
+  int scene;               /* soft: resident scene id; point shadows use scene 0 */
+  btg_camera cam;
+  btg_area_light light;    /* soft */
+  float pointLight[3];     /* point */
+  int exact;               /* soft: must be 1 (the server's passes take the exact integral) */
+  int32_t* tri;            /* soft outputs as btg_soft (W*H, any may be NULL); written */
+  float* vis;              /* before the request reports done; they must stay valid */
+  float* E;                /* until then */
+  float* Eu;
+} btg_request;
+
+typedef struct {
+  int done;                /* 1 once finished (successfully or not) */
+  int status;              /* BTG_OK or an error code (message: btg_server_error) */
+  double queueSeconds;     /* submit -> its work admitted to the device */
+  double latencySeconds;   /* submit -> results in host memory */
+  btg_polys* hits;         /* point: primary hit polygons, canonical order; the caller frees them */
+  btg_polys* polys;        /* point: shadowed polygons, canonical order; the caller frees them */
+} btg_request_status;
+
+int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** out);
+int btg_server_submit(btg_server* s, const btg_request* req, uint64_t* ticket);
+/* Waits up to timeoutSeconds (< 0: no limit) for a ticket; status->done = 0 on
+ * timeout. A finished ticket is released by this call. */
+int btg_server_wait(btg_server* s, uint64_t ticket, double timeoutSeconds, btg_request_status* status);
+const char* btg_server_error(btg_server* s);
+/* Finishes the requests in flight, then frees the server. */
+void btg_server_destroy(btg_server* s);
+
 #ifdef __cplusplus
 }
 #endif

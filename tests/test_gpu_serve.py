@@ -1,0 +1,136 @@
+"""The request server (btg_server_*): continuous batching of soft-shadow
+requests into a running persistent pass, point-shadow requests overlapping on
+their own stream. Every result must equal the same request rendered alone
+(bit for bit: the exact sums make per-pixel results independent of what else
+is in flight; polygon lists are compared in canonical order)."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from conftest import load_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def scaled_light(light9, s):
+    L = np.array(light9, np.float32).copy()
+    L[3:] = L[3:] * np.float32(s)
+    return L
+
+
+def soft_alone(name, cam, W, H, light):
+    from beamtrace2007_amd.gpu import GpuRenderer
+    r = GpuRenderer(load_scene(name)[0])
+    out = r.soft_shadows(cam, W, H, light, exact=True)
+    r.close()
+    return out
+
+
+def point_alone(name, cam, W, H, light3):
+    from beamtrace2007_amd.gpu import GpuRenderer
+    r = GpuRenderer(load_scene(name)[0])
+    hits, _ = r.primary(cam, W, H, cull=False, tile=16)
+    shadow, _ = r.point_shadows(cam, W, H, hits, light3)
+    r.close()
+    return hits, shadow
+
+
+def same_polys(a, b):
+    return len(a) == len(b) and all(np.array_equal(a[f], b[f]) for f in ("x", "y", "n", "tri", "area"))
+
+
+@pytest.fixture()
+def server_setup():
+    from beamtrace2007_amd.gpu import GpuRenderer, Server
+    r = GpuRenderer(load_scene("room")[0])
+    plant = r.add_scene(load_scene("plant")[0])
+    srv = Server(r)
+    yield srv, {"room": 0, "plant": plant}
+    srv.close()
+    r.close()
+
+
+def test_mixed_requests_equal_requests_alone(require_gpu, server_setup):
+    from beamtrace2007_amd.host import camera
+    srv, ids = server_setup
+    plan = [("soft", "plant", 64, 48, 1.0), ("point", "room", 48, 48, None), ("soft", "room", 40, 30, 1.0),
+            ("soft", "plant", 96, 96, 2.0), ("point", "room", 64, 32, None), ("soft", "room", 17, 9, 0.5),
+            ("soft", "plant", 33, 65, 0.7), ("point", "room", 16, 16, None)]
+    tickets = []
+    for kind, name, W, H, s in plan:
+        sp = load_scene(name)[3]
+        cam = camera(sp["view"], W, H)
+        if kind == "soft":
+            light = scaled_light(sp["area_light"], s)
+            tickets.append((kind, name, cam, W, H, light, srv.submit_soft(ids[name], cam, W, H, light)))
+        else:
+            tickets.append((kind, name, cam, W, H, sp["point_light"], srv.submit_point(cam, W, H, sp["point_light"])))
+    print("submitted", len(tickets), flush=True)
+    for kind, name, cam, W, H, light, t in tickets:
+        res = srv.wait(t, timeout=120)
+        assert res is not None and res["status"] == 0, res
+        assert res["latency_s"] >= res["queue_s"] >= 0
+        if kind == "soft":
+            a = soft_alone(name, cam, W, H, light)
+            for f in ("tri", "vis", "E", "Eu"):
+                assert np.array_equal(res[f], a[f]), (name, W, H, f)
+        else:
+            hits, shadow = point_alone(name, cam, W, H, light)
+            assert same_polys(res["hits"], hits) and same_polys(res["polys"], shadow), (name, W, H)
+
+
+def test_requests_join_a_running_pass(require_gpu, server_setup):
+    """A small request submitted while a large one is on the device is admitted
+    into the running pass and finishes first."""
+    from beamtrace2007_amd.host import camera
+    srv, ids = server_setup
+    sp = load_scene("plant")[3]
+    big_cam = camera(sp["view"], 384, 384)
+    big = srv.submit_soft(ids["plant"], big_cam, 384, 384, scaled_light(sp["area_light"], 2.0))
+    time.sleep(0.05)
+    small_cam = camera(sp["view"], 24, 24)
+    small = srv.submit_soft(ids["plant"], small_cam, 24, 24, sp["area_light"])
+    rs = srv.wait(small, timeout=120)
+    done_big_early = srv.wait(big, timeout=0)
+    rb = done_big_early or srv.wait(big, timeout=300)
+    assert rs["status"] == 0 and rb["status"] == 0
+    assert done_big_early is None, "the small request should not wait for the large one"
+    a = soft_alone("plant", small_cam, 24, 24, sp["area_light"])
+    assert np.array_equal(rs["vis"], a["vis"])
+
+
+def test_passes_alternate_when_full(require_gpu):
+    """With room for only a few requests per pass, the server fills one pass,
+    hands over to the other, and reuses each once drained; results stay exact."""
+    from beamtrace2007_amd.gpu import GpuRenderer, Server
+    from beamtrace2007_amd.host import camera
+    r = GpuRenderer(load_scene("plant")[0])
+    r.set_options(capacity=1 << 20)
+    # per pass: ~0.23 GiB of queues and ~2M pixels of images and root state
+    srv = Server(r, hbm_bytes=int(2 * (0.25 * 2**30 + 2.2e6 * 204)), jobs_per_pass=16)
+    sp = load_scene("plant")[3]
+    reqs = []
+    lock = threading.Lock()
+
+    def submitter():
+        for k in range(10):
+            side = 300 + 37 * k
+            cam = camera(sp["view"], side, side)
+            t = srv.submit_soft(0, cam, side, side, sp["area_light"])
+            with lock:
+                reqs.append((cam, side, t))
+            time.sleep(0.01)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    for cam, side, t in reqs:
+        res = srv.wait(t, timeout=300)
+        assert res is not None and res["status"] == 0, res
+        a = r_alone = soft_alone("plant", cam, side, side, sp["area_light"])
+        assert np.array_equal(res["vis"], a["vis"]) and np.array_equal(res["E"], a["E"]), side
+        del r_alone
+    srv.close()
+    r.close()
