@@ -1,21 +1,33 @@
-"""Benchmark: exact soft shadows on one MI355X (the plain, single-job form).
+"""Benchmark: exact soft shadows on one MI355X, batched vs plain, with
+served-stream latency.
 
-    python bench.py [--steps K] [--warmup W] [--workload plant] [--res 512]
+    python bench.py [--workload batch64|plant|room|serve] [--steps K] [--warmup W]
 
-One step = one exact soft-shadow frame through the GPU path (GPU primary-ray
-pass + wavefront shadow beams, as the reference times it: pixels / (primary_s
-+ trace_s), scripts/make_report.py:171), with the scene and kd-tree resident
-in HBM and the images left in device memory (no PCIe in the timed region).
-Prints one JSON line. Besides the headline value it reports:
+batch64 (default): one step = one batch of 64 uneven exact soft-shadow jobs
+on two resident scenes (a ~345K-triangle noisy blob and a ~66K-triangle
+atrium, sides 128..1024, light scales 0.25..4, seed 64) traced in one
+persistent wavefront pass, with the schedule chosen by the autotuner on a
+scaled-down copy of the batch. `value` = batch pixels / wall time; the plain
+form (one softShadows call per job) is timed on the same jobs beside it. The
+same line carries a served mixed stream (`serving`: Poisson arrivals of soft
+and point-shadow requests on a ~125K-triangle conference hall through the
+request server, p50/p99 latency) and the reference CPU build on a bounded
+sample of the batch.
+
+plant / room: one step = one exact soft-shadow frame of one job (GPU
+primary-ray pass + wavefront shadow beams, as the reference times it:
+pixels / (primary_s + trace_s), scripts/make_report.py:171), with a parity
+band against the oracle and the error against the double build.
+
+Scenes and trees are resident in HBM and images stay in device memory (no
+PCIe in the timed region). The line also reports:
   roofline      the trace kernel's algorithmic bytes (one 12-B kd node per
                 inner step and leaf visit, one 64-B leaf triangle per test,
                 one 4-B parent link per climb) over its HIP-event time,
                 against MI355X's 8 TB/s HBM3E;
   cpu_baseline  the reference's own CPU build (oracle/_ref, compiled from the
-                reference sources) on a band of rows of the same frame, all
-                host threads (up to 16), its default options (mailbox on);
-  parity        GPU vs the oracle on a band of rows (bit-exact fraction,
-                max |vis diff|) and the error against the double build.
+                reference sources) on a bounded sample of the same work, all
+                host threads (up to 16), its default options (mailbox on).
 """
 from __future__ import annotations
 
@@ -36,6 +48,9 @@ WORKLOADS = {
     "plant": ("plant", 512, "exact soft shadows, seeded procedural plant (curved stem, 292 bent 3x3-grid leaves "
                             "2-6 cm, ground quad; 5378 tris), 0.5x0.5 m area light 1 m above, light-scale 1"),
     "room": ("room", 512, "exact soft shadows, seeded 10x8x3 m room (888 tris), 1x1 m ceiling light"),
+    "serve": (None, 0, "served mixed stream on the seeded conference hall (20x12 m, 8 tables, 120 chairs; 124800 "
+                       "tris): Poisson arrivals (seed 5), 70% exact soft shadows / 30% point shadows, square side "
+                       "uniform 256..1024"),
     "batch64": (None, 0, "exact soft shadows, 64 jobs drawn with seed 64: scene A (noisy level-7 icosphere blob with "
                          "limbs, 345106 tris) or B (two-storey atrium with arches, columns and cloth, 65840 tris) by "
                          "coin, square side uniform 128..1024, light scale log-uniform 0.25..4"),
@@ -50,9 +65,10 @@ TRAFFIC = {("plant", 512): 196594220}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plant", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 2 for batch64, 10 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default: 1 for batch64, 2 otherwise)")
+    ap.add_argument("--workload", default="batch64", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-serve", action="store_true", help="batch64: skip the served-stream measurement")
     ap.add_argument("--res", type=int, default=0)
     ap.add_argument("--cpu-stride", type=int, default=8, help="the CPU baseline traces every k-th row of the frame")
     ap.add_argument("--no-cpu", action="store_true")
@@ -60,10 +76,17 @@ def parse():
     ap.add_argument("--budget", type=int, default=0, help="live sub-beams per root (0: library default)")
     ap.add_argument("--capacity", type=int, default=0, help="sub-beam records per queue (0: library default)")
     ap.add_argument("--leaf-cap", type=int, default=-1, help="leaves per sub-beam per trace launch (-1: default)")
+    ap.add_argument("--qps", type=float, default=3.0, help="serve: request arrival rate")
+    ap.add_argument("--requests", type=int, default=60, help="serve: requests in the stream")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (the known-size copy that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and print their size")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.steps is None:
+        a.steps = 2 if a.workload == "batch64" else 10
+    if a.warmup is None:
+        a.warmup = 1 if a.workload == "batch64" else 2
+    return a
 
 
 def copy_calibration(mib: int) -> int:
@@ -93,6 +116,12 @@ def main() -> int:
         return 2
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "serve":
+        res = bench_serve(args)
+        if res is None:
+            return 1
+        print(json.dumps(res), flush=True)
+        return 0
     from beamtrace2007_amd import gpu
     from beamtrace2007_amd.host import Scene, camera
 
@@ -226,7 +255,7 @@ def bench_batch(args) -> int:
                    cam=camera(scenes[j["name"]].spec()["view"], max(32, j["W"] // 4), max(32, j["H"] // 4)))
               for j in jobs]
     t1 = time.perf_counter()
-    tuned, trials = r.autotune(sample, exact=True, max_seconds=30.0)
+    tuned, trials = r.autotune(sample, exact=True, max_seconds=15.0)
     tune_s = time.perf_counter() - t1
     print(f"bench.py: autotuned in {tune_s:.1f} s: {tuned}", file=sys.stderr, flush=True)
     r.set_options(**tuned)
@@ -277,13 +306,125 @@ def bench_batch(args) -> int:
         "retry_passes": infos[-1]["retry_passes"],
         "kernel_ms_per_step": {k: round(sum(i["kernel_s"][k] for i in infos) / args.steps * 1e3, 3)
                                for k in gpu.KERNELS},
-        "roofline": roofline, "host_kd_build_s": round(build_s, 3), "cpu_baseline": None, "parity": None,
+        "roofline": roofline, "host_kd_build_s": round(build_s, 3),
+        "cpu_baseline": None if args.no_cpu else cpu_batch_sample(scenes, jobs, draw),
         "autotune": {"chosen": tuned, "seconds": round(tune_s, 2), "trials": trials,
                      "sample": "the 64 jobs at a quarter of their side"},
         "draw": [[d["scene"][0], d["side"], round(d["light_scale"], 4)] for d in draw],
     }
+    if not args.no_serve:
+        out["serving"] = bench_serve(args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def cpu_batch_sample(scenes, jobs, draw, budget_s: float = 20.0) -> dict | None:
+    """The reference CPU build on a bounded sample of the batch: the middle row
+    of each job, in draw order, until about budget_s seconds of tracing."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refpy import Ref, ref_available
+
+    if not ref_available("f"):
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    refs = {}
+    px = 0
+    secs = 0.0
+    used = 0
+    for j, d in zip(jobs, draw):
+        name = j["name"]
+        if name not in refs:
+            a = scenes[name].arrays()
+            refs[name] = Ref("f").load(a["pos"], a["tris"])
+        sp = scenes[name].spec()
+        y = j["H"] // 2
+        res = refs[name].soft_rows(sp["view"], j["W"], j["H"], j["light"], y, y + 1, exact=True, mailbox=True,
+                                   trail=False, threads=threads)
+        secs += float(res["seconds"][0] + res["seconds"][1])
+        px += j["W"]
+        used += 1
+        if secs >= budget_s:
+            break
+    for rf in refs.values():
+        rf.free()
+    print(f"bench.py: CPU reference sample {used} rows, {secs:.1f} s", file=sys.stderr, flush=True)
+    return {"value": round(px / secs, 1), "unit": "pixels/s", "cores": threads, "kind": "reference",
+            "sample": f"the middle row of the first {used} of the 64 jobs ({px} px, {secs:.2f} s: primary rays + "
+                      "shadow beams, mailbox on, frame stack)"}
+
+
+def bench_serve(args) -> dict | None:
+    """A Poisson stream of mixed requests through the request server: p50/p99
+    latency (submit -> results in host memory) and pixels/s over the stream."""
+    import threading
+
+    from beamtrace2007_amd import gpu
+    from beamtrace2007_amd.host import Scene, camera
+    from beamtrace2007_amd.workloads import served_draw
+
+    if gpu.device_info() is None:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return None
+    sc = Scene.generate("conference")
+    sc.build_kd(0.4)
+    sp = sc.spec()
+    r = gpu.GpuRenderer(sc)
+    # warm-up outside the stream: one small request of each kind
+    srv = gpu.Server(r)
+    for t in (srv.submit_soft(0, camera(sp["view"], 64, 64), 64, 64, sp["area_light"]),
+              srv.submit_point(camera(sp["view"], 64, 64), 64, 64, sp["point_light"])):
+        srv.wait(t)
+    draw = served_draw(args.requests, args.qps, seed=5)
+    tickets = [None] * len(draw)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, d in enumerate(draw):
+            delay = t0 + d["t"] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            cam = camera(sp["view"], d["side"], d["side"])
+            if d["kind"] == "soft":
+                tickets[i] = srv.submit_soft(0, cam, d["side"], d["side"], sp["area_light"])
+            else:
+                tickets[i] = srv.submit_point(cam, d["side"], d["side"], sp["point_light"])
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat = {"soft": [], "point": []}
+    queue = []
+    last_report = time.perf_counter()
+    for i, d in enumerate(draw):
+        while tickets[i] is None:
+            time.sleep(0.001)
+        res = srv.wait(tickets[i])
+        if res["status"] != 0:
+            print(f"bench.py: request {i} failed: {res.get('error')}", file=sys.stderr)
+            srv.close()
+            return None
+        lat[d["kind"]].append(res["latency_s"])
+        queue.append(res["queue_s"])
+        if time.perf_counter() - last_report > 10:
+            print(f"bench.py: served {i + 1}/{len(draw)}", file=sys.stderr, flush=True)
+            last_report = time.perf_counter()
+    th.join()
+    span = time.perf_counter() - t0
+    srv.close()
+    r.close()
+    allv = np.array(lat["soft"] + lat["point"])
+    px = sum(d["side"] ** 2 for d in draw)
+
+    def pct(v, q):
+        return round(float(np.percentile(np.array(v), q)) * 1e3, 2) if len(v) else None
+
+    return {"requests": len(draw), "qps": args.qps, "arrival_seed": 5, "seconds": round(span, 3),
+            "pixels": px, "pixels_per_s": round(px / span, 1),
+            "latency_ms": {"p50": pct(allv, 50), "p99": pct(allv, 99), "soft_p50": pct(lat["soft"], 50),
+                           "soft_p99": pct(lat["soft"], 99), "point_p50": pct(lat["point"], 50),
+                           "point_p99": pct(lat["point"], 99)},
+            "queue_ms": {"p50": pct(queue, 50), "p99": pct(queue, 99)},
+            "mix": {"soft": len(lat["soft"]), "point": len(lat["point"])},
+            "config": WORKLOADS["serve"][2]}
 
 
 def cpu_and_parity(args, s, sp, cam, W, H, light, full) -> dict:
