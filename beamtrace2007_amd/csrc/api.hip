@@ -167,6 +167,14 @@ float sceneOffset(const btg_scene_desc* d) {
   return btg::length(diag) * btg::kEpsRel;
 }
 
+// The double build's apex offset (kEpsRel = 1e-9, src/core/real.h:31-34), used
+// by FP64-guarded soft shadows.
+float sceneOffsetGuard(const btg_scene_desc* d) {
+  const btg::F3 diag = btg::f3(d->sceneBounds[3] - d->sceneBounds[0], d->sceneBounds[4] - d->sceneBounds[1],
+                               d->sceneBounds[5] - d->sceneBounds[2]);
+  return btg::length(diag) * btg::kEpsRelDouble;
+}
+
 btg::HostLight lightOf(const btg_area_light& l) {
   btg::HostLight L;
   L.c = btg::f3(l.c[0], l.c[1], l.c[2]);
@@ -187,7 +195,7 @@ int btg_create(btg_ctx** out, const btg_scene_desc* d) {
     btg_ctx* c = new btg_ctx();
     std::string err;
     c->offset = sceneOffset(d);
-    c->engine = btg::engineCreate(*d, c->offset, err);
+    c->engine = btg::engineCreate(*d, c->offset, sceneOffsetGuard(d), err);
     if (!c->engine) {
       delete c;
       return failWith(nullptr, BTG_ERR_DEVICE, err);
@@ -205,7 +213,7 @@ int btg_scene_add(btg_ctx* ctx, const btg_scene_desc* d, int* sceneId) {
   std::string why;
   if (!validScene(d, why)) return failWith(ctx, BTG_ERR_ARG, why);
   try {
-    const int id = btg::engineAddScene(ctx->engine, *d, sceneOffset(d), ctx->err);
+    const int id = btg::engineAddScene(ctx->engine, *d, sceneOffset(d), sceneOffsetGuard(d), ctx->err);
     if (id < 0) return id;
     *sceneId = id;
     return BTG_OK;
@@ -238,13 +246,14 @@ int btg_get_options(const btg_ctx* ctx, btg_options* o) {
   o->roundsPerCheck = x.roundsPerCheck;
   o->profileKernels = x.profile ? 1 : 0;
   o->leafCap = x.leafCap;
+  o->precision = x.precision;
   return BTG_OK;
 }
 
 int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   if (!ctx || !o) return BTG_ERR_ARG;
   if (o->budget < 1 || o->primaryBudget < 1 || o->capacity < 1024 || o->capacity > (int64_t(1) << 31) ||
-      o->roundsPerCheck < 1 || o->leafCap < 0)
+      o->roundsPerCheck < 1 || o->leafCap < 0 || o->precision < 0 || o->precision > 1)
     return failWith(ctx, BTG_ERR_ARG, "invalid options");
   btg::Options& x = btg::engineOptions(ctx->engine);
   x.budget = o->budget;
@@ -253,6 +262,7 @@ int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   x.roundsPerCheck = o->roundsPerCheck;
   x.profile = o->profileKernels != 0;
   x.leafCap = o->leafCap;
+  x.precision = o->precision;
   return BTG_OK;
 }
 
@@ -362,6 +372,7 @@ int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact,
   best->roundsPerCheck = cur.roundsPerCheck;
   best->profileKernels = start.profile ? 1 : 0;
   best->leafCap = cur.leafCap;
+  best->precision = start.precision;
   return BTG_OK;
 }
 

@@ -21,6 +21,7 @@
 namespace btg {
 
 constexpr float kEpsRel = 2e-5f;   // fuzzy split epsilon / beam extent (src/core/real.h:36)
+constexpr float kEpsRelDouble = 1e-9f;  // the double build's (real.h:31): FP64-guarded mode
 constexpr float kKdRel = 1e-5f;    // kd decision margin (real.h:37)
 constexpr float kDegRel = 2e-6f;   // edge-on detection (real.h:38)
 constexpr float kCopCos = 2e-6f;   // coplanar tie (src/beam/beam_core.h:26-27)

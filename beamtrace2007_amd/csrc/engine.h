@@ -64,6 +64,7 @@ struct Options {
   int64_t capacity = int64_t(1) << 25;
   int roundsPerCheck = 8;
   int leafCap = 8;
+  int precision = 0;  // soft shadows: 0 float build (bit-exact), 1 FP64-guarded
   bool profile = false;
 };
 
@@ -88,14 +89,15 @@ class Engine;
 
 // Creates the engine's device state for a scene (copies the arrays); the
 // scene becomes resident scene 0. `offset` is the scene's soft-shadow apex
-// offset (scene extent x the split epsilon).
-Engine* engineCreate(const btg_scene_desc& scene, float offset, std::string& err);
+// offset (scene extent x the split epsilon), `offsetGuard` the double
+// build's (FP64-guarded mode).
+Engine* engineCreate(const btg_scene_desc& scene, float offset, float offsetGuard, std::string& err);
 void engineDestroy(Engine* e);
 Options& engineOptions(Engine* e);
 
 // Adds a resident scene (copies the arrays); returns its id >= 1, or a
 // negative error code. Removing frees the id for reuse.
-int engineAddScene(Engine* e, const btg_scene_desc& scene, float offset, std::string& err);
+int engineAddScene(Engine* e, const btg_scene_desc& scene, float offset, float offsetGuard, std::string& err);
 int engineRemoveScene(Engine* e, int scene, std::string& err);
 // Device bytes a resident scene occupies (-1: no such scene).
 int64_t engineSceneBytes(const Engine* e, int scene);

@@ -2079,7 +2079,7 @@ struct SceneRec {
   Box bounds;
   int numTris = 0;
   uint32_t numVerts = 0;
-  float offset = 0;
+  float offset = 0, offsetGuard = 0;
   uint32_t nodeBase = 0, refBase = 0, triBase = 0, vertBase = 0;  // placement in the arena
 };
 
@@ -2367,7 +2367,7 @@ namespace {
 // Device format of a scene: per-triangle plane data, kd nodes, boxes, parent
 // links, and the leaf triangles in kd reference order (buildTriRefs,
 // beam_types.h:100-112).
-SceneRec makeSceneRec(const btg_scene_desc& d, float offset) {
+SceneRec makeSceneRec(const btg_scene_desc& d, float offset, float offsetGuard) {
   SceneRec sr;
   sr.live = true;
   sr.triNV.resize(size_t(d.numTris) * 2);
@@ -2399,6 +2399,7 @@ SceneRec makeSceneRec(const btg_scene_desc& d, float offset) {
   sr.numTris = int(d.numTris);
   sr.numVerts = uint32_t(d.numVerts);
   sr.offset = offset;
+  sr.offsetGuard = offsetGuard;
   return sr;
 }
 
@@ -2473,7 +2474,7 @@ int rebuildArena(Engine& e, std::string& err) {
 
 }  // namespace
 
-Engine* engineCreate(const btg_scene_desc& d, float offset, std::string& err) {
+Engine* engineCreate(const btg_scene_desc& d, float offset, float offsetGuard, std::string& err) {
   Engine* e = new Engine();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -2495,7 +2496,7 @@ Engine* engineCreate(const btg_scene_desc& d, float offset, std::string& err) {
       delete e;
       return nullptr;
     }
-  e->scenes.push_back(makeSceneRec(d, offset));
+  e->scenes.push_back(makeSceneRec(d, offset, offsetGuard));
   if (rebuildArena(*e, err)) {
     delete e;
     return nullptr;
@@ -2503,7 +2504,7 @@ Engine* engineCreate(const btg_scene_desc& d, float offset, std::string& err) {
   return e;
 }
 
-int engineAddScene(Engine* e, const btg_scene_desc& d, float offset, std::string& err) {
+int engineAddScene(Engine* e, const btg_scene_desc& d, float offset, float offsetGuard, std::string& err) {
   int id = -1;
   for (size_t i = 1; i < e->scenes.size(); ++i)
     if (!e->scenes[i].live) {
@@ -2514,7 +2515,7 @@ int engineAddScene(Engine* e, const btg_scene_desc& d, float offset, std::string
     id = int(e->scenes.size());
     e->scenes.emplace_back();
   }
-  e->scenes[size_t(id)] = makeSceneRec(d, offset);
+  e->scenes[size_t(id)] = makeSceneRec(d, offset, offsetGuard);
   BTG_TRY(hipStreamSynchronize(e->stream));
   if (int rc = rebuildArena(*e, err)) {
     e->scenes[size_t(id)] = SceneRec();
@@ -2569,7 +2570,7 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
     J.node0 = sr.nodeBase;
     J.triBase = int(sr.triBase);
     J.bounds = sr.bounds;
-    J.offset = sr.offset;
+    J.offset = e->opt.precision ? sr.offsetGuard : sr.offset;
     J.halfW = h.cam.halfW;
     J.halfH = h.cam.halfH;
     J.eye = h.cam.eye;
@@ -2696,6 +2697,7 @@ Engine* engineShare(Engine* base, std::string& err) {
     r.numTris = b.numTris;
     r.numVerts = b.numVerts;
     r.offset = b.offset;
+    r.offsetGuard = b.offsetGuard;
     r.nodeBase = b.nodeBase;
     r.refBase = b.refBase;
     r.triBase = b.triBase;
@@ -2858,7 +2860,7 @@ int openTryAdd(OpenPass* op, const SoftJobH& h, int* slot, std::string& err) {
   J.node0 = sr.nodeBase;
   J.triBase = int(sr.triBase);
   J.bounds = sr.bounds;
-  J.offset = sr.offset;
+  J.offset = e.opt.precision ? sr.offsetGuard : sr.offset;
   J.halfW = h.cam.halfW;
   J.halfH = h.cam.halfH;
   J.eye = h.cam.eye;

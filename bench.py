@@ -70,7 +70,7 @@ def parse():
     ap.add_argument("--workload", default="batch64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-serve", action="store_true", help="batch64: skip the served-stream measurement")
     ap.add_argument("--res", type=int, default=0)
-    ap.add_argument("--cpu-stride", type=int, default=8, help="the CPU baseline traces every k-th row of the frame")
+    ap.add_argument("--cpu-stride", type=int, default=1, help="plant / room: the CPU baseline traces every k-th row")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-launch HIP events")
     ap.add_argument("--budget", type=int, default=0, help="live sub-beams per root (0: library default)")
@@ -181,6 +181,8 @@ def main() -> int:
     t1 = time.perf_counter()
     full = r.soft_shadows(cam, W, H, light, exact=True, download=True)
     pcie_s = time.perf_counter() - t1
+    r.set_options(precision=1)
+    guarded = r.soft_shadows(cam, W, H, light, exact=True, download=True)
     r.close()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
@@ -198,7 +200,7 @@ def main() -> int:
     }
     if any(infos[-1]["phase_cycles"].values()):  # instrumented builds only
         out["trace_phase_cycles"] = infos[-1]["phase_cycles"]
-    out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full))
+    out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded))
     print(json.dumps(out), flush=True)
     return 0
 
@@ -276,6 +278,7 @@ def bench_batch(args) -> int:
         infos.append(r.soft_batch(jobs, exact=True, download=False)[1])
         print(f"bench.py: batched step {k + 1}: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
+    precision = None if args.no_cpu else batch_precision(r, scenes, jobs, px)
     r.close()
     st = infos[-1]["stats"]
     b_trace = 12 * (st["kdSteps"] + st["leafVisits"]) + 64 * st["triTests"] + 4 * st["climbs"]
@@ -308,6 +311,7 @@ def bench_batch(args) -> int:
                                for k in gpu.KERNELS},
         "roofline": roofline, "host_kd_build_s": round(build_s, 3),
         "cpu_baseline": None if args.no_cpu else cpu_batch_sample(scenes, jobs, draw),
+        "precision": precision,
         "autotune": {"chosen": tuned, "seconds": round(tune_s, 2), "trials": trials,
                      "sample": "the 64 jobs at a quarter of their side"},
         "draw": [[d["scene"][0], d["side"], round(d["light_scale"], 4)] for d in draw],
@@ -318,9 +322,64 @@ def bench_batch(args) -> int:
     return 0
 
 
+def batch_precision(r, scenes, jobs, px) -> dict | None:
+    """Error against the reference's double build, stated for both modes on a
+    row sample (the middle row of the first two jobs of each scene), and the
+    FP64-guarded mode's batched throughput on the full batch."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refpy import Ref, ref_available
+
+    saved = r.options
+    r.set_options(precision=1)
+    t0 = time.perf_counter()
+    r.soft_batch(jobs, exact=True, download=False)
+    guarded_s = time.perf_counter() - t0
+    r.set_options(**saved)
+    print(f"bench.py: guarded batch {guarded_s:.2f} s", file=sys.stderr, flush=True)
+    out = {"guarded": {"value": round(px / guarded_s, 1), "unit": "pixels/s", "seconds": round(guarded_s, 3),
+                       "mode": "precision 1: the double build's apex offset, FP64 Lambert integral"}}
+    if not ref_available("d"):
+        return out
+    sample = [j for n in ("blob", "atrium") for j in [x for x in jobs if x["name"] == n][:2]]
+    res = {}
+    for mode in (0, 1):
+        r.set_options(precision=mode)
+        res[mode] = r.soft_batch(sample, exact=True, download=True)[0]
+    r.set_options(**saved)
+    errs = {0: [], 1: []}
+    threads = max(1, min(16, os.cpu_count() or 1))
+    refs = {}
+    for j, o0, o1 in zip(sample, res[0], res[1]):
+        name = j["name"]
+        if name not in refs:
+            a = scenes[name].arrays()
+            refs[name] = Ref("d").load(a["pos"], a["tris"])
+        y = j["H"] // 2
+        d = refs[name].soft_rows(scenes[name].spec()["view"], j["W"], j["H"], j["light"], y, y + 1, exact=True,
+                                 mailbox=False, trail=True, threads=threads)["vis"][0]
+        for mode, o in ((0, o0), (1, o1)):
+            g = o["vis"][y].astype(np.float64)
+            m = (d >= 0) & (g >= 0)
+            errs[mode].append(np.abs(g - d)[m])
+    for rf in refs.values():
+        rf.free()
+
+    def summary(e):
+        e = np.concatenate(e)
+        return {"max": float(e.max()), "mean": float(e.mean()), "frac_above_1e-5": float(np.mean(e > 1e-5)),
+                "pixels": int(e.size)}
+
+    out["vis_err_vs_fp64"] = {"float_build": summary(errs[0]), "guarded": summary(errs[1]),
+                              "sample": "middle rows of the first two jobs of each scene, against bt_render_d's "
+                                        "algorithm (the reference compiled with Real = double)"}
+    return out
+
+
 def cpu_batch_sample(scenes, jobs, draw, budget_s: float = 20.0) -> dict | None:
-    """The reference CPU build on a bounded sample of the batch: the middle row
-    of each job, in draw order, until about budget_s seconds of tracing."""
+    """The reference CPU build on a bounded sample of the batch: rows at
+    fixed fractions of each job's height (1/2, 1/4, 3/4, 1/8, ...), one
+    fraction across all 64 jobs at a time, until about budget_s seconds of
+    tracing."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from refpy import Ref, ref_available
 
@@ -328,29 +387,32 @@ def cpu_batch_sample(scenes, jobs, draw, budget_s: float = 20.0) -> dict | None:
         return None
     threads = max(1, min(16, os.cpu_count() or 1))
     refs = {}
-    px = 0
+    px = rows = 0
     secs = 0.0
-    used = 0
-    for j, d in zip(jobs, draw):
-        name = j["name"]
-        if name not in refs:
-            a = scenes[name].arrays()
-            refs[name] = Ref("f").load(a["pos"], a["tris"])
-        sp = scenes[name].spec()
-        y = j["H"] // 2
-        res = refs[name].soft_rows(sp["view"], j["W"], j["H"], j["light"], y, y + 1, exact=True, mailbox=True,
-                                   trail=False, threads=threads)
-        secs += float(res["seconds"][0] + res["seconds"][1])
-        px += j["W"]
-        used += 1
+    fracs = [0.5, 0.25, 0.75, 0.125, 0.375, 0.625, 0.875]
+    for f in fracs:
+        for j in jobs:
+            name = j["name"]
+            if name not in refs:
+                a = scenes[name].arrays()
+                refs[name] = Ref("f").load(a["pos"], a["tris"])
+            sp = scenes[name].spec()
+            y = min(j["H"] - 1, int(j["H"] * f))
+            res = refs[name].soft_rows(sp["view"], j["W"], j["H"], j["light"], y, y + 1, exact=True, mailbox=True,
+                                       trail=False, threads=threads)
+            secs += float(res["seconds"][0] + res["seconds"][1])
+            px += j["W"]
+            rows += 1
+            if secs >= budget_s:
+                break
         if secs >= budget_s:
             break
     for rf in refs.values():
         rf.free()
-    print(f"bench.py: CPU reference sample {used} rows, {secs:.1f} s", file=sys.stderr, flush=True)
+    print(f"bench.py: CPU reference sample {rows} rows, {secs:.1f} s", file=sys.stderr, flush=True)
     return {"value": round(px / secs, 1), "unit": "pixels/s", "cores": threads, "kind": "reference",
-            "sample": f"the middle row of the first {used} of the 64 jobs ({px} px, {secs:.2f} s: primary rays + "
-                      "shadow beams, mailbox on, frame stack)"}
+            "sample": f"{rows} rows at fixed height fractions across the 64 jobs ({px} px, {secs:.2f} s: primary rays "
+                      "+ shadow beams, mailbox on, frame stack)"}
 
 
 def bench_serve(args) -> dict | None:
@@ -427,7 +489,7 @@ def bench_serve(args) -> dict | None:
             "config": WORKLOADS["serve"][2]}
 
 
-def cpu_and_parity(args, s, sp, cam, W, H, light, full) -> dict:
+def cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded=None) -> dict:
     """The CPU baseline and the parity band (test oracles, never the product)."""
     res = {"cpu_baseline": None, "parity": None}
     if args.no_cpu:
@@ -463,10 +525,15 @@ def cpu_and_parity(args, s, sp, cam, W, H, light, full) -> dict:
         refd = Ref("d").load(a["pos"], a["tris"])
         d = refd.soft_rows(sp["view"], W, H, light, py0, py1, exact=True, mailbox=False, trail=True, threads=threads)
         m = d["vis"] >= 0
-        err = np.abs(gv.astype(np.float64) - d["vis"])[m & (gv >= 0)]
-        par["vis_err_vs_fp64"] = {"max": float(err.max()) if err.size else 0.0,
-                                  "mean": float(err.mean()) if err.size else 0.0,
-                                  "frac_above_1e-5": float(np.mean(err > 1e-5)) if err.size else 0.0}
+
+        def err_of(v):
+            err = np.abs(v.astype(np.float64) - d["vis"])[m & (v >= 0)]
+            return {"max": float(err.max()) if err.size else 0.0, "mean": float(err.mean()) if err.size else 0.0,
+                    "frac_above_1e-5": float(np.mean(err > 1e-5)) if err.size else 0.0}
+
+        par["vis_err_vs_fp64"] = err_of(gv)
+        if guarded is not None:
+            par["vis_err_vs_fp64_guarded"] = err_of(guarded["vis"][py0:py1])
         refd.free()
     res["parity"] = par
     return res

@@ -1165,9 +1165,9 @@ float lightVis(Tracer& tr, const Light& L, V3 x, V3 ns, int tri, float offset, f
   return std::max(0.0f, std::min(1.0f, V));
 }
 
-float softOffset(const orc_scene& S) {
+float softOffset(const orc_scene& S, bool guard) {
   const V3 d = sub(S.sceneBounds.hi, S.sceneBounds.lo);
-  return len(d) * kEps;
+  return len(d) * (guard ? 1e-9f : kEps);  // the double build's kEpsRel when guarded (real.h:31)
 }
 
 // pointShadowSetup (point_core.h:15-36).
@@ -1274,10 +1274,10 @@ void orc_primary_rays(const orc_scene* s, const float* cam14, int W, int H, int3
 }
 
 void orc_soft(const orc_scene* s, const float* cam14, int W, int H, const float* light9, int exact, int threads,
-              int y0, int y1, int32_t* tri, float* vis, float* E, float* Eu, double* stats) {
+              int y0, int y1, int32_t* tri, float* vis, float* E, float* Eu, double* stats, int guard) {
   const Cam c = camOf(cam14, W, H);
   const Light L = lightOf(light9);
-  const float offset = softOffset(*s);
+  const float offset = softOffset(*s, guard != 0);
   const float A = L.area();
   if (y1 <= 0) {
     y0 = 0;

@@ -221,7 +221,7 @@ class Oracle:
             L.orc_scene_new.argtypes = [fp, i64, u32p, fp, i64, vp, i64, u32p, i64, u32p, fp, fp, fp]
             L.orc_scene_free.argtypes = [vp]
             L.orc_soft.argtypes = [vp, fp, C.c_int, C.c_int, fp, C.c_int, C.c_int, C.c_int, C.c_int, i32p, fp, fp, fp,
-                                   dp]
+                                   dp, C.c_int]
             L.orc_primary.restype = vp
             L.orc_primary.argtypes = [vp, fp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
             L.orc_point.restype = vp
@@ -253,13 +253,14 @@ class Oracle:
         except Exception:
             pass
 
-    def soft(self, cam14, W, H, light9, exact=True, threads=8, rows=None):
+    def soft(self, cam14, W, H, light9, exact=True, threads=8, rows=None, guard=False):
         n = W * H
         tri, vis = np.full(n, -1, np.int32), np.full(n, -1, np.float32)
         E, Eu, st = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(14)
         y0, y1 = rows if rows is not None else (0, 0)
         self.L.orc_soft(self.h, _f(cam14), W, H, _f(light9), int(exact), threads, y0, y1, tri.ctypes.data_as(i32p),
-                        vis.ctypes.data_as(fp), E.ctypes.data_as(fp), Eu.ctypes.data_as(fp), st.ctypes.data_as(dp))
+                        vis.ctypes.data_as(fp), E.ctypes.data_as(fp), Eu.ctypes.data_as(fp), st.ctypes.data_as(dp),
+                        int(guard))
         return {"tri": tri.reshape(H, W), "vis": vis.reshape(H, W), "E": E.reshape(H, W), "Eu": Eu.reshape(H, W),
                 "stats": stats_dict(st)}
 

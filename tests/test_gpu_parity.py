@@ -176,3 +176,23 @@ def test_invalid_arguments_are_errors_not_crashes(require_gpu):
         r.point_shadows(cam, 16, 16, bad, sp["point_light"])
     empty, _ = r.point_shadows(cam, 16, 16, bad[:0], sp["point_light"])
     assert len(empty) == 0
+
+
+@pytest.mark.parametrize("name", ["room", "plant", "random_tris"])
+def test_guarded_soft_shadows_match_guarded_oracle(require_gpu, name):
+    """precision = 1 (the double build's apex offset) on the GPU equals the
+    oracle's guarded variant; its error against the double build is checked
+    on the CPU (test_oracle.py)."""
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene(name)
+    W, H = 72, 56
+    cam = camera(sp["view"], W, H)
+    r = renderer(name)
+    saved = r.options
+    r.set_options(precision=1)
+    try:
+        g = r.soft_shadows(cam, W, H, sp["area_light"], exact=True)
+    finally:
+        r.set_options(**saved)
+    o = oracle(name).soft(cam, W, H, sp["area_light"], exact=True, guard=True)
+    check_soft(g, o)

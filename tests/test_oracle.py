@@ -164,3 +164,30 @@ def test_oracle_exact_and_eq2_agree_when_unoccluded():
     assert lit.sum() > 100
     rel = np.abs(o["E"][lit] - o["Eu"][lit]) / o["Eu"][lit]
     assert np.median(rel) < 0.05
+
+
+@pytest.mark.parametrize("name,max_err,max_frac", [("room", 1e-5, 0.0), ("random_tris", 1e-4, 0.02),
+                                                   ("plant", 5e-3, 0.05)])
+def test_guarded_soft_shadows_approach_the_double_build(name, max_err, max_frac):
+    """The FP64-guarded mode (the double build's apex offset) against the
+    reference's double build: errors shrink from the float build's (up to
+    ~4e-2 on the plant, half of the pixels above 1e-5) to the bounds here."""
+    if not ref_available("d"):
+        pytest.skip("reference double build not built")
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene(name)
+    W = H = 48
+    cam = camera(sp["view"], W, H)
+    refd = Ref("d").load(a["pos"], a["tris"])
+    d = refd.soft(sp["view"], W, H, sp["area_light"], exact=True, mailbox=False, trail=True, threads=8)
+    refd.free()
+    o = Oracle(a, k)
+    err = {}
+    for guard in (False, True):
+        g = o.soft(cam, W, H, sp["area_light"], exact=True, guard=guard)
+        assert np.array_equal(g["tri"], d["tri"])
+        m = (d["vis"] >= 0) & (g["vis"] >= 0)
+        err[guard] = np.abs(g["vis"].astype(np.float64) - d["vis"])[m]
+    assert err[True].max() <= max_err
+    assert np.mean(err[True] > 1e-5) <= max_frac
+    assert err[True].mean() <= err[False].mean()
