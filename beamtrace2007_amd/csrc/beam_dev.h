@@ -408,16 +408,19 @@ struct Plane {
     coords(O + r * (dot(n, d) / den), qx, qy);
     return true;
   }
-  BTG_HD static Plane make(F3 O, F3 p0, F3 u_, F3 v_) {
+  // Olo: the low part of an apex held as the float pair O + Olo (the
+  // FP64-guarded soft mode); zero otherwise, where (p0 - O) - 0 is p0 - O.
+  BTG_HD static Plane make(F3 O, F3 p0, F3 u_, F3 v_, F3 Olo = F3{0, 0, 0}) {
     Plane P;
     P.O = O;
     P.p0 = p0;
+    const F3 d = (p0 - O) - Olo;
     F3 n = normalize(cross(u_, v_));
-    if (dot(n, p0 - O) < 0) n = -n;
+    if (dot(n, d) < 0) n = -n;
     P.u = normalize(u_);
     P.v = cross(n, P.u);
     P.n = n;
-    P.d = p0 - O;
+    P.d = d;
     return P;
   }
 };

@@ -90,7 +90,8 @@ class Engine;
 // Creates the engine's device state for a scene (copies the arrays); the
 // scene becomes resident scene 0. `offset` is the scene's soft-shadow apex
 // offset (scene extent x the split epsilon), `offsetGuard` the double
-// build's (FP64-guarded mode).
+// build's (FP64-guarded mode: with the hit point in double and the apex as a
+// float pair).
 Engine* engineCreate(const btg_scene_desc& scene, float offset, float offsetGuard, std::string& err);
 void engineDestroy(Engine* e);
 Options& engineOptions(Engine* e);

@@ -43,7 +43,10 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 
-enum App { kSoft = 0, kPrimary = 1, kPoint = 2 };
+// kSoftG: soft shadows in the FP64-guarded mode (double hit point, the
+// double build's apex offset, the apex held as a float pair).
+enum App { kSoft = 0, kPrimary = 1, kPoint = 2, kSoftG = 3 };
+constexpr bool isSoft(int app) { return app == kSoft || app == kSoftG; }
 enum RootState { kTraced = 0, kKilled = 1, kNoBeam = 2 };
 enum { kNear = 0, kFar = 1, kBoth = 2 };
 enum { kKeep = 0, kAllIn = 1, kClip = 2 };
@@ -79,6 +82,7 @@ struct RootQ {
   int tri;      // point: receiver triangle
   float lightArea;
   F3 x, ns;     // soft: shading point and normal
+  F3 Olo;       // guarded soft: low part of the apex (plane.O + Olo)
 };
 
 struct WState {
@@ -158,6 +162,7 @@ struct Params {
   int exact;
   const int* pixTri;
   const float* pixX;
+  const float* pixXlo;  // guarded soft: low parts of the double hit points (null otherwise)
   const float* pixN;
   float* vis;
   float* E;
@@ -232,6 +237,7 @@ __device__ void flushCnt(const Params& p, const Cnt& c) {
 // ------------------------------------------------------------ beam context
 struct Ctx {
   Plane P;
+  F3 Olo;  // low part of the apex; a compile-time zero outside kSoftG
   float ext, eps, minArea, farCq;
   int flags, exclude;
   __device__ bool anyHit() const { return flags & fAnyHit; }
@@ -241,6 +247,7 @@ struct Ctx {
 __device__ __forceinline__ Ctx ctxOf(const RootQ& q) {
   Ctx c;
   c.P = q.plane;
+  c.Olo = f3(0, 0, 0);
   c.ext = q.ext;
   c.eps = q.eps;
   c.minArea = q.minArea;
@@ -260,8 +267,15 @@ template <int App>
 __device__ __forceinline__ Ctx modeCtx(const RootQ& q) {
   Ctx c = ctxOf(q);
   c.flags = App == kPrimary ? (q.flags & fCull) : (fAnyHit | fFarIsPlane);
+  if (App == kSoftG) c.Olo = q.Olo;
   return c;
 }
+
+// Vectors and coordinates relative to the apex: (p - O) - Olo. Outside the
+// guarded mode Olo is the constant +0, and x - (+0) is x for every float x,
+// so these are p - O exactly (and compile to it).
+__device__ __forceinline__ F3 relApex(const Ctx& c, F3 p) { return (p - c.P.O) - c.Olo; }
+__device__ __forceinline__ float relApex(const Ctx& c, float s, int a) { return (s - comp(c.P.O, a)) - comp(c.Olo, a); }
 
 // BeamCtx::setup (beam_core.h:123-132).
 __device__ __forceinline__ void setupQuery(RootQ& q, float extent) {
@@ -313,8 +327,7 @@ __device__ int decide(const Ctx& c, const Beam& b, int a, float Ns, const Box& b
     const float s = float(sg);
     const float entry = sg > 0 ? comp(box.lo, cc) : comp(box.hi, cc);
     const float exit = sg > 0 ? comp(box.hi, cc) : comp(box.lo, cc);
-    const float oc = comp(c.P.O, cc);
-    const float nmin = s * (entry - oc), nmax = s * (exit - oc);
+    const float nmin = s * relApex(c, entry, cc), nmax = s * relApex(c, exit, cc);
     const C4 lhs = scaled(awOf(b, cc), Ns);
     if (allBelow(lhs, scaled(awa, nmin), kKdRel)) return kFar;
     if (allAbove(lhs, scaled(awa, nmax), kKdRel)) return kNear;
@@ -329,9 +342,9 @@ __device__ bool overlaps(const Ctx& c, const Beam& b, const Box& box) {
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int sg = sgnOf(b, k);
-    const float s = float(sg), o = comp(c.P.O, k);
-    nmin[k] = s * ((sg > 0 ? comp(box.lo, k) : comp(box.hi, k)) - o);
-    nmax[k] = s * ((sg > 0 ? comp(box.hi, k) : comp(box.lo, k)) - o);
+    const float s = float(sg);
+    nmin[k] = s * relApex(c, sg > 0 ? comp(box.lo, k) : comp(box.hi, k), k);
+    nmax[k] = s * relApex(c, sg > 0 ? comp(box.hi, k) : comp(box.lo, k), k);
     if (nmax[k] < 0) return false;
   }
 #pragma unroll
@@ -416,7 +429,7 @@ __device__ __forceinline__ void hitPlane(const Params& p, const Ctx& c, int t, W
   w.hitA[0] = dot(nT, c.P.u);
   w.hitA[1] = dot(nT, c.P.v);
   w.hitA[2] = dot(nT, c.P.d);
-  w.hitC = dot(nT, f3(vv.x, vv.y, vv.z) - c.P.O);
+  w.hitC = dot(nT, relApex(c, f3(vv.x, vv.y, vv.z)));
 }
 
 // wfExpand (wavefront_core.h:92-112).
@@ -471,10 +484,9 @@ struct TriS {
 // Vertices relative to the apex, first edge normal, orientation and the
 // degenerate / back-face rejection.
 __device__ __forceinline__ void triBegin(const Ctx& c, const LeafTri& ref, TriS& ti) {
-  const F3& O = c.P.O;
-  ti.r[0] = f3(ref.v[0], ref.v[1], ref.v[2]) - O;
-  ti.r[1] = f3(ref.v[3], ref.v[4], ref.v[5]) - O;
-  ti.r[2] = f3(ref.v[6], ref.v[7], ref.v[8]) - O;
+  ti.r[0] = relApex(c, f3(ref.v[0], ref.v[1], ref.v[2]));
+  ti.r[1] = relApex(c, f3(ref.v[3], ref.v[4], ref.v[5]));
+  ti.r[2] = relApex(c, f3(ref.v[6], ref.v[7], ref.v[8]));
   ti.n = f3(ref.n[0], ref.n[1], ref.n[2]);
   ti.vi[0] = ref.vi[0];
   ti.vi[1] = ref.vi[1];
@@ -886,7 +898,7 @@ __device__ __forceinline__ void emitOutput(const Params& p, const Ctx& c, const 
   const float area = area4(b.qx, b.qy);
   if (tri >= 0) cnt.v[sHits]++;
   cnt.v[sBeams]++;
-  if (App == kSoft) {
+  if (isSoft(App)) {
     if (tri >= 0) return;  // occluded
     const RootQ& R = p.roots[root];
     if (area > 0) atomicAdd(&p.missFx[root], (unsigned long long)(double(area) * pow2(-R.unitExp)));
@@ -1016,7 +1028,7 @@ struct PhaseClock {
 #define BTG_SPLIT_WAVES_PER_EU 2
 #endif
 #ifdef BTG_TRACE_WAVES_PER_EU
-#define TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(App == kSoft ? BTG_SOFT_TRACE_WAVES_PER_EU : BTG_TRACE_WAVES_PER_EU)))
+#define TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(isSoft(App) ? BTG_SOFT_TRACE_WAVES_PER_EU : BTG_TRACE_WAVES_PER_EU)))
 #else
 #define TRACE_OCCUPANCY
 #endif
@@ -1050,7 +1062,7 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
         const int a = int(kn.data & 3u);
         const float s = kn.split;
         const int sga = sgnOf(B, a);
-        const float Ns = float(sga) * (s - comp(c.P.O, a));
+        const float Ns = float(sga) * relApex(c, s, a);
         const uint32_t left = r.node + 1, right = kn.data >> 2;
         const bool toRight = sga > 0;
         int dec = kFar;
@@ -1301,7 +1313,7 @@ struct EmitSink {
       if (!ok) {
         cnt->dropped += double(dropped);
       } else if (!queued) {
-        if (App == kSoft) {  // occluded by t: counted, nothing to accumulate
+        if (isSoft(App)) {  // occluded by t: counted, nothing to accumulate
           cnt->v[sHits]++;
           cnt->v[sBeams]++;
         } else {
@@ -1630,12 +1642,29 @@ __device__ void admitPoly(const Params& p, const Ctx& c, const Pg<8>& rootPoly, 
   }
 }
 
-// lightBeamSetup (soft_core.h:38-73).
-__device__ bool lightSetup(const SoftJob& p, F3 x, F3 ns, int tri, RootQ& q, Pg<8>& root) {
+// lightBeamSetup (soft_core.h:38-73). In the guarded mode the hit point is
+// the float pair x + xlo (the double hit point), the apex is lifted off it by
+// the double build's offset in double and held as the float pair O + Olo,
+// and the horizon line passes through the double hit point.
+template <int App>
+__device__ bool lightSetup(const SoftJob& p, F3 x, F3 xlo, F3 ns, int tri, RootQ& q, Pg<8>& root) {
   const F3 nL = normalize(cross(p.lU, p.lV));
   if (!(dot(x - p.lc, nL) > 0)) return false;
-  const F3 apex = x + ns * p.offset;
-  q.plane = Plane::make(apex, p.lc, p.lU, p.lV);
+  float hzc;
+  if (App == kSoftG) {
+    const double xd = double(x.x) + double(xlo.x), yd = double(x.y) + double(xlo.y), zd = double(x.z) + double(xlo.z);
+    const double off = double(p.offset);
+    const double ax = xd + double(ns.x) * off, ay = yd + double(ns.y) * off, az = zd + double(ns.z) * off;
+    const F3 O = f3(float(ax), float(ay), float(az));
+    q.Olo = f3(float(ax - double(O.x)), float(ay - double(O.y)), float(az - double(O.z)));
+    q.plane = Plane::make(O, p.lc, p.lU, p.lV, q.Olo);
+    hzc = float((double(ns.x) * (double(p.lc.x) - xd) + double(ns.y) * (double(p.lc.y) - yd)) +
+                double(ns.z) * (double(p.lc.z) - zd));
+  } else {
+    const F3 apex = x + ns * p.offset;
+    q.plane = Plane::make(apex, p.lc, p.lU, p.lV);
+    hzc = dot(ns, p.lc - x);
+  }
   q.flags = fAnyHit | fFarIsPlane;
   q.exclude = tri;
   const F3 hu = p.lU * 0.5f, hv = p.lV * 0.5f;
@@ -1649,7 +1678,7 @@ __device__ bool lightSetup(const SoftJob& p, F3 x, F3 ns, int tri, RootQ& q, Pg<
   }
   if (quad.area() < 0) quad.reverse();
   q.lightArea = quad.area();
-  const Ln hz{dot(ns, q.plane.u), dot(ns, q.plane.v), dot(ns, p.lc - x)};
+  const Ln hz{dot(ns, q.plane.u), dot(ns, q.plane.v), hzc};
   int above = 0;
   for (int i = 0; i < quad.n; ++i) above += hz.at(quad.x[i], quad.y[i]) > 0;
   if (above == 0) return false;
@@ -1728,12 +1757,12 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
   R.unitExp = 0;
   R.tri = -1;
   R.lightArea = 0;
-  R.x = R.ns = f3(0, 0, 0);
+  R.x = R.ns = R.Olo = f3(0, 0, 0);
   Pg<8> poly;
   poly.n = 0;
   uint32_t node0 = p.node0;
   Box bounds = p.rootBox;
-  if (App == kSoft) {
+  if (isSoft(App)) {
     p.missFx[k] = 0;
     p.gFx[k] = 0;
     const SoftJob& J = p.jobs[jobOfRoot(p, k)];
@@ -1754,8 +1783,9 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
     }
     const F3 x = f3(p.pixX[3 * pix], p.pixX[3 * pix + 1], p.pixX[3 * pix + 2]);
     const F3 ns = f3(p.pixN[3 * pix], p.pixN[3 * pix + 1], p.pixN[3 * pix + 2]);
+    const F3 xlo = App == kSoftG ? f3(p.pixXlo[3 * pix], p.pixXlo[3 * pix + 1], p.pixXlo[3 * pix + 2]) : f3(0, 0, 0);
     p.Eu[pix] = length(cross(J.lU, J.lV)) * centerG(J, x, ns);
-    if (!lightSetup(J, x, ns, T, R, poly)) {
+    if (!lightSetup<App>(J, x, xlo, ns, T, R, poly)) {
       p.vis[pix] = 0;
       p.E[pix] = 0;
       rootDone(p, k);
@@ -1787,7 +1817,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
     }
   }
   setupQuery(R, poly.extent());
-  if (App == kSoft) {
+  if (isSoft(App)) {
     // Unit of the exact area sum: ulp(minArea); every output area is a
     // multiple of it (areas >= minArea).
     int e = 0;
@@ -1906,7 +1936,7 @@ __device__ __forceinline__ float rayTri(const LeafTri& t, F3 o, F3 d) {
 // rayTraverse<false> (ray_core.h:42-132): nearest hit in (0, inf), coplanar
 // ties to the lower id within 4 ulp.
 __device__ bool rayNearest(const DevScene& sv, uint32_t node0, const Box& bounds, F3 o, F3 d, int& hitTri, float& hitT,
-                           Cnt& cnt) {
+                           uint32_t& hitRef, Cnt& cnt) {
   cnt.v[sRays]++;
   float t0 = 0, t1 = __builtin_inff();
   float inv[3];
@@ -1937,6 +1967,7 @@ __device__ bool rayNearest(const DevScene& sv, uint32_t node0, const Box& bounds
   uint32_t node = node0;
   float best = __builtin_inff();
   int bestTri = -1;
+  uint32_t bestRef = 0;
   for (;;) {
     const KdNodeD n = sv.nodes[node];
     if ((n.data & 3u) != 3u) {
@@ -1980,6 +2011,7 @@ __device__ bool rayNearest(const DevScene& sv, uint32_t node0, const Box& bounds
       if (bestTri < 0 || th < best - tol || (th <= best + tol && ref.tri < bestTri)) {
         best = th;
         bestTri = ref.tri;
+        bestRef = n.offset + i;
       }
     }
     if (bestTri >= 0 && best <= t1) break;
@@ -1993,7 +2025,27 @@ __device__ bool rayNearest(const DevScene& sv, uint32_t node0, const Box& bounds
   cnt.v[sHits]++;
   hitTri = bestTri;
   hitT = best;
+  hitRef = bestRef;
   return true;
+}
+
+// The hit point in double on the plane of the triangle's float corners, as
+// the float pair X + Xlo: o + d t with n = (v1 - v0) x (v2 - v0) and
+// t = n.(v0 - o) / n.d, every step in double (the guarded mode's receiver
+// point; the float pipeline's o + d t can sit ~1e-7 of the scene off that
+// plane, far more than the double build's apex offset).
+__device__ void hitPointD(const LeafTri& t, F3 o, F3 d, F3& X, F3& Xlo) {
+  const double v0x = t.v[0], v0y = t.v[1], v0z = t.v[2];
+  const double ax = double(t.v[3]) - v0x, ay = double(t.v[4]) - v0y, az = double(t.v[5]) - v0z;
+  const double bx = double(t.v[6]) - v0x, by = double(t.v[7]) - v0y, bz = double(t.v[8]) - v0z;
+  const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double den = (nx * double(d.x) + ny * double(d.y)) + nz * double(d.z);
+  const double num = (nx * (v0x - double(o.x)) + ny * (v0y - double(o.y))) + nz * (v0z - double(o.z));
+  const double th = den != 0 ? num / den : 0.0;
+  const double px = double(o.x) + double(d.x) * th, py = double(o.y) + double(d.y) * th,
+               pz = double(o.z) + double(d.z) * th;
+  X = f3(float(px), float(py), float(pz));
+  Xlo = f3(float(px - double(X.x)), float(py - double(X.y)), float(pz - double(X.z)));
 }
 
 // The soft-shadow primary pass (cuda/gpu_render.cu:222-244): one ray through
@@ -2021,13 +2073,22 @@ __global__ void __launch_bounds__(kBlock) primaryRayKernel(Params p) {
     const F3 d = J.fwd + J.right * qx + J.down * qy;
     int th_tri = -1;
     float th = 0;
+    uint32_t th_ref = 0;
     int* triOut = const_cast<int*>(p.pixTri);
     float* X = const_cast<float*>(p.pixX);
     float* N = const_cast<float*>(p.pixN);
     triOut[k] = -1;
-    if (rayNearest(p.sv, J.node0, J.bounds, J.eye, d, th_tri, th, cnt)) {
+    if (rayNearest(p.sv, J.node0, J.bounds, J.eye, d, th_tri, th, th_ref, cnt)) {
       triOut[k] = th_tri;
-      const F3 hp = J.eye + d * th;
+      F3 hp = J.eye + d * th;
+      if (p.pixXlo) {
+        F3 lo;
+        hitPointD(p.sv.refs[th_ref], J.eye, d, hp, lo);
+        float* XL = const_cast<float*>(p.pixXlo);
+        XL[3 * k] = lo.x;
+        XL[3 * k + 1] = lo.y;
+        XL[3 * k + 2] = lo.z;
+      }
       const float4 nn = p.sv.triNV[2 * size_t(th_tri)];
       F3 nrm = f3(nn.x, nn.y, nn.z);
       if (dot(nrm, d) > 0) nrm = -nrm;
@@ -2095,7 +2156,7 @@ class Engine {
   void* sceneMem[5] = {};
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
-  DevBuf pixTri, pixX, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
+  DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
   WState* hostState = nullptr;  // pinned
   uint64_t phaseCycles[8] = {};
   // per-launch timing (Options::profile): event pairs and the kernel kind
@@ -2145,7 +2206,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs};
+                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -2230,7 +2291,7 @@ template <int App>
 int runPassApp(Engine& e, Params& p, RunInfo& info, std::string& err) {
   const int gA = residentGrid(e, admitKernel<App>), gT = residentGrid(e, traceKernel<App>),
             gS = residentGrid(e, splitKernel<App>), gH = residentGrid(e, shadeKernel);
-  const bool shade = App == kSoft && p.exact;
+  const bool shade = isSoft(App) && p.exact;
   initKernel<<<1, 64, 0, e.stream>>>(p);
   BTG_TRY(hipGetLastError());
   int rounds = 0;
@@ -2273,6 +2334,7 @@ int runPassApp(Engine& e, Params& p, RunInfo& info, std::string& err) {
 
 int runPass(Engine& e, Params& p, RunInfo& info, std::string& err) {
   if (p.app == kSoft) return runPassApp<kSoft>(e, p, info, err);
+  if (p.app == kSoftG) return runPassApp<kSoftG>(e, p, info, err);
   if (p.app == kPrimary) return runPassApp<kPrimary>(e, p, info, err);
   return runPassApp<kPoint>(e, p, info, err);
 }
@@ -2592,14 +2654,17 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
     }
   }
   Params p;
-  int rc = prepare(*e, kSoft, int(R), e->opt.budget, p, err);
+  const bool guard = e->opt.precision != 0;
+  int rc = prepare(*e, guard ? kSoftG : kSoft, int(R), e->opt.budget, p, err);
   if (rc) return rc;
   const size_t np = size_t(npix);
   if ((rc = e->ensure(e->pixTri, np * 4, err)) || (rc = e->ensure(e->pixX, np * 12, err)) ||
       (rc = e->ensure(e->pixN, np * 12, err)) || (rc = e->ensure(e->vis, np * 4, err)) ||
       (rc = e->ensure(e->E, np * 4, err)) || (rc = e->ensure(e->Eu, np * 4, err)) ||
-      (rc = e->ensure(e->jobs, jt.size() * sizeof(SoftJob), err)))
+      (rc = e->ensure(e->jobs, jt.size() * sizeof(SoftJob), err)) ||
+      (guard && (rc = e->ensure(e->pixXlo, np * 12, err))))
     return rc;
+  if (guard) p.pixXlo = static_cast<const float*>(e->pixXlo.ptr);
   BTG_TRY(hipMemcpyAsync(e->jobs.ptr, jt.data(), jt.size() * sizeof(SoftJob), hipMemcpyHostToDevice, e->stream));
   p.jobs = static_cast<const SoftJob*>(e->jobs.ptr);
   p.numJobs = n;
@@ -2722,6 +2787,7 @@ struct OpenPass {
   DevBuf jobLeft, killed;
   hipEvent_t stepDone = nullptr;
   bool exact = true;
+  bool guard = false;  // FP64-guarded soft shadows (the options' precision at creation)
 };
 
 void openDestroy(OpenPass* op) {
@@ -2776,10 +2842,12 @@ OpenPass* openCreate(Engine* base, int64_t rootCap, int64_t pixCap, int jobCap, 
   op->pixCap = pixCap;
   op->jobCap = jobCap;
   op->exact = exact;
+  op->guard = e.opt.precision != 0;
   Params& p = op->p;
-  int rc = prepare(e, kSoft, int(rootCap), e.opt.budget, p, err);
+  int rc = prepare(e, op->guard ? kSoftG : kSoft, int(rootCap), e.opt.budget, p, err);
   const size_t np = size_t(pixCap);
   if (!rc && ((rc = e.ensure(e.pixTri, np * 4, err)) || (rc = e.ensure(e.pixX, np * 12, err)) ||
+              (op->guard && (rc = e.ensure(e.pixXlo, np * 12, err))) ||
               (rc = e.ensure(e.pixN, np * 12, err)) || (rc = e.ensure(e.vis, np * 4, err)) ||
               (rc = e.ensure(e.E, np * 4, err)) || (rc = e.ensure(e.Eu, np * 4, err)) ||
               (rc = e.ensure(e.jobs, size_t(jobCap) * sizeof(SoftJob), err)) ||
@@ -2806,6 +2874,7 @@ OpenPass* openCreate(Engine* base, int64_t rootCap, int64_t pixCap, int jobCap, 
   if (exact) p.shq = static_cast<ShadeRec*>(e.shq.ptr);
   p.pixTri = static_cast<const int*>(e.pixTri.ptr);
   p.pixX = static_cast<const float*>(e.pixX.ptr);
+  if (op->guard) p.pixXlo = static_cast<const float*>(e.pixXlo.ptr);
   p.pixN = static_cast<const float*>(e.pixN.ptr);
   p.vis = static_cast<float*>(e.vis.ptr);
   p.E = static_cast<float*>(e.E.ptr);
@@ -2860,7 +2929,7 @@ int openTryAdd(OpenPass* op, const SoftJobH& h, int* slot, std::string& err) {
   J.node0 = sr.nodeBase;
   J.triBase = int(sr.triBase);
   J.bounds = sr.bounds;
-  J.offset = e.opt.precision ? sr.offsetGuard : sr.offset;
+  J.offset = op->guard ? sr.offsetGuard : sr.offset;
   J.halfW = h.cam.halfW;
   J.halfH = h.cam.halfH;
   J.eye = h.cam.eye;
@@ -2898,18 +2967,30 @@ int openTryAdd(OpenPass* op, const SoftJobH& h, int* slot, std::string& err) {
   return 1;
 }
 
+namespace {
+template <int App>
+void openRounds(OpenPass* op, int rounds) {
+  Engine& e = *op->e;
+  Params& p = op->p;
+  const int gA = residentGrid(e, admitKernel<App>), gT = residentGrid(e, traceKernel<App>),
+            gS = residentGrid(e, splitKernel<App>), gH = residentGrid(e, shadeKernel);
+  for (int i = 0; i < rounds; ++i, ++op->rounds) {
+    const int cur = op->rounds & 1;
+    admitKernel<App><<<gA, kBlock, 0, e.stream>>>(p, cur);
+    traceKernel<App><<<gT, kBlock, 0, e.stream>>>(p, cur);
+    if (p.exact) shadeKernel<<<gH, kBlock, 0, e.stream>>>(p);
+    splitKernel<App><<<gS, kBlock, 0, e.stream>>>(p, cur);
+  }
+}
+}  // namespace
+
 int openStep(OpenPass* op, int rounds, std::string& err) {
   Engine& e = *op->e;
   Params& p = op->p;
-  const int gA = residentGrid(e, admitKernel<kSoft>), gT = residentGrid(e, traceKernel<kSoft>),
-            gS = residentGrid(e, splitKernel<kSoft>), gH = residentGrid(e, shadeKernel);
-  for (int i = 0; i < rounds; ++i, ++op->rounds) {
-    const int cur = op->rounds & 1;
-    admitKernel<kSoft><<<gA, kBlock, 0, e.stream>>>(p, cur);
-    traceKernel<kSoft><<<gT, kBlock, 0, e.stream>>>(p, cur);
-    if (p.exact) shadeKernel<<<gH, kBlock, 0, e.stream>>>(p);
-    splitKernel<kSoft><<<gS, kBlock, 0, e.stream>>>(p, cur);
-  }
+  if (op->guard)
+    openRounds<kSoftG>(op, rounds);
+  else
+    openRounds<kSoft>(op, rounds);
   BTG_TRY(hipGetLastError());
   BTG_TRY(hipMemcpyAsync(e.hostState, p.st, sizeof(WState), hipMemcpyDeviceToHost, e.stream));
   if (p.numJobs > 0)

@@ -237,6 +237,11 @@ struct KdNode {
 
 struct Plane {  // BeamPlane (beam_types.h:14-52)
   V3 O, p0, u, v, n, d;
+  V3 Olo{0, 0, 0};  // guarded soft mode: the apex is the float pair O + Olo
+  // Apex-relative vector / coordinate: (p - O) - Olo, which is p - O when Olo
+  // is zero (x - (+0) = x).
+  V3 rel(V3 p) const { return sub(sub(p, O), Olo); }
+  float rel(float s, int a) const { return (s - O.at(a)) - Olo.at(a); }
   V3 point(float qx, float qy) const { return add(add(p0, scl(u, qx)), scl(v, qy)); }
   V3 dir(float qx, float qy) const { return add(add(d, scl(u, qx)), scl(v, qy)); }
   void coords(V3 p, float& qx, float& qy) const {
@@ -252,16 +257,18 @@ struct Plane {  // BeamPlane (beam_types.h:14-52)
     coords(x, qx, qy);
     return true;
   }
-  static Plane make(V3 O, V3 p0, V3 u_, V3 v_) {
+  static Plane make(V3 O, V3 p0, V3 u_, V3 v_, V3 Olo = V3{0, 0, 0}) {
     Plane P;
     P.O = O;
+    P.Olo = Olo;
     P.p0 = p0;
+    const V3 d = P.rel(p0);
     V3 n = nrmz(crs(u_, v_));
-    if (dot(n, sub(p0, O)) < 0) n = neg(n);
+    if (dot(n, d) < 0) n = neg(n);
     P.u = nrmz(u_);
     P.v = crs(n, P.u);
     P.n = n;
-    P.d = sub(p0, O);
+    P.d = d;
     return P;
   }
 };
@@ -391,7 +398,7 @@ class Tracer {
         const int a = kn.axis();
         const float s = kn.split;
         const Beam& B = pool[size_t(w.beam)];
-        const float Ns = float(B.sgn[a]) * (s - Q.plane.O.at(a));
+        const float Ns = float(B.sgn[a]) * Q.plane.rel(s, a);
         const uint32_t left = node + 1, right = kn.hi();
         Box lb = box, rb = box;
         lb.hi.at(a) = s;
@@ -465,14 +472,14 @@ class Tracer {
 
   // decide (beam_core.h:139-164).
   int decide(const Beam& b, int a, float Ns, const Box& bx) const {
-    const V3& O = Q.plane.O;
+    const Plane& P = Q.plane;
     const Lanes& awa = b.aw[a];
     for (int k = 1; k <= 2; ++k) {
       const int c = a + k >= 3 ? a + k - 3 : a + k;
       const float sg = float(b.sgn[c]);
       const float entry = b.sgn[c] > 0 ? bx.lo.at(c) : bx.hi.at(c);
       const float exit = b.sgn[c] > 0 ? bx.hi.at(c) : bx.lo.at(c);
-      const float nmin = sg * (entry - O.at(c)), nmax = sg * (exit - O.at(c));
+      const float nmin = sg * P.rel(entry, c), nmax = sg * P.rel(exit, c);
       const Lanes lhs = mul(b.aw[c], Ns);
       Lanes rhs = mul(awa, nmin);
       if (allLess(lhs, rhs, tolOf(lhs, rhs, kKd))) return kFar;
@@ -488,12 +495,12 @@ class Tracer {
 
   // overlaps (beam_core.h:166-195).
   bool overlaps(const Beam& b, const Box& bx) const {
-    const V3& O = Q.plane.O;
+    const Plane& P = Q.plane;
     float nmin[3], nmax[3];
     for (int c = 0; c < 3; ++c) {
       const float sg = float(b.sgn[c]);
-      nmin[c] = sg * ((b.sgn[c] > 0 ? bx.lo.at(c) : bx.hi.at(c)) - O.at(c));
-      nmax[c] = sg * ((b.sgn[c] > 0 ? bx.hi.at(c) : bx.lo.at(c)) - O.at(c));
+      nmin[c] = sg * P.rel(b.sgn[c] > 0 ? bx.lo.at(c) : bx.hi.at(c), c);
+      nmax[c] = sg * P.rel(b.sgn[c] > 0 ? bx.hi.at(c) : bx.lo.at(c), c);
       if (nmax[c] < 0) return false;
     }
     for (int c = 0; c < 3; ++c)
@@ -558,8 +565,7 @@ class Tracer {
   TriSetup setupTri(int t) const {
     TriSetup ti;
     const uint32_t* T = &S.tri3[3 * size_t(t)];
-    const V3& O = Q.plane.O;
-    for (int k = 0; k < 3; ++k) ti.r[k] = sub(S.pos[T[k]], O);
+    for (int k = 0; k < 3; ++k) ti.r[k] = Q.plane.rel(S.pos[T[k]]);
     ti.n = S.triN[size_t(t)];
     ti.tri = t;
     ti.ok = false;
@@ -1096,11 +1102,28 @@ float formFactor(V3 x, V3 n, const V3* poly, int count) {
 }
 
 // lightBeamSetup (soft_core.h:38-73).
-bool lightSetup(const Light& L, V3 x, V3 ns, int tri, float offset, Query& q, Poly& root, float& lightArea) {
+// guard: the engine's FP64-guarded mode (not in the reference): x + xlo is the
+// hit point in double, the apex x + ns * offset is formed in double and kept as
+// the float pair O + Olo, the horizon line passes through the double point.
+bool lightSetup(const Light& L, V3 x, V3 xlo, bool guard, V3 ns, int tri, float offset, Query& q, Poly& root,
+                float& lightArea) {
   const V3 nL = L.n();
   if (!(dot(sub(x, L.c), nL) > 0)) return false;
-  const V3 apex = add(x, scl(ns, offset));
-  q.plane = Plane::make(apex, L.c, L.U, L.V);
+  float hzc;
+  if (guard) {
+    const double xd = double(x.x) + double(xlo.x), yd = double(x.y) + double(xlo.y), zd = double(x.z) + double(xlo.z);
+    const double off = double(offset);
+    const double ax = xd + double(ns.x) * off, ay = yd + double(ns.y) * off, az = zd + double(ns.z) * off;
+    const V3 O = mk(float(ax), float(ay), float(az));
+    const V3 Olo = mk(float(ax - double(O.x)), float(ay - double(O.y)), float(az - double(O.z)));
+    q.plane = Plane::make(O, L.c, L.U, L.V, Olo);
+    hzc = float((double(ns.x) * (double(L.c.x) - xd) + double(ns.y) * (double(L.c.y) - yd)) +
+                double(ns.z) * (double(L.c.z) - zd));
+  } else {
+    const V3 apex = add(x, scl(ns, offset));
+    q.plane = Plane::make(apex, L.c, L.U, L.V);
+    hzc = dot(ns, sub(L.c, x));
+  }
   q.anyHit = true;
   q.farIsPlane = true;
   q.excludeTri = tri;
@@ -1118,7 +1141,7 @@ bool lightSetup(const Light& L, V3 x, V3 ns, int tri, float offset, Query& q, Po
   Line hz;
   hz.a = dot(ns, q.plane.u);
   hz.b = dot(ns, q.plane.v);
-  hz.c = dot(ns, sub(L.c, x));
+  hz.c = hzc;
   int above = 0;
   for (int i = 0; i < quad.n; ++i) above += hz.at(quad.x[i], quad.y[i]) > 0;
   if (above == 0) return false;
@@ -1143,12 +1166,12 @@ float centerG(const Light& L, V3 x, V3 ns) {
 }
 
 // beamLightVisibility (render.cpp:279-303).
-float lightVis(Tracer& tr, const Light& L, V3 x, V3 ns, int tri, float offset, float* G) {
+float lightVis(Tracer& tr, const Light& L, V3 x, V3 xlo, bool guard, V3 ns, int tri, float offset, float* G) {
   if (G) *G = 0;
   Query q;
   Poly root;
   float lightArea;
-  if (!lightSetup(L, x, ns, tri, offset, q, root, lightArea)) return 0;
+  if (!lightSetup(L, x, xlo, guard, ns, tri, offset, q, root, lightArea)) return 0;
   tr.keep = G != nullptr;
   tr.trace(q, root);
   const float V = float(tr.missArea) / lightArea;
@@ -1163,6 +1186,25 @@ float lightVis(Tracer& tr, const Light& L, V3 x, V3 ns, int tri, float offset, f
     *G = g;
   }
   return std::max(0.0f, std::min(1.0f, V));
+}
+
+// The guarded mode's receiver point: the ray's hit on the plane of triangle
+// t's float corners, in double, as the float pair X + Xlo (n = (v1 - v0) x
+// (v2 - v0), th = n.(v0 - o) / n.d, X = o + d th).
+void hitPointD(const orc_scene& S, int t, V3 o, V3 d, V3& X, V3& Xlo) {
+  const uint32_t* T = &S.tri3[3 * size_t(t)];
+  const V3 v0 = S.pos[T[0]], v1 = S.pos[T[1]], v2 = S.pos[T[2]];
+  const double v0x = v0.x, v0y = v0.y, v0z = v0.z;
+  const double ax = double(v1.x) - v0x, ay = double(v1.y) - v0y, az = double(v1.z) - v0z;
+  const double bx = double(v2.x) - v0x, by = double(v2.y) - v0y, bz = double(v2.z) - v0z;
+  const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double den = (nx * double(d.x) + ny * double(d.y)) + nz * double(d.z);
+  const double num = (nx * (v0x - double(o.x)) + ny * (v0y - double(o.y))) + nz * (v0z - double(o.z));
+  const double th = den != 0 ? num / den : 0.0;
+  const double px = double(o.x) + double(d.x) * th, py = double(o.y) + double(d.y) * th,
+               pz = double(o.z) + double(d.z) * th;
+  X = mk(float(px), float(py), float(pz));
+  Xlo = mk(float(px - double(X.x)), float(py - double(X.y)), float(pz - double(X.z)));
 }
 
 float softOffset(const orc_scene& S, bool guard) {
@@ -1303,12 +1345,13 @@ void orc_soft(const orc_scene* s, const float* cam14, int W, int H, const float*
         float th;
         if (!rayNearest(*s, c.eye, d, t, th, rays)) continue;
         tri[k] = t;
-        const V3 p = add(c.eye, scl(d, th));
+        V3 p = add(c.eye, scl(d, th)), plo = mk(0, 0, 0);
+        if (guard) hitPointD(*s, t, c.eye, d, p, plo);
         V3 n = s->triN[size_t(t)];
         if (dot(n, d) > 0) n = neg(n);
         const float G0 = centerG(L, p, n);
         float Gx = 0;
-        const float V = lightVis(tr, L, p, n, t, offset, exact ? &Gx : nullptr);
+        const float V = lightVis(tr, L, p, plo, guard != 0, n, t, offset, exact ? &Gx : nullptr);
         vis[k] = V;
         Eu[k] = A * G0;
         E[k] = exact ? Gx : A * G0 * V;
@@ -1327,7 +1370,8 @@ void orc_soft(const orc_scene* s, const float* cam14, int W, int H, const float*
 float orc_light_vis(const orc_scene* s, const float* x3, const float* ns3, int tri, const float* light9, float offset,
                     float* G, double* stats) {
   Tracer tr(*s);
-  const float V = lightVis(tr, lightOf(light9), mk(x3[0], x3[1], x3[2]), mk(ns3[0], ns3[1], ns3[2]), tri, offset, G);
+  const float V = lightVis(tr, lightOf(light9), mk(x3[0], x3[1], x3[2]), mk(0, 0, 0), false, mk(ns3[0], ns3[1], ns3[2]),
+                           tri, offset, G);
   tr.st.put(stats);
   return V;
 }

@@ -41,7 +41,8 @@ void orc_scene_free(orc_scene* s);
  * with the trail and no mailbox). cam14 = eye, fwd, right, down, halfW,
  * halfH; light9 = centre, U, V. Rows [y0, y1) only (y1 <= 0: all rows);
  * outputs are full-image arrays. threads >= 1. guard = 1: the FP64-guarded
- * variant of the GPU library (the double build's apex offset). */
+ * variant of the GPU library (double hit point, the double build's apex
+ * offset, the apex as a float pair). */
 void orc_soft(const orc_scene* s, const float* cam14, int W, int H, const float* light9, int exact, int threads,
               int y0, int y1, int32_t* tri, float* vis, float* E, float* Eu, double* stats, int guard);
 

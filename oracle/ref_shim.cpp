@@ -191,7 +191,8 @@ void ref_soft(void* sh, void* kh, const double* cam, int W, int H, const double*
 // rows y0, y0 + stride, ... < y1: the bounded CPU-baseline sample (a strided
 // sample covers every region of the frame). Same calls, same threading.
 void ref_soft_rows(void* sh, void* kh, const double* cam, int W, int H, const double* light, int exact, int mailbox,
-                   int trail, int threads, int y0, int y1, int stride, float* vis, double* stats, double* seconds) {
+                   int trail, int threads, int y0, int y1, int stride, float* vis, double* stats, double* seconds,
+                   int32_t* triOut) {
   const Scene& s = *static_cast<Scene*>(sh);
   const KdTree& tree = *static_cast<KdTree*>(kh);
   Camera c = makeCam(cam, W, H);
@@ -232,6 +233,7 @@ void ref_soft_rows(void* sh, void* kh, const double* cam, int W, int H, const do
     for (auto& x : th) x.join();
   }
   const double primaryS = tp.seconds();
+  if (triOut) std::copy(triv.begin(), triv.end(), triOut);
   Timer ts;
   std::atomic<int> next(0);
   std::vector<TraceStats> st{size_t(threads), TraceStats()};

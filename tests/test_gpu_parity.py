@@ -178,11 +178,11 @@ def test_invalid_arguments_are_errors_not_crashes(require_gpu):
     assert len(empty) == 0
 
 
-@pytest.mark.parametrize("name", ["room", "plant", "random_tris"])
+@pytest.mark.parametrize("name", ["room", "plant", "random_tris", "conference", "atrium"])
 def test_guarded_soft_shadows_match_guarded_oracle(require_gpu, name):
-    """precision = 1 (the double build's apex offset) on the GPU equals the
-    oracle's guarded variant; its error against the double build is checked
-    on the CPU (test_oracle.py)."""
+    """precision = 1 (double hit point, the double build's apex offset, the
+    apex as a float pair) on the GPU equals the oracle's guarded variant; its
+    error against the double build is checked on the CPU (test_oracle.py)."""
     from beamtrace2007_amd.host import camera
     s, a, k, sp = load_scene(name)
     W, H = 72, 56

@@ -134,3 +134,28 @@ def test_passes_alternate_when_full(require_gpu):
         del r_alone
     srv.close()
     r.close()
+
+
+def test_guarded_requests_equal_guarded_alone(require_gpu):
+    """A server on a context in the FP64-guarded mode runs its passes guarded:
+    results equal the same requests rendered alone in that mode."""
+    from beamtrace2007_amd.gpu import GpuRenderer, Server
+    from beamtrace2007_amd.host import camera
+    r = GpuRenderer(load_scene("room")[0])
+    r.set_options(precision=1)
+    srv = Server(r)
+    sp = load_scene("room")[3]
+    plan = [(40, 30, 1.0), (17, 9, 0.5), (64, 48, 2.0)]
+    tickets = []
+    for W, H, s in plan:
+        cam = camera(sp["view"], W, H)
+        light = scaled_light(sp["area_light"], s)
+        tickets.append((cam, W, H, light, srv.submit_soft(0, cam, W, H, light)))
+    for cam, W, H, light, t in tickets:
+        res = srv.wait(t, timeout=120)
+        assert res is not None and res["status"] == 0, res
+        a = r.soft_shadows(cam, W, H, light, exact=True)
+        for f in ("tri", "vis", "E", "Eu"):
+            assert np.array_equal(res[f], a[f]), (W, H, f)
+    srv.close()
+    r.close()

@@ -166,17 +166,22 @@ def test_oracle_exact_and_eq2_agree_when_unoccluded():
     assert np.median(rel) < 0.05
 
 
-@pytest.mark.parametrize("name,max_err,max_frac", [("room", 1e-5, 0.0), ("random_tris", 1e-4, 0.02),
-                                                   ("plant", 5e-3, 0.05)])
-def test_guarded_soft_shadows_approach_the_double_build(name, max_err, max_frac):
-    """The FP64-guarded mode (the double build's apex offset) against the
-    reference's double build: errors shrink from the float build's (up to
-    ~4e-2 on the plant, half of the pixels above 1e-5) to the bounds here."""
+@pytest.mark.parametrize("name,side,max_err,max_frac,gain", [
+    ("room", 48, 2e-6, 0.0, 100), ("random_tris", 48, 1e-4, 0.02, 1), ("plant", 48, 2e-5, 0.002, 100),
+    ("atrium", 40, 1e-4, 0.01, 100), ("conference", 40, 5e-5, 0.01, 100), ("blob", 24, 5e-3, 0.03, 3)])
+def test_guarded_soft_shadows_approach_the_double_build(name, side, max_err, max_frac, gain):
+    """The FP64-guarded mode (double hit point, the double build's apex offset,
+    the apex as a float pair) against the reference's double build: the mean
+    visibility error falls by `gain` or more from the float build's (up to
+    ~4e-2 on the plant, half of its pixels above 1e-5), the largest error and
+    the share above 1e-5 stay within the bounds here. A float apex at the
+    double build's offset (no double hit point) self-shadows instead: errors
+    of 1.0 on the conference hall's floor."""
     if not ref_available("d"):
         pytest.skip("reference double build not built")
     from beamtrace2007_amd.host import camera
     s, a, k, sp = load_scene(name)
-    W = H = 48
+    W = H = side
     cam = camera(sp["view"], W, H)
     refd = Ref("d").load(a["pos"], a["tris"])
     d = refd.soft(sp["view"], W, H, sp["area_light"], exact=True, mailbox=False, trail=True, threads=8)
@@ -190,4 +195,4 @@ def test_guarded_soft_shadows_approach_the_double_build(name, max_err, max_frac)
         err[guard] = np.abs(g["vis"].astype(np.float64) - d["vis"])[m]
     assert err[True].max() <= max_err
     assert np.mean(err[True] > 1e-5) <= max_frac
-    assert err[True].mean() <= err[False].mean()
+    assert err[True].mean() * gain <= err[False].mean()
