@@ -247,13 +247,15 @@ int btg_get_options(const btg_ctx* ctx, btg_options* o) {
   o->profileKernels = x.profile ? 1 : 0;
   o->leafCap = x.leafCap;
   o->precision = x.precision;
+  o->useGraphs = x.graphs ? 1 : 0;
   return BTG_OK;
 }
 
 int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   if (!ctx || !o) return BTG_ERR_ARG;
   if (o->budget < 1 || o->primaryBudget < 1 || o->capacity < 1024 || o->capacity > (int64_t(1) << 31) ||
-      o->roundsPerCheck < 1 || o->leafCap < 0 || o->precision < 0 || o->precision > 1)
+      o->roundsPerCheck < 1 || o->leafCap < 0 || o->precision < 0 || o->precision > 1 || o->useGraphs < 0 ||
+      o->useGraphs > 1)
     return failWith(ctx, BTG_ERR_ARG, "invalid options");
   btg::Options& x = btg::engineOptions(ctx->engine);
   x.budget = o->budget;
@@ -263,6 +265,7 @@ int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   x.profile = o->profileKernels != 0;
   x.leafCap = o->leafCap;
   x.precision = o->precision;
+  x.graphs = o->useGraphs != 0;
   return BTG_OK;
 }
 
@@ -373,6 +376,7 @@ int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact,
   best->profileKernels = start.profile ? 1 : 0;
   best->leafCap = cur.leafCap;
   best->precision = start.precision;
+  best->useGraphs = start.graphs ? 1 : 0;
   return BTG_OK;
 }
 
@@ -419,6 +423,41 @@ int btg_primary(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, int can
     *out = P;
     fillInfo(ri, now() - t0, info);
     if (info) info->stats.hits = uint64_t(P->v.size());
+    return BTG_OK;
+  } catch (const std::bad_alloc&) {
+    return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
+  }
+}
+
+int btg_raster(btg_ctx* ctx, const btg_camera* cam, const btg_poly* polys, int64_t count, int shadowPass,
+               int32_t* ids, uint8_t* mask) {
+  if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");
+  if (!validCam(cam) || count < 0 || (count > 0 && !polys) || (shadowPass ? !mask : !ids))
+    return failWith(ctx, BTG_ERR_ARG, "invalid camera, polygons or output");
+  try {
+    return btg::engineRaster(ctx->engine, camOf(*cam), reinterpret_cast<const btg::OutPoly*>(polys), count,
+                             shadowPass != 0, ids, mask, ctx->err);
+  } catch (const std::bad_alloc&) {
+    return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
+  }
+}
+
+int btg_images(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, const float* light3, int32_t* ids,
+               uint8_t* mask, btg_run_info* primInfo, btg_run_info* pointInfo, double* rasterSeconds) {
+  if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");
+  if (!validCam(cam) || tile <= 0 || !ids || (light3 && !mask))
+    return failWith(ctx, BTG_ERR_ARG, "invalid camera, tile or output");
+  const double t0 = now();
+  try {
+    btg::RunInfo a, b;
+    double rasterMs = 0;
+    const btg::F3 L = light3 ? btg::f3(light3[0], light3[1], light3[2]) : btg::f3(0, 0, 0);
+    const int rc = btg::engineImages(ctx->engine, camOf(*cam), cull != 0, tile, light3 ? &L : nullptr, ids, mask, a, b,
+                                     rasterMs, ctx->err);
+    if (rc) return rc;
+    fillInfo(a, now() - t0, primInfo);
+    fillInfo(b, 0, pointInfo);
+    if (rasterSeconds) *rasterSeconds = rasterMs * 1e-3;
     return BTG_OK;
   } catch (const std::bad_alloc&) {
     return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");

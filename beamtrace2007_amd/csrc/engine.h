@@ -66,6 +66,7 @@ struct Options {
   int leafCap = 8;
   int precision = 0;  // soft shadows: 0 float build (bit-exact), 1 FP64-guarded
   bool profile = false;
+  bool graphs = true;  // replay each group of rounds from a captured HIP graph
 };
 
 struct HostCam {
@@ -86,6 +87,19 @@ struct OutPoly {
 static_assert(sizeof(OutPoly) == 44, "polygon layout");
 
 class Engine;
+
+// rasterize (src/render/render.cpp:49-95) on the device, one sample at each
+// pixel centre: shadow = false writes ids (the tri of the canonical last
+// polygon covering each pixel centre, -1 for none; tri < 0 polygons skipped),
+// shadow = true writes mask (1 where any polygon covers the centre).
+int engineRaster(Engine* e, const HostCam& cam, const OutPoly* polys, int64_t n, bool shadow, int32_t* ids,
+                 uint8_t* mask, std::string& err);
+// Primary visibility and (light != null) point shadows of one frame with both
+// rasterizations on the device: the primary polygons feed the ID image and
+// the point pass without a host round trip. rasterMs: wall time of the two
+// rasterizations including their downloads.
+int engineImages(Engine* e, const HostCam& cam, bool cull, int tile, const F3* light, int32_t* ids,
+                 uint8_t* mask, RunInfo& prim, RunInfo& point, double& rasterMs, std::string& err);
 
 // Creates the engine's device state for a scene (copies the arrays); the
 // scene becomes resident scene 0. `offset` is the scene's soft-shadow apex

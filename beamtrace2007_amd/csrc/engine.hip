@@ -33,6 +33,8 @@
 #include <chrono>
 #include <cstring>
 
+#include <rocprim/device/device_scan.hpp>
+
 #include "engine.h"
 
 namespace btg {
@@ -41,7 +43,6 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
 
 // kSoftG: soft shadows in the FP64-guarded mode (double hit point, the
 // double build's apex offset, the apex held as a float pair).
@@ -2157,6 +2158,7 @@ class Engine {
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
+  DevBuf rasterImg, rasterRows, rasterTmp;  // device rasterization: image, rows and offsets, scan scratch
   WState* hostState = nullptr;  // pinned
   uint64_t phaseCycles[8] = {};
   // per-launch timing (Options::profile): event pairs and the kernel kind
@@ -2206,7 +2208,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs};
+                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &rasterTmp, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -2286,19 +2288,32 @@ void setCamera(const HostCam& cam, Params& p) {
 }
 
 
-// Runs the rounds of one pass until the state reports done.
+// The instantiated graphs of one group of rounds, one per parity of the
+// group's first round (the queues ping-pong by round parity), freed on exit.
+struct RoundGraphs {
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  ~RoundGraphs() {
+    for (hipGraphExec_t x : exec)
+      if (x) (void)hipGraphExecDestroy(x);
+  }
+};
+
+// Runs the rounds of one pass until the state reports done: groups of
+// roundsPerCheck rounds (admit, trace, shade, split each), then one state
+// read-back. With Options::graphs each group is captured once into a HIP
+// graph and replayed; the kernels read all changing state from device memory,
+// so the replays are the same launches. Per-launch timing (Options::profile)
+// brackets every launch with HIP events, which needs direct launches: events
+// recorded by graph nodes give no elapsed times.
 template <int App>
 int runPassApp(Engine& e, Params& p, RunInfo& info, std::string& err) {
   const int gA = residentGrid(e, admitKernel<App>), gT = residentGrid(e, traceKernel<App>),
             gS = residentGrid(e, splitKernel<App>), gH = residentGrid(e, shadeKernel);
   const bool shade = isSoft(App) && p.exact;
-  initKernel<<<1, 64, 0, e.stream>>>(p);
-  BTG_TRY(hipGetLastError());
-  int rounds = 0;
-  for (;;) {
-    const int k = std::max(1, e.opt.roundsPerCheck);
-    for (int i = 0; i < k; ++i, ++rounds) {
-      const int cur = rounds & 1;
+  const int k = std::max(1, e.opt.roundsPerCheck);
+  auto group = [&](int first) {
+    for (int i = 0; i < k; ++i) {
+      const int cur = (first + i) & 1;
       int ls = e.beginLaunch(0);
       admitKernel<App><<<gA, kBlock, 0, e.stream>>>(p, cur);
       e.endLaunch(ls);
@@ -2314,6 +2329,29 @@ int runPassApp(Engine& e, Params& p, RunInfo& info, std::string& err) {
       splitKernel<App><<<gS, kBlock, 0, e.stream>>>(p, cur);
       e.endLaunch(ls);
     }
+  };
+  initKernel<<<1, 64, 0, e.stream>>>(p);
+  BTG_TRY(hipGetLastError());
+  RoundGraphs gr;
+  const bool graphs = e.opt.graphs && !e.opt.profile;
+  int rounds = 0;
+  for (;;) {
+    const int par = rounds & 1;
+    if (graphs) {
+      if (!gr.exec[par]) {
+        BTG_TRY(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+        group(par);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(e.stream, &g);
+        const hipError_t ie = ce == hipSuccess ? hipGraphInstantiate(&gr.exec[par], g, nullptr, nullptr, 0) : ce;
+        if (g) (void)hipGraphDestroy(g);
+        BTG_TRY(ie);
+      }
+      BTG_TRY(hipGraphLaunch(gr.exec[par], e.stream));
+    } else {
+      group(rounds);
+    }
+    rounds += k;
     BTG_TRY(hipGetLastError());
     BTG_TRY(hipMemcpyAsync(e.hostState, p.st, sizeof(WState), hipMemcpyDeviceToHost, e.stream));
     BTG_TRY(hipStreamSynchronize(e.stream));
@@ -3063,9 +3101,11 @@ int downloadPolys(Engine& e, int R, std::vector<OutPoly>& polys, std::string& er
 
 // Primary and point shadows share the polygon-output driver; the output
 // buffer grows and the call repeats if it was too small.
+// With download = false the polygons stay on the device (e.out, filtered by
+// outTag == epoch[outRoot]) for a device-side consumer.
 template <class Setup>
 int runPolys(Engine& e, int app, int R, int budget, size_t outCap, const HostCam& cam, Setup setup,
-             std::vector<OutPoly>& polys, RunInfo& info, std::string& err) {
+             std::vector<OutPoly>& polys, RunInfo& info, std::string& err, bool download = true) {
   for (;;) {
     Params p;
     int rc = prepare(e, app, R, budget, p, err);
@@ -3096,7 +3136,7 @@ int runPolys(Engine& e, int app, int R, int budget, size_t outCap, const HostCam
     ri.traceMs = ms;
     if ((rc = readStats(e, ri.stats, &ri.dropped, err))) return rc;
     e.collectLaunches(ri);
-    if ((rc = downloadPolys(e, R, polys, err))) return rc;
+    if (download && (rc = downloadPolys(e, R, polys, err))) return rc;
     info = ri;
     return 0;
   }
@@ -3142,6 +3182,259 @@ int enginePoint(Engine* e, const HostCam& cam, const OutPoly* prim, int64_t nPri
     p.prim = static_cast<const OutPoly*>(e->prim.ptr);
   };
   return runPolys(*e, kPoint, int(nPrim), e->opt.budget, outCap, cam, setup, out, info, err);
+}
+
+// ------------------------------------------------------------ rasterization
+namespace {
+
+// x86's double -> int32 conversion (cvttsd2si), which the host rasterizer's
+// int(std::ceil(...)) compiles to: truncation, and INT_MIN for NaN and for
+// values outside the int range (the device's own conversion clamps instead).
+__device__ __forceinline__ int toIntHost(double v) {
+  return (v > -2147483649.0 && v < 2147483648.0) ? int(v) : int(0x80000000u);
+}
+
+// The edges of a polygon as half-planes A x + B y + C >= 0 in pixel
+// coordinates, and its row range: rasterize (src/render/render.cpp:49-95)
+// with one sample at each pixel centre. Corners go to pixel coordinates in
+// float, as the reference computes them, everything after in double.
+struct RasterEdges {
+  double A[4], B[4], C[4];
+  int n, y0, y1;
+};
+
+__device__ __forceinline__ void rasterEdges(const OutPoly& pg, float halfW, float halfH, int W, int H, RasterEdges& r) {
+  double px[4], py[4];
+  double ylo = 1e300, yhi = -1e300;
+  r.n = pg.n;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float x = (pg.x[i] / halfW + 1) * 0.5f * float(W);
+    const float y = (pg.y[i] / halfH + 1) * 0.5f * float(H);
+    px[i] = double(x);
+    py[i] = double(y);
+    if (i < pg.n) {
+      ylo = py[i] < ylo ? py[i] : ylo;  // std::min(ylo, py)
+      yhi = yhi < py[i] ? py[i] : yhi;  // std::max(yhi, py)
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    // j = i + 1 wrapping at n; corners past n are never used
+    const double pxj = i + 1 == pg.n ? px[0] : (i == 3 ? px[0] : px[(i + 1) & 3]);
+    const double pyj = i + 1 == pg.n ? py[0] : (i == 3 ? py[0] : py[(i + 1) & 3]);
+    r.A[i] = -(pyj - py[i]);
+    r.B[i] = pxj - px[i];
+    r.C[i] = -(r.A[i] * px[i] + r.B[i] * py[i]);
+  }
+  const int y0 = toIntHost(ceil(ylo - 0.5)), y1 = toIntHost(floor(yhi - 0.5));
+  r.y0 = y0 > 0 ? y0 : 0;
+  r.y1 = y1 < H - 1 ? y1 : H - 1;
+}
+
+// Rows a polygon covers (0 if it is filtered out): valid output of its pass
+// (tag == epoch of its root, when the epoch filter is given), a hit polygon
+// unless this is the shadow pass, 3 or 4 corners.
+__global__ void rasterRowsKernel(const OutPoly* polys, const int* root, const int* tag, const int* epoch, int n,
+                                 float halfW, float halfH, int W, int H, int shadow, uint32_t* rows) {
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const OutPoly pg = polys[i];
+  bool ok = pg.n >= 3 && pg.n <= 4 && (shadow || pg.tri >= 0);
+  if (ok && epoch) ok = tag[i] == epoch[root[i]];
+  uint32_t c = 0;
+  if (ok) {
+    RasterEdges r;
+    rasterEdges(pg, halfW, halfH, W, H, r);
+    if (r.y1 >= r.y0) c = uint32_t(r.y1 - r.y0 + 1);
+  }
+  rows[i] = c;
+}
+
+// One (polygon, row) span per thread: the polygon is the last one whose first
+// span offset is <= the item (binary search over the exclusive scan).
+__global__ void rasterSpansKernel(const OutPoly* polys, const uint32_t* offs, int n, uint32_t total, float halfW,
+                                  float halfH, int W, int H, int shadow, int32_t* ids, uint8_t* mask) {
+  const uint32_t j = uint32_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const OutPoly pg = polys[lo];
+  RasterEdges r;
+  rasterEdges(pg, halfW, halfH, W, H, r);
+  const int y = r.y0 + int(j - offs[lo]);
+  const double sy = y + 0.5;
+  double xl = -1e300, xr = 1e300;
+  bool empty = false;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= r.n || empty) break;
+    const double rr = r.B[i] * sy + r.C[i];
+    if (r.A[i] > 0) {
+      const double v = -rr / r.A[i];
+      xl = xl < v ? v : xl;  // std::max(xl, v)
+    } else if (r.A[i] < 0) {
+      const double v = -rr / r.A[i];
+      xr = v < xr ? v : xr;  // std::min(xr, v)
+    } else if (rr < 0) {
+      empty = true;
+    }
+  }
+  if (empty || xl > xr) return;
+  int x0 = toIntHost(ceil(xl - 0.5)), x1 = toIntHost(floor(xr - 0.5));
+  x0 = x0 > 0 ? x0 : 0;
+  x1 = x1 < W - 1 ? x1 : W - 1;
+  const size_t row = size_t(y) * size_t(W);
+  for (int x = x0; x <= x1; ++x) {
+    if (shadow)
+      mask[row + size_t(x)] = 1;
+    else
+      atomicMax(&ids[row + size_t(x)], pg.tri);  // polygons sort by tri first: the canonical last writer
+  }
+}
+
+__global__ void fillInt(int32_t* p, size_t n, int32_t v) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+__global__ void fillByte(uint8_t* p, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0;
+}
+
+// Valid polygons of the last pass (tag == epoch[root]), compacted.
+__global__ void compactPolys(const OutPoly* polys, const int* root, const int* tag, const int* epoch, int n,
+                             OutPoly* dst, int* count) {
+  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || tag[i] != epoch[root[i]]) return;
+  dst[atomicAdd(count, 1)] = polys[i];
+}
+
+// Rasterizes n device polygons (epoch filter optional) into the device image
+// `img` (ids or mask, W x H) and copies it to the host.
+int rasterDevice(Engine& e, const HostCam& cam, const OutPoly* polys, const int* root, const int* tag,
+                 const int* epoch, int n, bool shadow, void* hostOut, std::string& err) {
+  const size_t np = size_t(cam.W) * size_t(cam.H);
+  int rc;
+  if ((rc = e.ensure(e.rasterImg, np * (shadow ? 1 : 4), err)) ||
+      (rc = e.ensure(e.rasterRows, size_t(std::max(n, 1)) * 8 + 16, err)))
+    return rc;
+  const int fillBlocks = int(std::min<size_t>((np + 255) / 256, 4096));
+  if (shadow)
+    fillByte<<<fillBlocks, 256, 0, e.stream>>>(static_cast<uint8_t*>(e.rasterImg.ptr), np);
+  else
+    fillInt<<<fillBlocks, 256, 0, e.stream>>>(static_cast<int32_t*>(e.rasterImg.ptr), np, -1);
+  BTG_TRY(hipGetLastError());
+  if (n > 0) {
+    uint32_t* rows = static_cast<uint32_t*>(e.rasterRows.ptr);
+    uint32_t* offs = rows + n;
+    rasterRowsKernel<<<(n + 255) / 256, 256, 0, e.stream>>>(polys, root, tag, epoch, n, cam.halfW, cam.halfH, cam.W,
+                                                            cam.H, shadow ? 1 : 0, rows);
+    BTG_TRY(hipGetLastError());
+    size_t tmpBytes = 0;
+    BTG_TRY(rocprim::exclusive_scan(nullptr, tmpBytes, rows, offs, uint32_t(0), size_t(n), rocprim::plus<uint32_t>(),
+                                    e.stream));
+    if ((rc = e.ensure(e.rasterTmp, tmpBytes, err))) return rc;
+    BTG_TRY(rocprim::exclusive_scan(e.rasterTmp.ptr, tmpBytes, rows, offs, uint32_t(0), size_t(n),
+                                    rocprim::plus<uint32_t>(), e.stream));
+    uint32_t last[2] = {0, 0};
+    BTG_TRY(hipMemcpyAsync(&last[0], rows + (n - 1), 4, hipMemcpyDeviceToHost, e.stream));
+    BTG_TRY(hipMemcpyAsync(&last[1], offs + (n - 1), 4, hipMemcpyDeviceToHost, e.stream));
+    BTG_TRY(hipStreamSynchronize(e.stream));
+    const uint64_t total = uint64_t(last[0]) + uint64_t(last[1]);
+    if (total > 0xffffffffull) {
+      err = "too many polygon rows to rasterize";
+      return BTG_ERR_LIMIT_CODE;
+    }
+    if (total > 0) {
+      rasterSpansKernel<<<unsigned((total + 255) / 256), 256, 0, e.stream>>>(
+          polys, offs, n, uint32_t(total), cam.halfW, cam.halfH, cam.W, cam.H, shadow ? 1 : 0,
+          static_cast<int32_t*>(e.rasterImg.ptr), static_cast<uint8_t*>(e.rasterImg.ptr));
+      BTG_TRY(hipGetLastError());
+    }
+  }
+  BTG_TRY(hipMemcpyAsync(hostOut, e.rasterImg.ptr, np * (shadow ? 1 : 4), hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipStreamSynchronize(e.stream));
+  return 0;
+}
+
+}  // namespace
+
+int engineRaster(Engine* e, const HostCam& cam, const OutPoly* polys, int64_t n, bool shadow, int32_t* ids,
+                 uint8_t* mask, std::string& err) {
+  if (cam.W <= 0 || cam.H <= 0 || n < 0 || n > (1LL << 30) || (shadow ? !mask : !ids)) {
+    err = "invalid raster arguments";
+    return BTG_ERR_ARG_CODE;
+  }
+  int rc = e->ensure(e->prim, size_t(std::max<int64_t>(n, 1)) * sizeof(OutPoly), err);
+  if (rc) return rc;
+  if (n > 0)
+    BTG_TRY(hipMemcpyAsync(e->prim.ptr, polys, size_t(n) * sizeof(OutPoly), hipMemcpyHostToDevice, e->stream));
+  return rasterDevice(*e, cam, static_cast<const OutPoly*>(e->prim.ptr), nullptr, nullptr, nullptr, int(n), shadow,
+                      shadow ? static_cast<void*>(mask) : static_cast<void*>(ids), err);
+}
+
+int engineImages(Engine* e, const HostCam& cam, bool cull, int tile, const F3* light, int32_t* ids, uint8_t* mask,
+                 RunInfo& prim, RunInfo& point, double& rasterMs, std::string& err) {
+  if (tile <= 0 || cam.W <= 0 || cam.H <= 0 || !ids || (light && !mask)) {
+    err = "invalid image arguments";
+    return BTG_ERR_ARG_CODE;
+  }
+  const int tilesW = (cam.W + tile - 1) / tile, tilesH = (cam.H + tile - 1) / tile, R = tilesW * tilesH;
+  const size_t outCap = std::max<size_t>(size_t(1) << 16, size_t(cam.W) * size_t(cam.H) * 2);
+  auto setup = [&](Params& p) {
+    p.tile = tile;
+    p.tilesW = tilesW;
+    p.cull = cull ? 1 : 0;
+  };
+  std::vector<OutPoly> none;
+  int rc = runPolys(*e, kPrimary, R, e->opt.primaryBudget, outCap, cam, setup, none, prim, err, false);
+  if (rc) return rc;
+  const int nOut = std::min(e->hostState->outCount, int(e->out.bytes / sizeof(OutPoly)));
+  auto t0 = std::chrono::steady_clock::now();
+  if ((rc = rasterDevice(*e, cam, static_cast<const OutPoly*>(e->out.ptr), static_cast<const int*>(e->outRoot.ptr),
+                         static_cast<const int*>(e->outTag.ptr), static_cast<const int*>(e->epoch.ptr), nOut, false,
+                         ids, err)))
+    return rc;
+  rasterMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (!light) return 0;
+  // the primary hits become the point pass's roots without leaving the device
+  if ((rc = e->ensure(e->prim, size_t(std::max(nOut, 1)) * sizeof(OutPoly), err)) ||
+      (rc = e->ensure(e->listCount, 4, err)))
+    return rc;
+  clearInt<<<1, 64, 0, e->stream>>>(static_cast<int*>(e->listCount.ptr));
+  if (nOut > 0)
+    compactPolys<<<(nOut + 255) / 256, 256, 0, e->stream>>>(
+        static_cast<const OutPoly*>(e->out.ptr), static_cast<const int*>(e->outRoot.ptr),
+        static_cast<const int*>(e->outTag.ptr), static_cast<const int*>(e->epoch.ptr), nOut,
+        static_cast<OutPoly*>(e->prim.ptr), static_cast<int*>(e->listCount.ptr));
+  BTG_TRY(hipGetLastError());
+  int nPrim = 0;
+  BTG_TRY(hipMemcpyAsync(&nPrim, e->listCount.ptr, 4, hipMemcpyDeviceToHost, e->stream));
+  BTG_TRY(hipStreamSynchronize(e->stream));
+  const size_t np = size_t(cam.W) * size_t(cam.H);
+  if (nPrim == 0) {
+    std::memset(mask, 0, np);
+    return 0;
+  }
+  const size_t pcap = std::max<size_t>(size_t(1) << 16, size_t(nPrim) * 4);
+  auto psetup = [&](Params& p) {
+    p.pointLight = *light;
+    p.prim = static_cast<const OutPoly*>(e->prim.ptr);
+  };
+  if ((rc = runPolys(*e, kPoint, nPrim, e->opt.budget, pcap, cam, psetup, none, point, err, false))) return rc;
+  const int nShadow = std::min(e->hostState->outCount, int(e->out.bytes / sizeof(OutPoly)));
+  t0 = std::chrono::steady_clock::now();
+  rc = rasterDevice(*e, cam, static_cast<const OutPoly*>(e->out.ptr), static_cast<const int*>(e->outRoot.ptr),
+                    static_cast<const int*>(e->outTag.ptr), static_cast<const int*>(e->epoch.ptr), nShadow, true, mask,
+                    err);
+  rasterMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
 }
 
 }  // namespace btg

@@ -36,7 +36,7 @@ class btg_scene_desc(C.Structure):
 class btg_options(C.Structure):
     _fields_ = [("budget", C.c_int), ("primaryBudget", C.c_int), ("capacity", C.c_int64),
                 ("roundsPerCheck", C.c_int), ("profileKernels", C.c_int), ("leafCap", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("useGraphs", C.c_int)]
 
 
 class btg_camera(C.Structure):
@@ -94,7 +94,7 @@ assert POLY_DTYPE.itemsize == 44
 EXPORTS = ["btg_device_query", "btg_create", "btg_destroy", "btg_last_error", "btg_get_options", "btg_set_options",
            "btg_soft", "btg_scene_add", "btg_scene_remove", "btg_scene_bytes", "btg_soft_batch", "btg_autotune", "btg_server_create", "btg_server_submit", "btg_server_wait",
            "btg_server_error", "btg_server_destroy", "btg_polys_count",
-           "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point"]
+           "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point", "btg_raster", "btg_images"]
 
 
 def lib() -> C.CDLL:
@@ -137,6 +137,9 @@ def lib() -> C.CDLL:
                                   C.POINTER(btg_run_info)]
         L.btg_point.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.POINTER(C.c_float), C.c_int,
                                 C.POINTER(vp), C.POINTER(btg_run_info)]
+        L.btg_raster.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.c_int, vp, vp]
+        L.btg_images.argtypes = [vp, C.POINTER(btg_camera), C.c_int, C.c_int, vp, vp, vp, C.POINTER(btg_run_info),
+                                 C.POINTER(btg_run_info), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -300,13 +303,13 @@ class GpuRenderer:
         lib().btg_get_options(self._h, C.byref(o))
         return {"budget": o.budget, "primary_budget": o.primaryBudget, "capacity": o.capacity,
                 "rounds_per_check": o.roundsPerCheck, "profile": bool(o.profileKernels), "leaf_cap": o.leafCap,
-                "precision": o.precision}
+                "precision": o.precision, "graphs": bool(o.useGraphs)}
 
     def set_options(self, **kw) -> None:
         cur = self.options
         cur.update(kw)
         o = btg_options(cur["budget"], cur["primary_budget"], cur["capacity"], cur["rounds_per_check"],
-                        int(cur["profile"]), int(cur["leaf_cap"]), int(cur["precision"]))
+                        int(cur["profile"]), int(cur["leaf_cap"]), int(cur["precision"]), int(cur["graphs"]))
         rc = lib().btg_set_options(self._h, C.byref(o))
         if rc:
             self._err("btg_set_options", rc)
@@ -357,6 +360,31 @@ class GpuRenderer:
         if rc:
             self._err("btg_point", rc)
         return self._take(ph), info_dict(ri)
+
+    def raster(self, cam14, W: int, H: int, polys: np.ndarray, shadow: bool = False) -> np.ndarray:
+        """rasterize (src/render/render.cpp:49-95) on the device: an int32 ID image
+        (shadow = False) or a uint8 mask (shadow = True), H x W."""
+        polys = np.ascontiguousarray(polys, dtype=POLY_DTYPE)
+        out = np.empty((H, W), np.uint8 if shadow else np.int32)
+        rc = lib().btg_raster(self._h, C.byref(make_camera(cam14, W, H)), polys.ctypes.data if len(polys) else None,
+                              len(polys), int(shadow), None if shadow else out.ctypes.data,
+                              out.ctypes.data if shadow else None)
+        if rc:
+            self._err("btg_raster", rc)
+        return out
+
+    def images(self, cam14, W: int, H: int, light3=None, cull: bool = False, tile: int = 16):
+        """Primary IDs and (light3 given) the point-shadow mask of one frame, both
+        rasterized on the device: (ids, mask or None, info)."""
+        ids = np.empty((H, W), np.int32)
+        mask = np.empty((H, W), np.uint8) if light3 is not None else None
+        lt = (C.c_float * 3)(*[float(v) for v in light3]) if light3 is not None else None
+        a, b, rs = btg_run_info(), btg_run_info(), C.c_double()
+        rc = lib().btg_images(self._h, C.byref(make_camera(cam14, W, H)), int(cull), int(tile), lt, ids.ctypes.data,
+                              mask.ctypes.data if mask is not None else None, C.byref(a), C.byref(b), C.byref(rs))
+        if rc:
+            self._err("btg_images", rc)
+        return ids, mask, {"primary": info_dict(a), "point": info_dict(b), "raster_s": rs.value}
 
 
 def _take_polys(ph) -> np.ndarray:

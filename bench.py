@@ -57,9 +57,12 @@ WORKLOADS = {
 }
 METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request latency"
 # Measured HBM bytes per trace-kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-# scaled by a 256 MiB copy: profiles/trace_plant512_hbm_traffic.txt), valid
-# for the default options on that workload; other runs report null.
-TRAFFIC = {("plant", 512): 196594220}
+# scaled by a 256 MiB copy: profiles/trace_plant512_hbm_traffic.txt and
+# profiles/trace_batch64_hbm_traffic.txt), valid for the schedule they were
+# measured with; other runs report null.
+TRAFFIC = {("plant", 512): 198479765}
+# batch64: keyed by the schedule (budget, rounds per check, leaf cap)
+TRAFFIC_BATCH = {(1024, 8, 4): 652811615}
 
 
 def parse():
@@ -69,6 +72,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default: 1 for batch64, 2 otherwise)")
     ap.add_argument("--workload", default="batch64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-serve", action="store_true", help="batch64: skip the served-stream measurement")
+    ap.add_argument("--no-plain", action="store_true", help="batch64: skip the plain form (one call per job)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batch64: skip the autotuner and keep the options given (for profiling the batched steps)")
     ap.add_argument("--res", type=int, default=0)
     ap.add_argument("--cpu-stride", type=int, default=1, help="plant / room: the CPU baseline traces every k-th row")
     ap.add_argument("--no-cpu", action="store_true")
@@ -257,20 +263,24 @@ def bench_batch(args) -> int:
                    cam=camera(scenes[j["name"]].spec()["view"], max(32, j["W"] // 4), max(32, j["H"] // 4)))
               for j in jobs]
     t1 = time.perf_counter()
-    tuned, trials = r.autotune(sample, exact=True, max_seconds=15.0)
+    if args.no_tune:
+        o = r.options
+        tuned, trials = {k: o[k] for k in ("budget", "rounds_per_check", "leaf_cap")}, []
+    else:
+        tuned, trials = r.autotune(sample, exact=True, max_seconds=15.0)
     tune_s = time.perf_counter() - t1
     print(f"bench.py: autotuned in {tune_s:.1f} s: {tuned}", file=sys.stderr, flush=True)
     r.set_options(**tuned)
     # plain form: one call per job, library defaults
-    t0 = time.perf_counter()
-    plain_beams = 0
-    for j in jobs:
-        info = plain[j["name"]].soft_shadows(j["cam"], j["W"], j["H"], j["light"], exact=True, download=False)["info"]
-        plain_beams += info["stats"]["beams"]
-    plain_s = time.perf_counter() - t0
+    plain_s = None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        for j in jobs:
+            plain[j["name"]].soft_shadows(j["cam"], j["W"], j["H"], j["light"], exact=True, download=False)
+        plain_s = time.perf_counter() - t0
+        print(f"bench.py: plain form {plain_s:.2f} s", file=sys.stderr, flush=True)
     for p_ in plain.values():
         p_.close()
-    print(f"bench.py: plain form {plain_s:.2f} s", file=sys.stderr, flush=True)
     # batched form: the timed steps
     infos = []
     t0 = time.perf_counter()
@@ -287,8 +297,9 @@ def bench_batch(args) -> int:
     roofline = None
     if trace_s > 0:
         achieved = b_trace * args.steps / trace_s / 1e9
+        traffic = TRAFFIC_BATCH.get((tuned["budget"], tuned["rounds_per_check"], tuned["leaf_cap"]))
         roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "traceKernel",
+                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": "traceKernel",
                     "launches_per_step": trace_launches / args.steps,
                     "avg_launch_ms": round(trace_s / max(1, trace_launches) * 1e3, 4), "alg_bytes_per_step": int(b_trace)}
     value = px * args.steps / wall
@@ -301,9 +312,9 @@ def bench_batch(args) -> int:
                    "jobs": len(jobs), "pixels_per_step": px, "tris": {n: scenes[n].num_tris for n in scenes},
                    "draw_seed": 64, "scene_seed": 2007},
         "device": dev["name"], "arch": dev["arch"],
-        "plain": {"value": round(px / plain_s, 1), "unit": "pixels/s", "seconds": round(plain_s, 3),
-                  "form": "one softShadows call per job"},
-        "batched_over_plain": round(plain_s / (wall / args.steps), 3),
+        "plain": None if plain_s is None else {"value": round(px / plain_s, 1), "unit": "pixels/s",
+                                               "seconds": round(plain_s, 3), "form": "one softShadows call per job"},
+        "batched_over_plain": None if plain_s is None else round(plain_s / (wall / args.steps), 3),
         "beams_per_s": round(st["beams"] * args.steps / wall, 1),
         "rounds": infos[-1]["rounds"], "killed_roots": infos[-1]["killed_roots"],
         "retry_passes": infos[-1]["retry_passes"],
@@ -312,8 +323,8 @@ def bench_batch(args) -> int:
         "roofline": roofline, "host_kd_build_s": round(build_s, 3),
         "cpu_baseline": None if args.no_cpu else cpu_batch_sample(scenes, jobs, draw),
         "precision": precision,
-        "autotune": {"chosen": tuned, "seconds": round(tune_s, 2), "trials": trials,
-                     "sample": "the 64 jobs at a quarter of their side"},
+        "autotune": None if args.no_tune else {"chosen": tuned, "seconds": round(tune_s, 2), "trials": trials,
+                                               "sample": "the 64 jobs at a quarter of their side"},
         "draw": [[d["scene"][0], d["side"], round(d["light_scale"], 4)] for d in draw],
     }
     if not args.no_serve:

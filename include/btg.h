@@ -91,6 +91,10 @@ typedef struct {
                          a float pair in every apex-relative difference; the Lambert integral sums in double.
                          Results approach bt_render_d's (mean visibility error 100-1000x below the float build's
                          on the test scenes; bench.py reports it). Same throughput as 0. */
+  int useGraphs;      /* 1 (default): each group of roundsPerCheck rounds is captured once per pass into a HIP graph
+                         and replayed (no counterpart in the reference, which launches every kernel from the host,
+                         cuda/wavefront.cu:641-676); 0: plain launches. Results are the same either way.
+                         profileKernels = 1 launches directly (it times every launch with HIP events). */
 } btg_options;
 int btg_get_options(const btg_ctx* ctx, btg_options* opt);
 int btg_set_options(btg_ctx* ctx, const btg_options* opt);
@@ -217,6 +221,26 @@ int btg_primary(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, int can
  * receiver facing away from the light yields its whole primary polygon. */
 int btg_point(btg_ctx* ctx, const btg_camera* cam, const btg_poly* prim, int64_t numPrim, const float* light3,
               int canonical, btg_polys** out, btg_run_info* info);
+
+/* rasterize (src/render/render.cpp:49-95) on the device, one sample at each
+ * pixel centre, bit for bit with the reference's CPU post-pass run over the
+ * polygons in canonical order: shadowPass = 0 fills ids[W*H] with the tri of
+ * the last covering polygon in canonical order (the largest tri, as the order
+ * sorts by tri first; -1 where none; polygons with tri < 0 are skipped);
+ * shadowPass = 1 fills mask[W*H] with 1 where any polygon covers the pixel
+ * centre, else 0. The polygons are uploaded, the image downloaded. */
+int btg_raster(btg_ctx* ctx, const btg_camera* cam, const btg_poly* polys, int64_t count, int shadowPass,
+               int32_t* ids, uint8_t* mask);
+
+/* One frame of primary visibility and (light3 != NULL) point shadows with
+ * both rasterizations on the device (bt_render's primary / point output,
+ * apps/bt_render.cpp:228-247, without the CPU post-pass): ids[W*H] as
+ * btg_raster of btg_primary's polygons, mask[W*H] as btg_raster of
+ * btg_point's output for those polygons. The polygons never leave the
+ * device. primInfo / pointInfo: the two passes (either may be NULL);
+ * rasterSeconds: the two rasterizations with their image downloads. */
+int btg_images(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, const float* light3, int32_t* ids,
+               uint8_t* mask, btg_run_info* primInfo, btg_run_info* pointInfo, double* rasterSeconds);
 
 /* ---------------------------------------------------------------- serving
  * A request server over one context (no counterpart in the reference, which

@@ -120,3 +120,42 @@ def test_autotune_returns_valid_options_and_results_do_not_change(require_gpu, b
     for b, a in zip(before, after):
         for f in ("tri", "vis", "E", "Eu"):
             assert np.array_equal(b[f], a[f]), f
+
+
+def test_graph_replays_equal_direct_launches(require_gpu, batch_renderer):
+    """Rounds replayed from captured HIP graphs (the default) give the same
+    results as direct launches, on every replay: three batched calls in a row
+    with other work between them, then the same batch with graphs off (and
+    with per-launch profiling, which launches directly); primary and
+    point-shadow passes likewise."""
+    from beamtrace2007_amd.host import camera
+    r, ids = batch_renderer
+    jobs = jobs_for(ids)[:5]
+    saved = r.options
+    try:
+        r.set_options(graphs=True, profile=False, rounds_per_check=4)
+        runs = []
+        for k in range(3):
+            outs, info = r.soft_batch(jobs, exact=True)
+            runs.append(outs)
+            r.soft_shadows(jobs[k]["cam"], jobs[k]["W"], jobs[k]["H"], jobs[k]["light"], exact=False)
+        for opts in ({"graphs": False}, {"graphs": True, "profile": True}):
+            r.set_options(**opts)
+            direct, dinfo = r.soft_batch(jobs, exact=True)
+            for outs in runs:
+                for a, b in zip(outs, direct):
+                    for f in ("tri", "vis", "E", "Eu"):
+                        assert np.array_equal(a[f], b[f]), f
+        r.set_options(profile=False)
+        sp = load_scene("room")[3]
+        cam = camera(sp["view"], 96, 64)
+        polys = {}
+        for g in (True, False):
+            r.set_options(graphs=g)
+            hits, _ = r.primary(cam, 96, 64, cull=False, tile=8)
+            shadow, _ = r.point_shadows(cam, 96, 64, hits, sp["point_light"])
+            polys[g] = (hits, shadow)
+        for a, b in zip(polys[True], polys[False]):
+            assert len(a) == len(b) and all(np.array_equal(a[f], b[f]) for f in ("x", "y", "n", "tri", "area"))
+    finally:
+        r.set_options(**saved)

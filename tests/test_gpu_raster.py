@@ -1,0 +1,133 @@
+"""Rasterization on the device (btg_raster, btg_images): the reference's CPU
+post-pass (rasterize, src/render/render.cpp:49-95) run on the GPU.
+
+* bit for bit against the reference's own raster images (golden fixtures
+  written by the reference build from its polygons), through btg_raster;
+* bit for bit against the host rasterizer over the GPU's polygons in
+  canonical order, at the room configuration's full size (512 x 512, 8 x 8
+  tiles) and on the 2M-triangle building at 1024 x 1024 (primary + point
+  shadows), through btg_images, whose polygons never leave the device;
+* against the per-pixel ray caster (the primary-ray pass of soft shadows):
+  beam IDs at pixel centres agree except where a centre lies on an edge.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_scene
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = ["analytic", "room", "random_tris", "sphere"]
+
+
+def golden(name):
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", f"{name}.npz"))
+
+
+def as_polys(arr):
+    from beamtrace2007_amd.gpu import POLY_DTYPE
+    out = np.zeros(len(arr), POLY_DTYPE)
+    out["x"] = arr["xy"][:, 0::2]
+    out["y"] = arr["xy"][:, 1::2]
+    out["n"] = arr["n"]
+    out["tri"] = arr["tri"]
+    return out
+
+
+def host_raster(cam, W, H, polys, shadow):
+    from beamtrace2007_amd.host import rasterize
+    q = {"x": polys["x"], "y": polys["y"], "n": polys["n"], "tri": polys["tri"]}
+    return rasterize(cam, W, H, q, shadow=shadow, canonical=True)
+
+
+_renderers = {}
+
+
+def renderer(name):
+    from beamtrace2007_amd.gpu import GpuRenderer
+    if name not in _renderers:
+        _renderers[name] = GpuRenderer(load_scene(name)[0])
+    return _renderers[name]
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_device_raster_matches_reference_images(require_gpu, name):
+    g = golden(name)
+    r = renderer(name)
+    ids = r.raster(g["cam64"], 64, 64, as_polys(g["prim8_polys"]), shadow=False)
+    assert np.array_equal(ids, g["raster_ids"])
+    mask = r.raster(g["cam64"], 64, 64, as_polys(g["point_polys"]), shadow=True)
+    assert np.array_equal(mask, g["raster_mask"])
+
+
+def test_device_raster_edge_cases(require_gpu):
+    """No polygons; polygons outside the image; a polygon covering all of it;
+    tri < 0 polygons (skipped for IDs, drawn into the mask); odd sizes."""
+    from beamtrace2007_amd.gpu import POLY_DTYPE
+    from beamtrace2007_amd.host import camera
+    r = renderer("room")
+    sp = load_scene("room")[3]
+    W, H = 37, 23
+    cam = camera(sp["view"], W, H)
+    hw, hh = float(cam[12]), float(cam[13])
+    p = np.zeros(4, POLY_DTYPE)
+    quad = [(-2 * hw, -2 * hh), (2 * hw, -2 * hh), (2 * hw, 2 * hh), (-2 * hw, 2 * hh)]
+    p["n"] = 4
+    p["tri"] = [5, -1, 7, 3]
+    for i, (x, y) in enumerate(quad):
+        p["x"][0, i], p["y"][0, i] = x, y
+        p["x"][1, i], p["y"][1, i] = x * 0.25, y * 0.25
+        p["x"][2, i], p["y"][2, i] = x + 10 * hw, y  # entirely right of the image
+        p["x"][3, i], p["y"][3, i] = x * 0.1 + 0.3 * hw, y * 0.1
+    p["n"][3] = 3
+    for shadow in (False, True):
+        assert np.array_equal(r.raster(cam, W, H, p[:0], shadow), host_raster(cam, W, H, p[:0], shadow))
+        assert np.array_equal(r.raster(cam, W, H, p, shadow), host_raster(cam, W, H, p, shadow))
+    with pytest.raises(RuntimeError):
+        r.raster(cam, 0, H, p)
+
+
+@pytest.mark.parametrize("name,W,H,tile", [("room", 512, 512, 8), ("plant", 160, 120, 16), ("sphere", 96, 80, 8)])
+def test_images_equal_host_raster_of_gpu_polygons(require_gpu, name, W, H, tile):
+    """btg_images (polygons stay on the device) against btg_primary /
+    btg_point's canonical polygon lists rasterized on the host."""
+    from beamtrace2007_amd.host import camera
+    sp = load_scene(name)[3]
+    cam = camera(sp["view"], W, H)
+    r = renderer(name)
+    ids, mask, info = r.images(cam, W, H, sp["point_light"], tile=tile)
+    hits, _ = r.primary(cam, W, H, cull=False, tile=tile)
+    shadow, _ = r.point_shadows(cam, W, H, hits, sp["point_light"])
+    assert np.array_equal(ids, host_raster(cam, W, H, hits, False))
+    assert np.array_equal(mask, host_raster(cam, W, H, shadow, True))
+    assert info["raster_s"] > 0
+    ids2, none, _ = r.images(cam, W, H, None, tile=tile)
+    assert none is None and np.array_equal(ids2, ids)
+
+
+def test_building_frame_at_full_size(require_gpu):
+    """The 2M-triangle building at 1024 x 1024 (16 x 16 tiles) with its point
+    light: device images equal the host raster of the device's polygons; beam
+    IDs equal the ray caster's at >= 99.9% of pixels; shadowed pixels are hit
+    pixels."""
+    from beamtrace2007_amd.gpu import GpuRenderer
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene("building")
+    W = H = 1024
+    cam = camera(sp["view"], W, H)
+    r = GpuRenderer(s)
+    print("building", s.num_tris, flush=True)
+    ids, mask, info = r.images(cam, W, H, sp["point_light"], tile=16)
+    hits, _ = r.primary(cam, W, H, cull=False, tile=16)
+    shadow, _ = r.point_shadows(cam, W, H, hits, sp["point_light"])
+    assert np.array_equal(ids, host_raster(cam, W, H, hits, False))
+    assert np.array_equal(mask, host_raster(cam, W, H, shadow, True))
+    tiny = np.array(sp["area_light"], np.float32)
+    tiny[3:] *= np.float32(1e-3)  # only the primary-ray pass matters here
+    rays = r.soft_shadows(cam, W, H, tiny, exact=False)["tri"]
+    assert np.mean(ids == rays) >= 0.999
+    assert np.mean((mask == 0) | (ids >= 0)) >= 0.9999
+    assert 0.01 < np.mean(mask) < 0.99
+    r.close()
