@@ -11,12 +11,12 @@
 #include <unordered_map>
 #include <vector>
 
+#include "../host/scene_file.h"
 #include "engine.h"
 
 struct btg_ctx {
   btg::Engine* engine = nullptr;
   std::string err;
-  btg_scene_desc desc{};
   float offset = 0;
 };
 
@@ -200,7 +200,6 @@ int btg_create(btg_ctx** out, const btg_scene_desc* d) {
       delete c;
       return failWith(nullptr, BTG_ERR_DEVICE, err);
     }
-    c->desc = *d;
     *out = c;
     return BTG_OK;
   } catch (const std::bad_alloc&) {
@@ -217,6 +216,62 @@ int btg_scene_add(btg_ctx* ctx, const btg_scene_desc* d, int* sceneId) {
     if (id < 0) return id;
     *sceneId = id;
     return BTG_OK;
+  } catch (const std::bad_alloc&) {
+    return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
+  }
+}
+
+namespace {
+// A scene file's arrays as a scene description (the description borrows them).
+int readSceneFile(const char* path, btsf::Data& d, btg_scene_desc& s, std::string& err) {
+  if (!path) {
+    err = "null path";
+    return BTG_ERR_ARG;
+  }
+  if (!btsf::read(path, d, err)) return BTG_ERR_ARG;
+  if (d.h.numNodes == 0) {
+    err = std::string("scene file ") + path + " holds no kd-tree";
+    return BTG_ERR_ARG;
+  }
+  s = btg_scene_desc{};
+  s.pos = d.pos.data();
+  s.numVerts = d.h.numVerts;
+  s.tri3 = d.tri3.data();
+  s.triN = d.triN.data();
+  s.numTris = d.h.numTris;
+  s.nodes = d.nodes.data();
+  s.numNodes = d.h.numNodes;
+  s.triIndices = d.triIndices.data();
+  s.numRefs = d.h.numRefs;
+  s.parent = d.parent.data();
+  s.nodeBox = d.nodeBox.data();
+  std::memcpy(s.kdBounds, d.h.kdBounds, sizeof s.kdBounds);
+  std::memcpy(s.sceneBounds, d.h.sceneBounds, sizeof s.sceneBounds);
+  return BTG_OK;
+}
+}  // namespace
+
+int btg_create_file(btg_ctx** out, const char* path) {
+  if (!out) return failWith(nullptr, BTG_ERR_ARG, "null argument");
+  *out = nullptr;
+  try {
+    btsf::Data d;
+    btg_scene_desc s;
+    std::string err;
+    if (int rc = readSceneFile(path, d, s, err)) return failWith(nullptr, rc, err);
+    return btg_create(out, &s);
+  } catch (const std::bad_alloc&) {
+    return failWith(nullptr, BTG_ERR_MEMORY, "host allocation failed");
+  }
+}
+
+int btg_scene_add_file(btg_ctx* ctx, const char* path, int* sceneId) {
+  if (!ctx || !sceneId) return failWith(ctx, BTG_ERR_ARG, "null argument");
+  try {
+    btsf::Data d;
+    btg_scene_desc s;
+    if (int rc = readSceneFile(path, d, s, ctx->err)) return rc;
+    return btg_scene_add(ctx, &s, sceneId);
   } catch (const std::bad_alloc&) {
     return failWith(ctx, BTG_ERR_MEMORY, "host allocation failed");
   }

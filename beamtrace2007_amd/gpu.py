@@ -94,7 +94,8 @@ assert POLY_DTYPE.itemsize == 44
 EXPORTS = ["btg_device_query", "btg_create", "btg_destroy", "btg_last_error", "btg_get_options", "btg_set_options",
            "btg_soft", "btg_scene_add", "btg_scene_remove", "btg_scene_bytes", "btg_soft_batch", "btg_autotune", "btg_server_create", "btg_server_submit", "btg_server_wait",
            "btg_server_error", "btg_server_destroy", "btg_polys_count",
-           "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point", "btg_raster", "btg_images"]
+           "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point", "btg_raster", "btg_images",
+           "btg_create_file", "btg_scene_add_file"]
 
 
 def lib() -> C.CDLL:
@@ -138,6 +139,8 @@ def lib() -> C.CDLL:
         L.btg_point.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.POINTER(C.c_float), C.c_int,
                                 C.POINTER(vp), C.POINTER(btg_run_info)]
         L.btg_raster.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.c_int, vp, vp]
+        L.btg_create_file.argtypes = [C.POINTER(vp), C.c_char_p]
+        L.btg_scene_add_file.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
         L.btg_images.argtypes = [vp, C.POINTER(btg_camera), C.c_int, C.c_int, vp, vp, vp, C.POINTER(btg_run_info),
                                  C.POINTER(btg_run_info), C.POINTER(C.c_double)]
         _lib = L
@@ -216,18 +219,30 @@ class GpuRenderer:
     """The reference's GpuRenderer over one device context. The scene must
     have its kd-tree built (Scene.build_kd)."""
 
-    def __init__(self, scene: Scene):
+    def __init__(self, scene):
+        """`scene`: a Scene with its kd-tree built, or the path of a resident
+        scene file (Scene.save), uploaded without rebuilding the tree."""
         L = lib()
-        d, keep = _scene_desc(scene)
         h = C.c_void_p()
-        if L.btg_create(C.byref(h), C.byref(d)) != 0:
-            raise RuntimeError("btg_create: " + L.btg_last_error(None).decode())
+        if isinstance(scene, (str, os.PathLike)):
+            if L.btg_create_file(C.byref(h), os.fsencode(scene)) != 0:
+                raise RuntimeError("btg_create_file: " + L.btg_last_error(None).decode())
+        else:
+            d, keep = _scene_desc(scene)
+            if L.btg_create(C.byref(h), C.byref(d)) != 0:
+                raise RuntimeError("btg_create: " + L.btg_last_error(None).decode())
         self._h = h
 
-    def add_scene(self, scene: Scene) -> int:
-        """Makes another scene resident (its kd-tree built); returns its id for soft_batch jobs."""
-        d, keep = _scene_desc(scene)
+    def add_scene(self, scene) -> int:
+        """Makes another scene resident (a Scene with its kd-tree built, or a
+        resident scene file's path); returns its id for soft_batch jobs."""
         sid = C.c_int(-1)
+        if isinstance(scene, (str, os.PathLike)):
+            rc = lib().btg_scene_add_file(self._h, os.fsencode(scene), C.byref(sid))
+            if rc:
+                self._err("btg_scene_add_file", rc)
+            return sid.value
+        d, keep = _scene_desc(scene)
         rc = lib().btg_scene_add(self._h, C.byref(d), C.byref(sid))
         if rc:
             self._err("btg_scene_add", rc)

@@ -40,6 +40,8 @@ def lib() -> C.CDLL:
         L.bth_kd_sizes.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_int)]
         L.bth_kd_arrays.argtypes = [vp, C.POINTER(vp), C.POINTER(u32p), C.POINTER(u32p), C.POINTER(f32p), f32p]
         L.bth_camera.argtypes = [f32p, C.c_int, C.c_int, f32p]
+        L.bth_scene_save.argtypes = [vp, C.c_char_p]
+        L.bth_scene_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.bth_rasterize.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, i32p, i32p, i64, C.c_int, C.c_int, i32p,
                                     C.POINTER(C.c_uint8)]
         _lib = L
@@ -90,6 +92,18 @@ class Scene:
         h = C.c_void_p()
         _check(lib().bth_scene_from_arrays(fptr(pos), len(pos), u32ptr(tris), len(tris), C.byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def load(cls, path: str) -> "Scene":
+        """A scene (and its kd-tree, when the file holds one) from a resident
+        scene file written by `save`; the file's hash and indices are checked."""
+        h = C.c_void_p()
+        _check(lib().bth_scene_load(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    def save(self, path: str) -> None:
+        """Writes the resident scene file (.btgs): scene, view and lights, kd-tree."""
+        _check(lib().bth_scene_save(self._h, os.fsencode(path)))
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and self._h.value and _lib is not None:

@@ -9,6 +9,7 @@
 
 #include "../../include/bth.h"
 #include "scene.h"
+#include "scene_file.h"
 
 using namespace bth;
 
@@ -153,6 +154,91 @@ int bth_kd_arrays(const bth_scene* h, const void** nodes, const uint32_t** triIn
     bounds6[3 + k] = h->kd.bounds.hi[k];
   }
   return 0;
+}
+
+int bth_scene_save(const bth_scene* h, const char* path) {
+  if (!h || !path) return fail("null argument");
+  static_assert(sizeof(V3) == 12 && sizeof(KdNode) == 12 && sizeof(Box3) == 24, "array layouts");
+  btsf::View v;
+  v.h.specFlags = uint32_t(h->spec.hasPoint ? 1 : 0) | (h->spec.hasArea ? 2 : 0) | (h->spec.cull ? 4 : 0);
+  v.h.numVerts = int64_t(h->s.pos.size());
+  v.h.numTris = int64_t(h->s.tris.size());
+  v.h.numNodes = h->hasKd ? int64_t(h->kd.nodes.size()) : 0;
+  v.h.numRefs = h->hasKd ? int64_t(h->kd.triIndices.size()) : 0;
+  v.h.kdDepth = h->hasKd ? h->kd.depth : 0;
+  for (int k = 0; k < 3; ++k) {
+    v.h.sceneBounds[k] = h->s.bounds.lo[k];
+    v.h.sceneBounds[3 + k] = h->s.bounds.hi[k];
+    v.h.kdBounds[k] = h->hasKd ? h->kd.bounds.lo[k] : 0.0f;
+    v.h.kdBounds[3 + k] = h->hasKd ? h->kd.bounds.hi[k] : 0.0f;
+  }
+  bth_scene_spec(h, v.h.view, v.h.point, v.h.light);
+  v.pos = reinterpret_cast<const float*>(h->s.pos.data());
+  v.tri3 = reinterpret_cast<const uint32_t*>(h->s.tris.data());
+  v.triN = reinterpret_cast<const float*>(h->s.triN.data());
+  v.triArea = h->s.triArea.data();
+  if (h->hasKd) {
+    v.nodes = h->kd.nodes.data();
+    v.triIndices = h->kd.triIndices.data();
+    v.parent = h->kd.parent.data();
+    v.nodeBox = reinterpret_cast<const float*>(h->kd.nodeBox.data());
+  }
+  std::string err;
+  return btsf::write(path, v, err) ? 0 : fail(err);
+}
+
+int bth_scene_load(const char* path, bth_scene** out) {
+  if (!path || !out) return fail("null argument");
+  try {
+    btsf::Data d;
+    std::string err;
+    if (!btsf::read(path, d, err)) return fail(err);
+    const btsf::Header& hd = d.h;
+    for (uint32_t i : d.tri3)
+      if (i >= uint64_t(hd.numVerts)) return fail("scene file: vertex index out of range");
+    for (uint32_t i : d.triIndices)
+      if (i >= uint64_t(hd.numTris)) return fail("scene file: leaf triangle out of range");
+    bth_scene* h = new bth_scene();
+    h->s.pos.resize(size_t(hd.numVerts));
+    std::memcpy(h->s.pos.data(), d.pos.data(), d.pos.size() * 4);
+    h->s.tris.resize(size_t(hd.numTris));
+    std::memcpy(h->s.tris.data(), d.tri3.data(), d.tri3.size() * 4);
+    h->s.triN.resize(size_t(hd.numTris));
+    std::memcpy(h->s.triN.data(), d.triN.data(), d.triN.size() * 4);
+    h->s.triArea = std::move(d.triArea);
+    for (int k = 0; k < 3; ++k) {
+      h->s.bounds.lo[k] = hd.sceneBounds[k];
+      h->s.bounds.hi[k] = hd.sceneBounds[3 + k];
+    }
+    SceneSpec& sp = h->spec;
+    sp.view = viewFrom(hd.view);
+    sp.hasPoint = (hd.specFlags & 1) != 0;
+    sp.hasArea = (hd.specFlags & 2) != 0;
+    sp.cull = (hd.specFlags & 4) != 0;
+    sp.pointLight = V3(hd.point[0], hd.point[1], hd.point[2]);
+    sp.lightC = V3(hd.light[0], hd.light[1], hd.light[2]);
+    sp.lightU = V3(hd.light[3], hd.light[4], hd.light[5]);
+    sp.lightV = V3(hd.light[6], hd.light[7], hd.light[8]);
+    if (hd.numNodes > 0) {
+      KdTree& kd = h->kd;
+      kd.nodes.resize(size_t(hd.numNodes));
+      std::memcpy(kd.nodes.data(), d.nodes.data(), d.nodes.size());
+      kd.triIndices = std::move(d.triIndices);
+      kd.parent = std::move(d.parent);
+      kd.nodeBox.resize(size_t(hd.numNodes));
+      std::memcpy(kd.nodeBox.data(), d.nodeBox.data(), d.nodeBox.size() * 4);
+      for (int k = 0; k < 3; ++k) {
+        kd.bounds.lo[k] = hd.kdBounds[k];
+        kd.bounds.hi[k] = hd.kdBounds[3 + k];
+      }
+      kd.depth = hd.kdDepth;
+      h->hasKd = true;
+    }
+    *out = h;
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e.what());
+  }
 }
 
 void bth_camera(const float* view10, int W, int H, float* cam14) {

@@ -157,6 +157,11 @@ int btg_soft(btg_ctx* ctx, const btg_camera* cam, const btg_area_light* light, i
  * returns its id (>= 1); btg_scene_remove frees it (the id may be reused).
  * btg_scene_bytes: device bytes the scene occupies (-1: no such scene). */
 int btg_scene_add(btg_ctx* ctx, const btg_scene_desc* scene, int* sceneId);
+/* The same from a resident scene file (.btgs, written by bth_scene_save,
+ * include/bth.h): the file's hash and indices are verified, its kd-tree is
+ * uploaded as stored (no build). No counterpart in the reference. */
+int btg_create_file(btg_ctx** out, const char* path);
+int btg_scene_add_file(btg_ctx* ctx, const char* path, int* sceneId);
 int btg_scene_remove(btg_ctx* ctx, int sceneId);
 int64_t btg_scene_bytes(const btg_ctx* ctx, int sceneId);
 

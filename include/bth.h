@@ -52,6 +52,15 @@ int bth_kd_sizes(const bth_scene* scene, int64_t* numNodes, int64_t* numRefs, in
 int bth_kd_arrays(const bth_scene* scene, const void** nodes, const uint32_t** triIndices, const uint32_t** parent,
                   const float** nodeBox, float* bounds6);
 
+/* Resident scene format (.btgs; layout in beamtrace2007_amd/host/scene_file.h):
+ * the scene, its view and lights, and (when built) its kd-tree, with a payload
+ * hash. No counterpart in the reference, which rebuilds the tree on every
+ * start (src/accel/kdtree.cpp:237-281). bth_scene_load verifies the hash and
+ * the indices; the loaded scene's kd-tree needs no build. btg_scene_add_file
+ * (include/btg.h) uploads such a file straight into a GPU context. */
+int bth_scene_save(const bth_scene* scene, const char* path);
+int bth_scene_load(const char* path, bth_scene** out);
+
 /* Camera::make (src/render/render.cpp:14-25). view10 as above; cam14 = eye,
  * fwd, right, down (3 floats each), halfW, halfH. */
 void bth_camera(const float* view10, int W, int H, float* cam14);

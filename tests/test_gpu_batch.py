@@ -159,3 +159,33 @@ def test_graph_replays_equal_direct_launches(require_gpu, batch_renderer):
             assert len(a) == len(b) and all(np.array_equal(a[f], b[f]) for f in ("x", "y", "n", "tri", "area"))
     finally:
         r.set_options(**saved)
+
+
+def test_scenes_from_resident_files(require_gpu, tmp_path):
+    """A context created from a scene file and a scene added from one render
+    exactly as the same scenes uploaded from memory."""
+    from beamtrace2007_amd.gpu import GpuRenderer
+    from beamtrace2007_amd.host import camera
+    paths = {}
+    for name in ("room", "plant"):
+        paths[name] = str(tmp_path / f"{name}.btgs")
+        load_scene(name)[0].save(paths[name])
+    a = GpuRenderer(load_scene("room")[0])
+    pid = a.add_scene(load_scene("plant")[0])
+    b = GpuRenderer(paths["room"])
+    fid = b.add_scene(paths["plant"])
+    assert b.scene_bytes(fid) == a.scene_bytes(pid) and b.scene_bytes(0) == a.scene_bytes(0)
+    jobs = []
+    for name, sid in (("room", 0), ("plant", pid)):
+        sp = load_scene(name)[3]
+        jobs.append({"scene": sid, "cam": camera(sp["view"], 48, 40), "W": 48, "H": 40, "light": sp["area_light"]})
+    ra, _ = a.soft_batch(jobs, exact=True)
+    jobs[1]["scene"] = fid
+    rb, _ = b.soft_batch(jobs, exact=True)
+    for x, y in zip(ra, rb):
+        for f in ("tri", "vis", "E", "Eu"):
+            assert np.array_equal(x[f], y[f]), f
+    with pytest.raises(RuntimeError):
+        b.add_scene(str(tmp_path / "missing.btgs"))
+    a.close()
+    b.close()
