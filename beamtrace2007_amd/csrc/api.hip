@@ -7,6 +7,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -456,6 +457,42 @@ bool polyLess(const btg::OutPoly& a, const btg::OutPoly& b) {
   return a.area < b.area;
 }
 
+// The canonical order of a polygon list, sorted on up to 8 threads: parts
+// sorted on their own threads, then merged pairwise level by level. The order
+// is total on distinct polygons and equal polygons are identical bytes, so
+// the list equals a single std::sort's.
+void sortCanonical(std::vector<btg::OutPoly>& v) {
+  const size_t n = v.size();
+  const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  size_t parts = 1;
+  while (parts * 2 <= hw) parts *= 2;
+  if (n < 65536 || parts < 2) {
+    std::sort(v.begin(), v.end(), polyLess);
+    return;
+  }
+  std::vector<size_t> b(parts + 1);
+  for (size_t i = 0; i <= parts; ++i) b[i] = n * i / parts;
+  auto run = [&](auto fn, size_t count) {
+    std::vector<std::thread> th;
+    for (size_t i = 1; i < count; ++i) {
+      try {
+        th.emplace_back(fn, i);
+      } catch (const std::system_error&) {
+        fn(i);  // no thread to spare: this part on the calling thread
+      }
+    }
+    fn(size_t(0));
+    for (auto& t : th) t.join();
+  };
+  run([&](size_t i) { std::sort(v.begin() + long(b[i]), v.begin() + long(b[i + 1]), polyLess); }, parts);
+  for (size_t w = 1; w < parts; w *= 2)
+    run([&](size_t k) {
+      const size_t i = k * 2 * w;
+      std::inplace_merge(v.begin() + long(b[i]), v.begin() + long(b[i + w]), v.begin() + long(b[std::min(i + 2 * w, parts)]),
+                         polyLess);
+    }, parts / (2 * w));
+}
+
 }  // namespace
 
 extern "C" {
@@ -474,7 +511,7 @@ int btg_primary(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, int can
       delete P;
       return rc;
     }
-    if (canonical) std::sort(P->v.begin(), P->v.end(), polyLess);
+    if (canonical) sortCanonical(P->v);
     *out = P;
     fillInfo(ri, now() - t0, info);
     if (info) info->stats.hits = uint64_t(P->v.size());
@@ -535,7 +572,7 @@ int btg_point(btg_ctx* ctx, const btg_camera* cam, const btg_poly* prim, int64_t
       delete P;
       return rc;
     }
-    if (canonical) std::sort(P->v.begin(), P->v.end(), polyLess);
+    if (canonical) sortCanonical(P->v);
     *out = P;
     fillInfo(ri, now() - t0, info);
     return BTG_OK;
@@ -724,14 +761,21 @@ void pointLoop(btg_server* s) {
       const btg::HostCam cam = camOf(r->req.cam);
       rc = btg::enginePrimary(s->pointEngine, cam, false, 16, hits->v, ri, err);
       if (!rc) {
-        // the point pass takes the primary polygons in the order the reference's
-        // renderer hands them over: canonical
-        std::sort(hits->v.begin(), hits->v.end(), polyLess);
-        rc = btg::enginePoint(s->pointEngine, cam, hits->v.data(), int64_t(hits->v.size()),
-                              btg::f3(r->req.pointLight[0], r->req.pointLight[1], r->req.pointLight[2]), polys->v,
-                              ri2, err);
+        // the point pass's output set does not depend on the order of its
+        // roots, so it runs on the device's order while the hits are sorted
+        std::vector<btg::OutPoly> roots = hits->v;
+        std::thread sorter([&] { sortCanonical(hits->v); });
+        try {
+          rc = btg::enginePoint(s->pointEngine, cam, roots.data(), int64_t(roots.size()),
+                                btg::f3(r->req.pointLight[0], r->req.pointLight[1], r->req.pointLight[2]),
+                                polys->v, ri2, err);
+        } catch (...) {
+          sorter.join();
+          throw;
+        }
+        sorter.join();
       }
-      if (!rc) std::sort(polys->v.begin(), polys->v.end(), polyLess);
+      if (!rc) sortCanonical(polys->v);
     } catch (const std::bad_alloc&) {
       rc = BTG_ERR_MEMORY;
       err = "host allocation failed";

@@ -3101,6 +3101,14 @@ int downloadPolys(Engine& e, int R, std::vector<OutPoly>& polys, std::string& er
 
 // Primary and point shadows share the polygon-output driver; the output
 // buffer grows and the call repeats if it was too small.
+// Output polygon capacity for a pass: 2^20 at least (the visible fragments
+// of a scene, not the image size, set the count: 230K for the conference hall
+// at any resolution), and never less than an earlier pass grew the buffer to,
+// so that a pass is not traced twice for want of room.
+size_t outCapFor(const Engine& e, size_t want) {
+  return std::max({size_t(1) << 20, want, e.out.bytes / sizeof(OutPoly)});
+}
+
 // With download = false the polygons stay on the device (e.out, filtered by
 // outTag == epoch[outRoot]) for a device-side consumer.
 template <class Setup>
@@ -3151,7 +3159,7 @@ int enginePrimary(Engine* e, const HostCam& cam, bool cull, int tile, std::vecto
     return BTG_ERR_ARG_CODE;
   }
   const int tilesW = (cam.W + tile - 1) / tile, tilesH = (cam.H + tile - 1) / tile;
-  const size_t outCap = std::max<size_t>(size_t(1) << 16, size_t(cam.W) * size_t(cam.H) * 2);
+  const size_t outCap = outCapFor(*e, size_t(cam.W) * size_t(cam.H) * 2);
   auto setup = [&](Params& p) {
     p.tile = tile;
     p.tilesW = tilesW;
@@ -3176,7 +3184,7 @@ int enginePoint(Engine* e, const HostCam& cam, const OutPoly* prim, int64_t nPri
   int rc = e->ensure(e->prim, size_t(nPrim) * sizeof(OutPoly), err);
   if (rc) return rc;
   BTG_TRY(hipMemcpyAsync(e->prim.ptr, prim, size_t(nPrim) * sizeof(OutPoly), hipMemcpyHostToDevice, e->stream));
-  const size_t outCap = std::max<size_t>(size_t(1) << 16, size_t(nPrim) * 4);
+  const size_t outCap = outCapFor(*e, size_t(nPrim) * 4);
   auto setup = [&](Params& p) {
     p.pointLight = light;
     p.prim = static_cast<const OutPoly*>(e->prim.ptr);
@@ -3386,7 +3394,7 @@ int engineImages(Engine* e, const HostCam& cam, bool cull, int tile, const F3* l
     return BTG_ERR_ARG_CODE;
   }
   const int tilesW = (cam.W + tile - 1) / tile, tilesH = (cam.H + tile - 1) / tile, R = tilesW * tilesH;
-  const size_t outCap = std::max<size_t>(size_t(1) << 16, size_t(cam.W) * size_t(cam.H) * 2);
+  const size_t outCap = outCapFor(*e, size_t(cam.W) * size_t(cam.H) * 2);
   auto setup = [&](Params& p) {
     p.tile = tile;
     p.tilesW = tilesW;
@@ -3422,7 +3430,7 @@ int engineImages(Engine* e, const HostCam& cam, bool cull, int tile, const F3* l
     std::memset(mask, 0, np);
     return 0;
   }
-  const size_t pcap = std::max<size_t>(size_t(1) << 16, size_t(nPrim) * 4);
+  const size_t pcap = outCapFor(*e, size_t(nPrim) * 4);
   auto psetup = [&](Params& p) {
     p.pointLight = *light;
     p.prim = static_cast<const OutPoly*>(e->prim.ptr);
