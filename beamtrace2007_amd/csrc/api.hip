@@ -370,13 +370,20 @@ int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact,
   const btg::Options start = opt;
   // Coordinate descent over the schedule knobs, one knob at a time, each
   // value timed on the sample (GPU time of the beam kernels and primary
-  // rays); the best value of a knob is kept before the next knob is tuned.
+  // rays); a value replaces the incumbent only when it is more than 2%
+  // faster, since a scaled-down sample cannot resolve closer calls, and the
+  // best value of a knob is kept before the next knob is tuned. No leaf cap
+  // (0) is not tried: uncapped stragglers made every measured batch slower
+  // (3x on the 64-job batch). The per-root budget is not tuned: it decides
+  // which roots are abandoned and traced again, and how many roots the queue
+  // capacity admits at once, so a scaled-down sample (fewer roots, retries
+  // weighing more) misjudges it: 2048 won on the quarter-side sample of the
+  // 64-job batch and lost by 7% on the batch itself.
   struct Knob {
     int btg::Options::*field;
     std::vector<int> values;
   };
-  const Knob knobs[] = {{&btg::Options::leafCap, {2, 4, 8, 16, 32, 0}},
-                        {&btg::Options::budget, {256, 1024, 4096}},
+  const Knob knobs[] = {{&btg::Options::leafCap, {4, 16, 2, 32, 8}},
                         {&btg::Options::roundsPerCheck, {4, 8, 16, 32}}};
   std::vector<btg::SoftJobH> h(static_cast<size_t>(numJobs));
   for (int j = 0; j < numJobs; ++j) {
@@ -415,7 +422,7 @@ int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact,
       double sec = 0;
       if ((rc = timeOf(o, sec))) break;
       record(o, sec);
-      if (sec < bestSec) {
+      if (sec < bestSec * 0.98) {
         bestSec = sec;
         bestVal = v;
       }
@@ -601,7 +608,7 @@ struct SrvReq {
 struct btg_server {
   btg_ctx* ctx = nullptr;
   int rounds = 8;
-  int64_t rootCap = 0, pixCap = 0;
+  int64_t rootCap = 0, pixCap = 0, queueCap = 0;
   int jobsPerPass = 0;
   btg::OpenPass* pass[2] = {nullptr, nullptr};
   std::unordered_map<int, SrvReq*> slots[2];  // pass slot -> request
@@ -727,7 +734,8 @@ void softLoop(btg_server* s) {
     // a failed pass is rebuilt before it takes requests again
     for (int g = 0; g < 2; ++g)
       if (!s->pass[g]) {
-        btg::OpenPass* op = btg::openCreate(s->ctx->engine, s->rootCap, s->pixCap, s->jobsPerPass, true, err);
+        btg::OpenPass* op =
+            btg::openCreate(s->ctx->engine, s->rootCap, s->pixCap, s->jobsPerPass, true, s->queueCap, err);
         std::lock_guard<std::mutex> lk(s->m);
         s->pass[g] = op;
         if (!op) {  // no pass left to run on: fail what waits
@@ -805,9 +813,13 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
   const int jobs = c.jobsPerPass > 0 ? c.jobsPerPass : 512;
   const int rounds = c.roundsPerStep > 0 ? c.roundsPerStep : 8;
   const btg::Options& opt = btg::engineOptions(ctx->engine);
-  // per pass: three sub-beam queues (64 B records) and the shade queue (40 B),
-  // then per pixel 40 B of images and per root ~164 B of state
-  const int64_t queues = int64_t(opt.capacity) * (3 * 64 + 40);
+  // per pass: three sub-beam queues (64 B records) and the shade queue (40 B)
+  // in at most a quarter of hbmBytes (the largest power-of-two capacity that
+  // fits, and no more than the context's), then per pixel 40 B of images and
+  // per root ~164 B of state
+  int64_t queueCap = int64_t(1) << 16;
+  while (queueCap * 2 <= opt.capacity && queueCap * 2 * (3 * 64 + 40) <= hbm / 4) queueCap *= 2;
+  const int64_t queues = queueCap * (3 * 64 + 40);
   const int64_t perPass = hbm / 2 - queues;
   if (perPass < (int64_t(1) << 26))
     return failWith(ctx, BTG_ERR_ARG, "hbmBytes too small for two passes at the configured queue capacity");
@@ -819,10 +831,11 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
     s->rounds = rounds;
     s->rootCap = rootCap;
     s->pixCap = pixCap;
+    s->queueCap = queueCap;
     s->jobsPerPass = jobs;
     std::string err;
     for (int g = 0; g < 2 && err.empty(); ++g) {
-      s->pass[g] = btg::openCreate(ctx->engine, rootCap, pixCap, jobs, true, err);
+      s->pass[g] = btg::openCreate(ctx->engine, rootCap, pixCap, jobs, true, queueCap, err);
       if (!s->pass[g] && err.empty()) err = "open pass creation failed";
     }
     if (err.empty()) s->pointEngine = btg::engineShare(ctx->engine, err);
@@ -836,6 +849,8 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
     }
     btg::engineOptions(s->pointEngine) = opt;
     btg::engineOptions(s->fallback) = opt;
+    btg::engineOptions(s->pointEngine).capacity = std::min<int64_t>(opt.capacity, queueCap);
+    btg::engineOptions(s->fallback).capacity = std::min<int64_t>(opt.capacity, queueCap);
     s->softThread = std::thread(softLoop, s);
     s->pointThread = std::thread(pointLoop, s);
     *out = s;

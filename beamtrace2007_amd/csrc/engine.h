@@ -61,7 +61,7 @@ struct RunInfo {
 struct Options {
   int budget = 1024;
   int primaryBudget = 16384;
-  int64_t capacity = int64_t(1) << 25;
+  int64_t capacity = int64_t(1) << 25;  // set by device memory at creation (engineCreate)
   int roundsPerCheck = 8;
   int leafCap = 8;
   int precision = 0;  // soft shadows: 0 float build (bit-exact), 1 FP64-guarded
@@ -141,7 +141,9 @@ Engine* engineShare(Engine* base, std::string& err);
 // to which jobs are appended while it runs. Capacities: roots, pixels and
 // jobs the pass holds until it is reset (after all its jobs finished).
 struct OpenPass;
-OpenPass* openCreate(Engine* base, int64_t rootCap, int64_t pixCap, int jobCap, bool exact, std::string& err);
+// queueCap > 0 caps the pass's sub-beam queue capacity below the base's.
+OpenPass* openCreate(Engine* base, int64_t rootCap, int64_t pixCap, int jobCap, bool exact, int64_t queueCap,
+                     std::string& err);
 void openDestroy(OpenPass* op);
 // Appends a job (launches its primary rays); 1 = added as *slot, 0 = no room
 // left in this pass, negative = error.

@@ -60,9 +60,9 @@ METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request 
 # scaled by a 256 MiB copy: profiles/trace_plant512_hbm_traffic.txt and
 # profiles/trace_batch64_hbm_traffic.txt), valid for the schedule they were
 # measured with; other runs report null.
-TRAFFIC = {("plant", 512): 198479765}
-# batch64: keyed by the schedule (budget, rounds per check, leaf cap)
-TRAFFIC_BATCH = {(1024, 8, 4): 652811615}
+TRAFFIC = {("plant", 512): 475259666}
+# batch64: keyed by the schedule (queue capacity, budget, rounds per check, leaf cap)
+TRAFFIC_BATCH = {(1 << 27, 1024, 8, 8): 3024328906}
 
 
 def parse():
@@ -288,6 +288,7 @@ def bench_batch(args) -> int:
         infos.append(r.soft_batch(jobs, exact=True, download=False)[1])
         print(f"bench.py: batched step {k + 1}: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
+    capacity = r.options["capacity"]
     precision = None if args.no_cpu else batch_precision(r, scenes, jobs, px)
     r.close()
     st = infos[-1]["stats"]
@@ -297,7 +298,7 @@ def bench_batch(args) -> int:
     roofline = None
     if trace_s > 0:
         achieved = b_trace * args.steps / trace_s / 1e9
-        traffic = TRAFFIC_BATCH.get((tuned["budget"], tuned["rounds_per_check"], tuned["leaf_cap"]))
+        traffic = TRAFFIC_BATCH.get((capacity, tuned["budget"], tuned["rounds_per_check"], tuned["leaf_cap"]))
         roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": "traceKernel",
                     "launches_per_step": trace_launches / args.steps,
@@ -312,6 +313,7 @@ def bench_batch(args) -> int:
                    "jobs": len(jobs), "pixels_per_step": px, "tris": {n: scenes[n].num_tris for n in scenes},
                    "draw_seed": 64, "scene_seed": 2007},
         "device": dev["name"], "arch": dev["arch"],
+        "schedule": dict(tuned, capacity=capacity),
         "plain": None if plain_s is None else {"value": round(px / plain_s, 1), "unit": "pixels/s",
                                                "seconds": round(plain_s, 3), "form": "one softShadows call per job"},
         "batched_over_plain": None if plain_s is None else round(plain_s / (wall / args.steps), 3),

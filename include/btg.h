@@ -187,9 +187,11 @@ int btg_soft_batch(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exac
 /* Schedule autotuner (no counterpart in the reference, whose knobs are fixed,
  * cuda/gpu_render.h:56-58 and cuda/gpu_common.cuh:54-58): times the sample
  * batch `jobs` (typically the real batch at reduced resolution) under the
- * current options, then tunes one knob at a time - leaf cap, per-root
- * sub-beam budget, rounds between host checks - keeping each knob's fastest
- * value, within about maxSeconds. Writes the best options to `best` (the
+ * current options, then tunes one knob at a time - leaf cap, rounds between
+ * host checks - keeping a value when it is more than 2% faster than the
+ * incumbent, within about maxSeconds. The per-root budget and the queue
+ * capacity are the caller's (they set how many roots run at once, which a
+ * scaled-down sample cannot measure). Writes the best options to `best` (the
  * context's options are left unchanged: apply them with btg_set_options)
  * and, if `trials` is given, up to maxTrials timed settings. Results do not
  * depend on these knobs, only speed does. */
