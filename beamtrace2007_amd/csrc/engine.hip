@@ -84,6 +84,7 @@ struct RootQ {
   float lightArea;
   F3 x, ns;     // soft: shading point and normal
   F3 Olo;       // guarded soft: low part of the apex (plane.O + Olo)
+  F3 xlo;       // guarded soft: low part of the shading point (x + xlo, the double hit point)
 };
 
 struct WState {
@@ -433,7 +434,11 @@ __device__ __forceinline__ void hitPlane(const Params& p, const Ctx& c, int t, W
   w.hitC = dot(nT, relApex(c, f3(vv.x, vv.y, vv.z)));
 }
 
-// wfExpand (wavefront_core.h:92-112).
+// wfExpand (wavefront_core.h:92-112). Any-hit applications (soft and point
+// shadows) never queue a beam with a hit (occluded pieces end where they are
+// found), so their records always carry hit = -1 and the far plane is the
+// query's; fixing that at compile time keeps the hit plane out of registers.
+template <int App>
 __device__ void expand(const Params& p, const Ctx& c, const Rec& r, Work& w) {
   Beam& b = w.b;
   const int n = r.n;
@@ -446,9 +451,10 @@ __device__ void expand(const Params& p, const Ctx& c, const Rec& r, Work& w) {
   b.s1 = (r.negBits & 2) ? -1 : 1;
   b.s2 = (r.negBits & 4) ? -1 : 1;
   setAw(b, c);
-  if (r.hit >= 0) {
-    hitPlane(p, c, r.hit, w);
-    setStatus(c, b, r.hit, w.hitA, w.hitC);
+  const int hit = App == kPrimary ? r.hit : -1;
+  if (hit >= 0) {
+    hitPlane(p, c, hit, w);
+    setStatus(c, b, hit, w.hitA, w.hitC);
   } else {
     w.hitA[0] = w.hitA[1] = w.hitA[2] = w.hitC = 0;
     w.hitN = f3(0, 0, 0);
@@ -839,6 +845,36 @@ __device__ float formFactor(F3 x, F3 n, const F3 (&poly)[4], int count) {
   return float((sum < 0 ? -sum : sum) * 0.5);
 }
 
+// polygonFormFactorG as the double build computes it (Real = double): the
+// guarded mode's Lambert integral, every step in double from the double
+// shading point and the piece's corners formed in double on the light plane.
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 d3sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ double d3dot(D3 a, D3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ D3 d3cross(D3 a, D3 b) {
+  return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ double formFactorD(D3 x, D3 n, const D3 (&poly)[4], int count) {
+  double sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= count) break;
+    const D3 a = d3sub(poly[i], x), b = d3sub(i + 1 < count ? poly[(i + 1) & 3] : poly[0], x);
+    const double la = sqrt(d3dot(a, a)), lb = sqrt(d3dot(b, b));
+    if (la == 0 || lb == 0) continue;
+    double cc = d3dot(a, b) / (la * lb);
+    cc = cc < -1 ? -1 : (cc > 1 ? 1 : cc);
+    const double theta = acos(cc);
+    const D3 g = d3cross(a, b);
+    const double lg = sqrt(d3dot(g, g));
+    if (lg == 0) continue;
+    sum += theta * d3dot(g, n) / lg;
+  }
+  return (sum < 0 ? -sum : sum) * 0.5;
+}
+
 // Signed area of a beam's 3- or 4-gon, as PolyN::area (beam_geom.h:27-33)
 // sums it. A triangle's fourth corner repeats its third; the extra term
 // x2*y2 - x2*y2 is exactly +0 and leaves the sum unchanged.
@@ -1209,7 +1245,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
             liveSub(p, r.root, 1);
           } else {
             c = modeCtx<App>(p.roots[r.root]);
-            expand(p, c, r, w);
+            expand<App>(p, c, r, w);
             if (r.cursor < 0) {
               ss.phase = kDescend;
               ss.box = p.sv.nodeBox[r.node];
@@ -1367,7 +1403,7 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
         liveSub(p, r.root, 1);
       } else {
         c = modeCtx<App>(p.roots[r.root]);
-        expand(p, c, r, w);
+        expand<App>(p, c, r, w);
         have = true;
       }
     }
@@ -1758,7 +1794,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
   R.unitExp = 0;
   R.tri = -1;
   R.lightArea = 0;
-  R.x = R.ns = R.Olo = f3(0, 0, 0);
+  R.x = R.ns = R.Olo = R.xlo = f3(0, 0, 0);
   Pg<8> poly;
   poly.n = 0;
   uint32_t node0 = p.node0;
@@ -1793,6 +1829,7 @@ __device__ void admitRoot(const Params& p, int k, int cur, Cnt& cnt) {
       return;
     }
     R.x = x;
+    R.xlo = xlo;
     R.ns = ns;
   } else if (App == kPrimary) {
     const int tx = k % p.tilesW, ty = k / p.tilesW;
@@ -1863,11 +1900,26 @@ __global__ void __launch_bounds__(kBlock) shadeKernel(Params p) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const ShadeRec sr = p.shq[i];
     const RootQ& R = p.roots[sr.root];
-    F3 P[4];
+    double G;
+    if (p.pixXlo) {  // guarded: in double throughout
+      const Plane& pl = R.plane;
+      D3 P[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) P[k] = R.plane.point(sr.qx[k], sr.qy[k]);
-    const float G = formFactor(R.x, R.ns, P, sr.n);
-    atomicAdd(&p.gFx[sr.root], (unsigned long long)rint(double(G) * pow2(60)));
+      for (int k = 0; k < 4; ++k) {
+        const double qx = sr.qx[k], qy = sr.qy[k];
+        P[k] = D3{(double(pl.p0.x) + double(pl.u.x) * qx) + double(pl.v.x) * qy,
+                  (double(pl.p0.y) + double(pl.u.y) * qx) + double(pl.v.y) * qy,
+                  (double(pl.p0.z) + double(pl.u.z) * qx) + double(pl.v.z) * qy};
+      }
+      const D3 xd{double(R.x.x) + double(R.xlo.x), double(R.x.y) + double(R.xlo.y), double(R.x.z) + double(R.xlo.z)};
+      G = formFactorD(xd, D3{R.ns.x, R.ns.y, R.ns.z}, P, sr.n);
+    } else {
+      F3 P[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) P[k] = R.plane.point(sr.qx[k], sr.qy[k]);
+      G = double(formFactor(R.x, R.ns, P, sr.n));
+    }
+    atomicAdd(&p.gFx[sr.root], (unsigned long long)rint(G * pow2(60)));
   }
 }
 

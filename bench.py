@@ -361,6 +361,7 @@ def batch_precision(r, scenes, jobs, px) -> dict | None:
         res[mode] = r.soft_batch(sample, exact=True, download=True)[0]
     r.set_options(**saved)
     errs = {0: [], 1: []}
+    eerrs = {0: [], 1: []}
     threads = max(1, min(16, os.cpu_count() or 1))
     refs = {}
     for j, o0, o1 in zip(sample, res[0], res[1]):
@@ -371,11 +372,13 @@ def batch_precision(r, scenes, jobs, px) -> dict | None:
         y = j["H"] // 2
         rr = refs[name].soft_rows(scenes[name].spec()["view"], j["W"], j["H"], j["light"], y, y + 1, exact=True,
                                   mailbox=False, trail=True, threads=threads)
-        d, dtri = rr["vis"][0], rr["tri"][0]
+        d, dtri, dE = rr["vis"][0], rr["tri"][0], rr["E"][0].astype(np.float64)
         for mode, o in ((0, o0), (1, o1)):
             g = o["vis"][y].astype(np.float64)
             m = (d >= 0) & (g >= 0) & (o["tri"][y] == dtri)  # same primary hit: the shadow computation compared
             errs[mode].append(np.abs(g - d)[m])
+            if m.any():  # irradiance, relative to the row's largest exact E
+                eerrs[mode].append(np.abs(o["E"][y].astype(np.float64) - dE)[m] / max(np.abs(dE[m]).max(), 1e-30))
     for rf in refs.values():
         rf.free()
 
@@ -387,6 +390,8 @@ def batch_precision(r, scenes, jobs, px) -> dict | None:
     out["vis_err_vs_fp64"] = {"float_build": summary(errs[0]), "guarded": summary(errs[1]),
                               "sample": "middle rows of the first two jobs of each scene, pixels with the same primary hit, against bt_render_d's "
                                         "algorithm (the reference compiled with Real = double)"}
+    out["E_err_vs_fp64"] = {"float_build": summary(eerrs[0]), "guarded": summary(eerrs[1]),
+                            "unit": "|E - E_fp64| / the row's largest E_fp64", "sample": "the same pixels"}
     return out
 
 

@@ -88,7 +88,7 @@ typedef struct {
   int precision;      /* soft shadows: 0 = the reference's float build, bit for bit (default); 1 = FP64-guarded:
                          the receiver point is the primary ray's hit computed in double, the beam apex sits at
                          the double build's offset (1e-9 x scene extent) from it, formed in double and carried as
-                         a float pair in every apex-relative difference; the Lambert integral sums in double.
+                         a float pair in every apex-relative difference; the Lambert integral runs in double throughout.
                          Results approach bt_render_d's (mean visibility error 100-1000x below the float build's
                          on the test scenes; bench.py reports it). Same throughput as 0. */
   int useGraphs;      /* 1 (default): each group of roundsPerCheck rounds is captured once per pass into a HIP graph

@@ -1101,6 +1101,29 @@ float formFactor(V3 x, V3 n, const V3* poly, int count) {
   return float((sum < 0 ? -sum : sum) * 0.5);
 }
 
+// polygonFormFactorG (soft_core.h:18-33) in double throughout: the guarded
+// mode's Lambert integral (x: the double shading point; corners formed in
+// double on the light plane from the float piece corners).
+double formFactorD(const double* x, V3 n, const double (*P)[3], int count) {
+  double sum = 0;
+  for (int i = 0; i < count; ++i) {
+    const double* pa = P[i];
+    const double* pb = P[i + 1 == count ? 0 : i + 1];
+    const double a[3] = {pa[0] - x[0], pa[1] - x[1], pa[2] - x[2]}, b[3] = {pb[0] - x[0], pb[1] - x[1], pb[2] - x[2]};
+    const double la = std::sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]);
+    const double lb = std::sqrt((b[0] * b[0] + b[1] * b[1]) + b[2] * b[2]);
+    if (la == 0 || lb == 0) continue;
+    double c = ((a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]) / (la * lb);
+    c = c < -1 ? -1 : (c > 1 ? 1 : c);
+    const double theta = std::acos(c);
+    const double g[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double lg = std::sqrt((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+    if (lg == 0) continue;
+    sum += theta * ((g[0] * double(n.x) + g[1] * double(n.y)) + g[2] * double(n.z)) / lg;
+  }
+  return (sum < 0 ? -sum : sum) * 0.5;
+}
+
 // lightBeamSetup (soft_core.h:38-73).
 // guard: the engine's FP64-guarded mode (not in the reference): x + xlo is the
 // hit point in double, the apex x + ns * offset is formed in double and kept as
@@ -1175,7 +1198,23 @@ float lightVis(Tracer& tr, const Light& L, V3 x, V3 xlo, bool guard, V3 ns, int 
   tr.keep = G != nullptr;
   tr.trace(q, root);
   const float V = float(tr.missArea) / lightArea;
-  if (G) {
+  if (G && guard) {
+    const double xd[3] = {double(x.x) + double(xlo.x), double(x.y) + double(xlo.y), double(x.z) + double(xlo.z)};
+    const Plane& pl = q.plane;
+    double g = 0;
+    for (const Out& ob : tr.out) {
+      if (ob.tri >= 0) continue;
+      double P[4][3];
+      for (int i = 0; i < ob.n; ++i) {
+        const double qx = ob.x[i], qy = ob.y[i];
+        P[i][0] = (double(pl.p0.x) + double(pl.u.x) * qx) + double(pl.v.x) * qy;
+        P[i][1] = (double(pl.p0.y) + double(pl.u.y) * qx) + double(pl.v.y) * qy;
+        P[i][2] = (double(pl.p0.z) + double(pl.u.z) * qx) + double(pl.v.z) * qy;
+      }
+      g += formFactorD(xd, ns, P, ob.n);
+    }
+    *G = float(g);
+  } else if (G) {
     float g = 0;
     for (const Out& ob : tr.out) {
       if (ob.tri >= 0) continue;

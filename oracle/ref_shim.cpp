@@ -192,7 +192,7 @@ void ref_soft(void* sh, void* kh, const double* cam, int W, int H, const double*
 // sample covers every region of the frame). Same calls, same threading.
 void ref_soft_rows(void* sh, void* kh, const double* cam, int W, int H, const double* light, int exact, int mailbox,
                    int trail, int threads, int y0, int y1, int stride, float* vis, double* stats, double* seconds,
-                   int32_t* triOut) {
+                   int32_t* triOut, float* Gout) {
   const Scene& s = *static_cast<Scene*>(sh);
   const KdTree& tree = *static_cast<KdTree*>(kh);
   Camera c = makeCam(cam, W, H);
@@ -246,9 +246,11 @@ void ref_soft_rows(void* sh, void* kh, const double* cam, int W, int H, const do
       for (int x = 0; x < W; ++x) {
         size_t k = size_t(r) * size_t(W) + size_t(x);
         vis[k] = -1.f;
+        if (Gout) Gout[k] = 0.f;
         if (triv[k] < 0) continue;
         Real Gx = 0;
         vis[k] = float(beamLightVisibility(btr, X[k], N[k], triv[k], L, offset, exact ? &Gx : nullptr));
+        if (Gout) Gout[k] = float(Gx);  // the exact irradiance E (render.cpp:298-302)
       }
     st[size_t(id)].add(btr.stats);
   };

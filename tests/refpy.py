@@ -64,7 +64,7 @@ class Ref:
             L.ref_soft.argtypes = [vp, vp, dp, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int, i32p, fp, fp,
                                    fp, dp, dp]
             L.ref_soft_rows.argtypes = [vp, vp, dp, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                        C.c_int, C.c_int, fp, dp, dp, i32p]
+                                        C.c_int, C.c_int, fp, dp, dp, i32p, fp]
             L.ref_light_vis.restype = C.c_double
             L.ref_light_vis.argtypes = [vp, vp, dp, dp, C.c_int, dp, C.c_double, C.c_int, C.c_int, dp, dp]
             L.ref_primary.restype = vp
@@ -148,10 +148,14 @@ class Ref:
         rows = (y1 - y0 + stride - 1) // stride
         vis, st, sec = np.zeros(rows * W, np.float32), np.zeros(14), np.zeros(2)
         tri = np.zeros(rows * W, np.int32)
+        E = np.zeros(rows * W, np.float32)
         self.L.ref_soft_rows(self.scene, self.kd, _d(view10), W, H, _d(light9), int(exact), int(mailbox), int(trail),
                              threads, y0, y1, stride, vis.ctypes.data_as(fp), st.ctypes.data_as(dp),
-                             sec.ctypes.data_as(dp), tri.ctypes.data_as(i32p))
-        return {"vis": vis.reshape(rows, W), "tri": tri.reshape(rows, W), "stats": stats_dict(st), "seconds": sec}
+                             sec.ctypes.data_as(dp), tri.ctypes.data_as(i32p), E.ctypes.data_as(fp))
+        out = {"vis": vis.reshape(rows, W), "tri": tri.reshape(rows, W), "stats": stats_dict(st), "seconds": sec}
+        if exact:
+            out["E"] = E.reshape(rows, W)
+        return out
 
     def light_vis(self, x, ns, tri, light9, offset, mailbox=False, trail=True):
         G, st = np.zeros(1), np.zeros(14)

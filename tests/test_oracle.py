@@ -171,7 +171,8 @@ def test_oracle_exact_and_eq2_agree_when_unoccluded():
     ("atrium", 40, 1e-4, 0.01, 100), ("conference", 40, 5e-5, 0.01, 100), ("blob", 24, 5e-3, 0.03, 3)])
 def test_guarded_soft_shadows_approach_the_double_build(name, side, max_err, max_frac, gain):
     """The FP64-guarded mode (double hit point, the double build's apex offset,
-    the apex as a float pair) against the reference's double build: the mean
+    the apex as a float pair, the Lambert integral in double) against the
+    reference's double build: the mean
     visibility error falls by `gain` or more from the float build's (up to
     ~4e-2 on the plant, half of its pixels above 1e-5), the largest error and
     the share above 1e-5 stay within the bounds here. A float apex at the
@@ -187,12 +188,18 @@ def test_guarded_soft_shadows_approach_the_double_build(name, side, max_err, max
     d = refd.soft(sp["view"], W, H, sp["area_light"], exact=True, mailbox=False, trail=True, threads=8)
     refd.free()
     o = Oracle(a, k)
-    err = {}
+    err, eerr = {}, {}
     for guard in (False, True):
         g = o.soft(cam, W, H, sp["area_light"], exact=True, guard=guard)
         assert np.array_equal(g["tri"], d["tri"])
         m = (d["vis"] >= 0) & (g["vis"] >= 0)
         err[guard] = np.abs(g["vis"].astype(np.float64) - d["vis"])[m]
+        eerr[guard] = np.abs(g["E"].astype(np.float64) - d["E"])[m] / np.abs(d["E"][m]).max()
     assert err[True].max() <= max_err
     assert np.mean(err[True] > 1e-5) <= max_frac
     assert err[True].mean() * gain <= err[False].mean()
+    # the Lambert integral in double: irradiance errors (relative to the
+    # image's largest E) fall by 30x or more (the float images' own rounding,
+    # ~6e-8, is the floor)
+    assert eerr[True].mean() * 30 <= eerr[False].mean()
+    assert eerr[True].max() <= 20 * max_err
