@@ -100,6 +100,72 @@ def test_point_shadows_and_masks_match_oracle(require_gpu, name):
                           rasterize(cam, W, H, os_, shadow=True))
 
 
+LARGE = ["atrium", "conference", "blob", "building_small"]
+
+
+@pytest.mark.parametrize("name", LARGE)
+def test_soft_shadows_on_benchmark_scenes_match_oracle(require_gpu, name):
+    """The batched workload's scenes (blob, atrium), the served one
+    (conference hall) and a smaller building, in the default float mode."""
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene(name)
+    W, H = 64, 48
+    cam = camera(sp["view"], W, H)
+    print(f"soft {name} {W}x{H}", flush=True)
+    g = renderer(name).soft_shadows(cam, W, H, sp["area_light"], exact=True)
+    o = oracle(name).soft(cam, W, H, sp["area_light"], exact=True)
+    check_soft(g, o)
+    if g["info"]["killed_roots"] == 0:
+        for key in ("kdSteps", "leafVisits", "triTests", "hits", "splits", "beams", "climbs"):
+            assert g["info"]["stats"][key] == o["stats"][key], key
+
+
+@pytest.mark.parametrize("name", ["atrium", "conference", "building_small"])
+def test_primary_and_point_shadows_on_benchmark_scenes_match_oracle(require_gpu, name):
+    from beamtrace2007_amd.gpu import polys_to_dict
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene(name)
+    W, H = 96, 72
+    cam = camera(sp["view"], W, H)
+    print(f"primary + point {name}", flush=True)
+    gp, _ = renderer(name).primary(cam, W, H, tile=16)
+    op = oracle(name).primary(cam, W, H, tile=16)
+    assert np.array_equal(canon(polys_to_dict(gp)), canon(op))
+    gs, _ = renderer(name).point_shadows(cam, W, H, gp, sp["point_light"])
+    os_ = oracle(name).point(cam, W, H, op, sp["point_light"])
+    assert np.array_equal(canon(polys_to_dict(gs)), canon(os_))
+
+
+def test_plant_at_full_size_matches_oracle_on_row_bands(require_gpu):
+    """The soft-shadow configuration at its full size (512 x 512): the GPU's
+    whole image against the oracle on five bands of eight rows."""
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene("plant")
+    W = H = 512
+    cam = camera(sp["view"], W, H)
+    g = renderer("plant").soft_shadows(cam, W, H, sp["area_light"], exact=True)
+    for y0 in (0, 128, 256, 384, 504):
+        print(f"plant 512 rows {y0}..{y0 + 8}", flush=True)
+        o = oracle("plant").soft(cam, W, H, sp["area_light"], exact=True, threads=16, rows=(y0, y0 + 8))
+        band = slice(y0, y0 + 8)
+        check_soft({f: g[f][band] for f in ("tri", "vis", "E", "Eu")}, {f: o[f][band] for f in ("tri", "vis", "E", "Eu")})
+
+
+def test_room_at_full_size_primary_matches_oracle(require_gpu):
+    """The primary-visibility configuration at its full size: 512 x 512 in
+    8 x 8 tiles (4096 roots), polygons and per-pixel ids."""
+    from beamtrace2007_amd.gpu import polys_to_dict
+    from beamtrace2007_amd.host import camera, rasterize
+    s, a, k, sp = load_scene("room")
+    W = H = 512
+    cam = camera(sp["view"], W, H)
+    gp, info = renderer("room").primary(cam, W, H, tile=8)
+    op = oracle("room").primary(cam, W, H, tile=8)
+    assert np.array_equal(canon(polys_to_dict(gp)), canon(op))
+    assert np.array_equal(rasterize(cam, W, H, polys_to_dict(gp)), rasterize(cam, W, H, op))
+    assert info["stats"]["triTests"] == op["stats"]["triTests"]
+
+
 def test_odd_image_sizes_and_partial_tiles(require_gpu):
     from beamtrace2007_amd.gpu import polys_to_dict
     from beamtrace2007_amd.host import camera
