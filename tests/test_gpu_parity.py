@@ -100,6 +100,22 @@ def test_point_shadows_and_masks_match_oracle(require_gpu, name):
                           rasterize(cam, W, H, os_, shadow=True))
 
 
+@pytest.mark.parametrize("name", ["room", "sphere", "random_tris"])
+def test_primary_with_back_face_culling_matches_oracle(require_gpu, name):
+    """primary(cull = true) (cuda/gpu_render.h:41): back faces never hit."""
+    from beamtrace2007_amd.gpu import polys_to_dict
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene(name)
+    W, H = 96, 72
+    cam = camera(sp["view"], W, H)
+    gp, _ = renderer(name).primary(cam, W, H, cull=True, tile=8)
+    op = oracle(name).primary(cam, W, H, cull=True, tile=8)
+    assert np.array_equal(canon(polys_to_dict(gp)), canon(op))
+    go, _ = renderer(name).primary(cam, W, H, cull=False, tile=8)
+    if name == "random_tris":  # open, randomly oriented triangles: culling changes what is seen
+        assert not np.array_equal(canon(polys_to_dict(gp)), canon(polys_to_dict(go)))
+
+
 LARGE = ["atrium", "conference", "blob", "building_small"]
 
 

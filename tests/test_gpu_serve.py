@@ -159,3 +159,27 @@ def test_guarded_requests_equal_guarded_alone(require_gpu):
             assert np.array_equal(res[f], a[f]), (W, H, f)
     srv.close()
     r.close()
+
+
+def test_bad_requests_fail_alone(require_gpu, server_setup):
+    """A request for a scene that is not resident, or one larger than a whole
+    pass, finishes with an error status and a message; requests around it
+    are served normally. An invalid camera is refused at submission."""
+    from beamtrace2007_amd.host import camera
+    srv, ids = server_setup
+    sp = load_scene("room")[3]
+    good_cam = camera(sp["view"], 32, 24)
+    t0 = srv.submit_soft(ids["room"], good_cam, 32, 24, sp["area_light"])
+    bad = srv.submit_soft(99, good_cam, 32, 24, sp["area_light"])
+    huge = srv.submit_soft(ids["room"], camera(sp["view"], 8192, 8192), 8192, 8192, sp["area_light"])
+    t1 = srv.submit_soft(ids["room"], good_cam, 32, 24, sp["area_light"])
+    for t in (bad, huge):
+        res = srv.wait(t, timeout=120)
+        assert res is not None and res["status"] != 0
+    for t in (t0, t1):
+        res = srv.wait(t, timeout=120)
+        assert res is not None and res["status"] == 0
+        a = soft_alone("room", good_cam, 32, 24, sp["area_light"])
+        assert np.array_equal(res["vis"], a["vis"])
+    with pytest.raises(RuntimeError):
+        srv.submit_soft(ids["room"], camera(sp["view"], 32, 24), 0, 24, sp["area_light"])
