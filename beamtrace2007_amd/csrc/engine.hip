@@ -1214,13 +1214,16 @@ __device__ __forceinline__ bool moreInLeaf(const Stepper& ss, const Rec& r) {
   return uint32_t(r.cursor) < ss.leafCount;
 }
 
-// Soft shadows test leaf triangles wave-cooperatively (coopTest); build with
-// -DBTG_COOP_TEST=0 for each lane's own sequential loop (testOne). Results and
-// counters are the same either way (tested); batch64 runs 1.7% faster.
+// Soft shadows and primary visibility test leaf triangles wave-cooperatively
+// (coopTest); build with -DBTG_COOP_TEST=0 for each lane's own sequential loop
+// (testOne). Results and counters are the same either way (tested); batch64
+// runs 1.7% faster. Point shadows keep the sequential loop (the pooled form
+// spills there).
 #ifndef BTG_COOP_TEST
 #define BTG_COOP_TEST 1
 #endif
 constexpr bool kCoopTest = BTG_COOP_TEST != 0;
+__host__ __device__ constexpr bool coopApp(int App) { return isSoft(App) || App == kPrimary; }
 
 __device__ __forceinline__ F3 shflF3(F3 v, int src) {
   return f3(__shfl(v.x, src, kWave), __shfl(v.y, src, kWave), __shfl(v.z, src, kWave));
@@ -1232,15 +1235,18 @@ __device__ __forceinline__ C4 shflC4(const C4& v, int src) {
   return r;
 }
 
-// The leaf tests of the whole wave, dealt 64 at a time (soft shadows).
-// Lanes with `want` own a leaf with triangles pending from r.cursor; every
-// (owner, triangle) pair becomes one lane's test, run on the owner's beam and
-// query fetched by cross-lane reads. For each owner the first triangle in
-// leaf order that is not Keep decides, and only the tests up to it are
-// counted, exactly as the owner's own sequential loop would. All 64 lanes
-// call it.
+// The leaf tests of the whole wave, dealt 64 at a time. Lanes with `want` own
+// a leaf with triangles pending from r.cursor; every (owner, triangle) pair
+// becomes one lane's test, run on the owner's beam and query fetched by
+// cross-lane reads. For each owner the first triangle in leaf order that is
+// not Keep decides, and only the tests up to it are counted, exactly as the
+// owner's own sequential loop would. All 64 lanes call it.
+// Primary visibility keeps the nearest hit: an AllIn there narrows the beam
+// to the triangle's plane (setStatus) and the owner returns kContinue with
+// the cursor past it, so the rest of the leaf is tested against the new hit,
+// again as the sequential loop does.
 template <int App>
-__device__ int coopTest(const Params& p, const Ctx& c, Rec& r, const Work& w, const Stepper& ss, bool want,
+__device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const Stepper& ss, bool want,
                         Cnt& cnt) {
   const int k = want ? int(ss.leafCount) - r.cursor : 0;
   int incl = k;
@@ -1254,6 +1260,8 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, const Work& w, co
   bool resolved = !(want && k > 0);
   int first = -1, firstTri = -1, counted = 0;
   bool firstAllIn = false;
+  float hA0 = 0, hA1 = 0, hA2 = 0, hC = 0;  // the first AllIn's plane (primary)
+  F3 hN = f3(0, 0, 0);
   uint64_t open = __ballot(!resolved);
   for (int base = 0; base < total && open; base += kWave) {
     const int t = base + lane();
@@ -1282,14 +1290,23 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, const Work& w, co
     oc.eps = __shfl(c.eps, owner, kWave);
     oc.minArea = 0;
     oc.farCq = __shfl(c.farCq, owner, kWave);
-    oc.flags = c.flags;
+    oc.flags = App == kPrimary ? __shfl(c.flags, owner, kWave) : c.flags;
     oc.exclude = __shfl(c.exclude, owner, kWave);
     Work ow;
     ow.b.qx = shflC4(w.b.qx, owner);
     ow.b.qy = shflC4(w.b.qy, owner);
     ow.b.n = __shfl(w.b.n, owner, kWave);
     ow.b.hit = -1;
+    if (App == kPrimary) {  // the current hit's plane, for the closer-than-hit line
+      ow.b.hit = __shfl(w.b.hit, owner, kWave);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) ow.hitA[k] = __shfl(w.hitA[k], owner, kWave);
+      ow.hitC = __shfl(w.hitC, owner, kWave);
+      ow.hitN = shflF3(w.hitN, owner);
+    }
     int status = 0, tri = -1;  // 0 skipped (uncounted), 1 Keep, 2 AllIn, 3 Clip
+    float tA0 = 0, tA1 = 0, tA2 = 0, tC = 0;  // an AllIn triangle's plane (primary)
+    F3 tN = f3(0, 0, 0);
     if (live) {
       const uint32_t idx = uint32_t(oCursor + (t - oExcl));
       const LeafTri ref = p.sv.refs[uint32_t(oOff) + idx];
@@ -1298,8 +1315,19 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, const Work& w, co
         TriS ti;
         triBegin(oc, ref, ti);
         if (ti.ok) {
-          const int rr = classifyLazy(oc, ow, ti);
-          status = rr == kKeep ? 1 : (rr == kAllIn ? 2 : 3);
+          if (App == kPrimary && tri == ow.b.hit) {
+            status = 1;  // the current hit itself: counted, kept
+          } else {
+            const int rr = classifyLazy(oc, ow, ti);
+            status = rr == kKeep ? 1 : (rr == kAllIn ? 2 : 3);
+            if (App == kPrimary && rr == kAllIn) {
+              tA0 = ti.A[0];
+              tA1 = ti.A[1];
+              tA2 = ti.A[2];
+              tC = ti.C;
+              tN = ti.n;
+            }
+          }
         }
       }
     }
@@ -1329,12 +1357,36 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, const Work& w, co
     }
     const int gotTri = __shfl(tri, src, kWave);
     if (took) firstTri = gotTri;
+    if (App == kPrimary) {
+      const float a0 = __shfl(tA0, src, kWave), a1 = __shfl(tA1, src, kWave), a2 = __shfl(tA2, src, kWave);
+      const float cc = __shfl(tC, src, kWave);
+      const F3 nn = shflF3(tN, src);
+      if (took) {
+        hA0 = a0;
+        hA1 = a1;
+        hA2 = a2;
+        hC = cc;
+        hN = nn;
+      }
+    }
     open = __ballot(!resolved);
   }
   if (!want) return kContinue;
   cnt.v[sTests] += uint32_t(counted);
   if (first >= 0) {
     if (firstAllIn) {
+      if (App == kPrimary) {  // testOne's nearest-hit update
+        const float A[3] = {hA0, hA1, hA2};
+        setStatus(c, w.b, firstTri, A, hC);
+        w.hitA[0] = hA0;
+        w.hitA[1] = hA1;
+        w.hitA[2] = hA2;
+        w.hitC = hC;
+        w.hitN = hN;
+        r.hit = firstTri;
+        r.cursor = first + 1;
+        return kContinue;
+      }
       emitOutput<App>(p, c, w.b, r.root, firstTri, cnt);
       return kFinished;
     }
@@ -1401,7 +1453,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
     int res = kContinue;
     if (active) res = advance<App>(p, c, r, w, ss, path, cnt PHASE_PASS);
     PHASE_MARK(kPhRest);
-    if (kCoopTest && isSoft(App)) {
+    if (kCoopTest && coopApp(App)) {
       const bool want = active && res == kContinue;  // a leaf with triangles pending
       const int cr = coopTest<App>(p, c, r, w, ss, want, cnt);
       if (want) res = cr;
