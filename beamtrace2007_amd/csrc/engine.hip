@@ -43,6 +43,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
+constexpr int kPrimaryLeafCap = 4;  // leaf cap of primary passes (at most; see prepare)
 
 // kSoftG: soft shadows in the FP64-guarded mode (double hit point, the
 // double build's apex offset, the apex held as a float pair).
@@ -1214,16 +1215,14 @@ __device__ __forceinline__ bool moreInLeaf(const Stepper& ss, const Rec& r) {
   return uint32_t(r.cursor) < ss.leafCount;
 }
 
-// Soft shadows and primary visibility test leaf triangles wave-cooperatively
-// (coopTest); build with -DBTG_COOP_TEST=0 for each lane's own sequential loop
-// (testOne). Results and counters are the same either way (tested); batch64
-// runs 1.7% faster. Point shadows keep the sequential loop (the pooled form
-// spills there).
+// Every application tests leaf triangles wave-cooperatively (coopTest); build
+// with -DBTG_COOP_TEST=0 for each lane's own sequential loop (testOne).
+// Results and counters are the same either way (tested); batch64 runs 1.7%
+// faster, a primary pass 18% and a point-shadow pass 13%.
 #ifndef BTG_COOP_TEST
 #define BTG_COOP_TEST 1
 #endif
 constexpr bool kCoopTest = BTG_COOP_TEST != 0;
-__host__ __device__ constexpr bool coopApp(int App) { return isSoft(App) || App == kPrimary; }
 
 __device__ __forceinline__ F3 shflF3(F3 v, int src) {
   return f3(__shfl(v.x, src, kWave), __shfl(v.y, src, kWave), __shfl(v.z, src, kWave));
@@ -1453,7 +1452,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
     int res = kContinue;
     if (active) res = advance<App>(p, c, r, w, ss, path, cnt PHASE_PASS);
     PHASE_MARK(kPhRest);
-    if (kCoopTest && coopApp(App)) {
+    if (kCoopTest) {
       const bool want = active && res == kContinue;  // a leaf with triangles pending
       const int cr = coopTest<App>(p, c, r, w, ss, want, cnt);
       if (want) res = cr;
@@ -2508,7 +2507,10 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
   p.cycles = static_cast<unsigned long long*>(e.cycles.ptr);
   p.dropped = static_cast<double*>(e.dropped.ptr);
   p.st = static_cast<WState*>(e.st.ptr);
-  p.leafCap = e.opt.leafCap;
+  // Primary passes are chains of short rounds whose trace launches last as
+  // long as their slowest lane: they yield after at most kPrimaryLeafCap
+  // leaves (measured on the conference hall: 45 ms against 50 ms at 8).
+  p.leafCap = app == kPrimary && e.opt.leafCap > 0 ? std::min(e.opt.leafCap, kPrimaryLeafCap) : e.opt.leafCap;
   return 0;
 }
 

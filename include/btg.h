@@ -84,7 +84,8 @@ typedef struct {
   int64_t capacity;
   int roundsPerCheck;
   int profileKernels; /* 1: time every launch with HIP events (btg_run_info.kernelSeconds) */
-  int leafCap;        /* leaves a sub-beam visits per trace launch before it yields to the next round (0: no cap) */
+  int leafCap;        /* leaves a sub-beam visits per trace launch before it yields to the next round (0: no cap);
+                         primary-visibility passes use at most 4 (latency-bound chains of short rounds) */
   int precision;      /* soft shadows: 0 = the reference's float build, bit for bit (default); 1 = FP64-guarded:
                          the receiver point is the primary ray's hit computed in double, the beam apex sits at
                          the double build's offset (1e-9 x scene extent) from it, formed in double and carried as
