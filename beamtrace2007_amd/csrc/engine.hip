@@ -996,17 +996,25 @@ __device__ __forceinline__ void killRoot(const Params& p, int root) {
 // ------------------------------------------------------------- the trace
 // Kd depths whose ancestors a lane keeps in LDS (deeper ones are reached by
 // parent links). The SAH build limits depth to 8 + 1.3 log2(triangles), 36 at
-// 2M triangles.
-constexpr int kPathLevels = 40;
+// 2M triangles, unless a deeper limit is asked for (63 at most). A build with
+// -DBTG_PATH_LEVELS=4 sends most climbs through the parent-link walk (a test
+// build for that path).
+#ifndef BTG_PATH_LEVELS
+#define BTG_PATH_LEVELS 40
+#endif
+constexpr int kPathLevels = BTG_PATH_LEVELS;
 
 struct Stepper {
   int phase;
   Box box;                         // descend: box of r.node
+  KdNodeD kn;                      // descend: node r.node
   uint32_t leafOffset, leafCount;  // the current leaf
-  int validFrom;                   // path levels [validFrom, min(r.depth, kPathLevels)) hold r.node's ancestors
+  int validFrom;                   // path levels [validFrom, min(r.depth, kPathLevels)) are valid
 };
 
-// A lane's column of the LDS ancestor path: level k at col[k * kBlock].
+// A lane's column of the LDS ancestor path: level k at col[k * kBlock] holds
+// the far child (the child the rays move into) of r.node's ancestor at depth
+// k, which is where a climb to level k goes.
 struct Path {
   uint32_t* col;
   __device__ __forceinline__ uint32_t get(int k) const { return col[k * kBlock]; }
@@ -1079,10 +1087,11 @@ struct PhaseClock {
 // wfAdvance (wavefront_core.h:157-231): descend to a leaf, skip the excluded
 // triangle, climb to the next pending far child; returns kContinue with a
 // triangle pending at r.cursor, or kFinished after emitting the output.
-// The descent records each ancestor in the lane's LDS path, so a climb reads
-// the pending child's parent there instead of following parent links; the
-// climb counter still adds one per level climbed, as the reference's walk
-// does. Both children of an inner node are fetched while it is decided.
+// The descent records each ancestor's far child in the lane's LDS path, so a
+// climb reads the pending child there and fetches its box, its parent and
+// the child itself at once, instead of following parent links; the climb
+// counter still adds one per level climbed, as the reference's walk does.
+// Both children of an inner node are fetched while it is decided.
 template <int App>
 __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& ss, const Path& path,
                        Cnt& cnt PHASE_ARG) {
@@ -1090,19 +1099,19 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
   const Beam& B = w.b;
   for (;;) {
     if (ss.phase == kDescend) {
-      KdNodeD kn = sv.nodes[r.node];
+      KdNodeD kn = ss.kn;
       for (;;) {
         if ((kn.data & 3u) == 3u) break;
         const KdNodeD kl = sv.nodes[r.node + 1], kr = sv.nodes[kn.data >> 2];
         cnt.v[sKd]++;
         const int d = r.depth++;
-        if (d < kPathLevels) path.set(d, r.node);
         const int a = int(kn.data & 3u);
         const float s = kn.split;
         const int sga = sgnOf(B, a);
         const float Ns = float(sga) * relApex(c, s, a);
         const uint32_t left = r.node + 1, right = kn.data >> 2;
         const bool toRight = sga > 0;
+        if (d < kPathLevels) path.set(d, toRight ? right : left);
         int dec = kFar;
         if (Ns > 0) dec = decide(c, B, a, Ns, ss.box);
         // kFar: into the child the rays move into; otherwise the apex side
@@ -1136,31 +1145,36 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
       const int d = highestBit(r.trail);
       cnt.v[sClimbs] += uint32_t(r.depth - d);
       PHASE_MARK(kPhRest);
-      uint32_t node;
+      uint32_t cs;
       if (d >= ss.validFrom && d < kPathLevels) {
-        node = path.get(d);
+        cs = path.get(d);
       } else {
-        // parent links from the shallowest known ancestor, recording the path
+        // parent links from the shallowest known ancestor below depth d (the
+        // parent of a recorded far child, or r.node) up to depth d
         int depth = r.depth;
-        node = r.node;
-        if (ss.validFrom < depth && ss.validFrom < kPathLevels) {
+        uint32_t node = r.node;
+        if (d < ss.validFrom && ss.validFrom < depth && ss.validFrom < kPathLevels) {
           depth = ss.validFrom;
-          node = path.get(depth);
+          node = sv.parent[path.get(depth)];
         }
         while (depth > d) {
           node = sv.parent[node];
           --depth;
-          if (depth < kPathLevels) path.set(depth, node);
         }
-        if (d < kPathLevels) ss.validFrom = d;
+        const KdNodeD kn = sv.nodes[node];
+        cs = sgnOf(B, int(kn.data & 3u)) > 0 ? (kn.data >> 2) : node + 1;
+        if (d < kPathLevels) {
+          path.set(d, cs);
+          ss.validFrom = d;
+        }
       }
       PHASE_MARK(kPhWalk);
       r.trail &= (uint64_t(1) << d) - 1;
-      const KdNodeD kn = sv.nodes[node];
-      const uint32_t cs = sgnOf(B, int(kn.data & 3u)) > 0 ? (kn.data >> 2) : node + 1;
+      const Box cb = sv.nodeBox[cs];
+      const uint32_t node = sv.parent[cs];
+      const KdNodeD kc = sv.nodes[cs];
       r.node = node;
       r.depth = int16_t(d);
-      const Box cb = sv.nodeBox[cs];
       const bool ov = overlaps(c, B, cb);
       PHASE_MARK(kPhBox);
       if (ov) {
@@ -1169,6 +1183,7 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
         r.cursor = -1;
         ss.phase = kDescend;
         ss.box = cb;
+        ss.kn = kc;
         break;
       }
     }
@@ -1431,6 +1446,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
             if (r.cursor < 0) {
               ss.phase = kDescend;
               ss.box = p.sv.nodeBox[r.node];
+              ss.kn = p.sv.nodes[r.node];
             } else {
               const KdNodeD kn = p.sv.nodes[r.node];
               ss.leafOffset = kn.offset;
