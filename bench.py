@@ -118,7 +118,7 @@ def main() -> int:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the tracer shards by independent roots and needs no collectives, see DESIGN.md", file=sys.stderr)
+              "the tracer shards by independent roots (pixels, tiles, jobs) and needs no collectives", file=sys.stderr)
         return 2
     if args.workload == "batch64":
         return bench_batch(args)
