@@ -60,9 +60,9 @@ METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request 
 # scaled by a 256 MiB copy: profiles/trace_plant512_hbm_traffic.txt and
 # profiles/trace_batch64_hbm_traffic.txt), valid for the schedule they were
 # measured with; other runs report null.
-TRAFFIC = {("plant", 512): 485239856}
+TRAFFIC = {("plant", 512): 485560317}
 # batch64: keyed by the schedule (queue capacity, budget, rounds per check, leaf cap)
-TRAFFIC_BATCH = {(1 << 27, 1024, 8, 8): 3308584644}
+TRAFFIC_BATCH = {(1 << 27, 1024, 8, 8): 3374877286}
 
 
 def parse():
