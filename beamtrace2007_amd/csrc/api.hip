@@ -621,6 +621,7 @@ struct btg_server {
   std::unordered_map<uint64_t, SrvReq*> reqs;
   uint64_t nextId = 1;
   bool stop = false;
+  std::string softDown;  // set when the soft-shadow loop has ended on an error
   std::string err;
   std::thread softThread, pointThread;
 };
@@ -667,8 +668,12 @@ void softLoop(btg_server* s) {
         btg::OpenPass* acc = s->pass[s->active];
         if (!acc) break;
         if (btg::openIdle(acc) && btg::openJobs(acc) > 0) {
-          if (btg::openReset(acc, err)) break;
           s->slots[s->active].clear();
+          if (btg::openReset(acc, err)) {  // rebuilt below, before it takes requests
+            btg::openDestroy(acc);
+            s->pass[s->active] = nullptr;
+            break;
+          }
         }
         int slot = -1;
         const int rc = btg::openTryAdd(acc, jobOf(r->req), &slot, err);
@@ -685,8 +690,12 @@ void softLoop(btg_server* s) {
         }
         const int o = 1 - s->active;
         if (s->pass[o] && btg::openIdle(s->pass[o])) {
-          if (btg::openJobs(s->pass[o]) > 0 && btg::openReset(s->pass[o], err)) break;
           s->slots[o].clear();
+          if (btg::openJobs(s->pass[o]) > 0 && btg::openReset(s->pass[o], err)) {
+            btg::openDestroy(s->pass[o]);
+            s->pass[o] = nullptr;
+            break;
+          }
           s->active = o;
           continue;
         }
@@ -694,7 +703,7 @@ void softLoop(btg_server* s) {
       }
       const bool busy0 = s->pass[0] && !btg::openIdle(s->pass[0]);
       const bool busy1 = s->pass[1] && !btg::openIdle(s->pass[1]);
-      if (!busy0 && !busy1) {
+      if (!busy0 && !busy1 && s->pass[0] && s->pass[1]) {  // a missing pass is rebuilt first
         if (s->stop && s->softQ.empty()) return;
         s->cvSoft.wait(lk);
         continue;
@@ -715,12 +724,16 @@ void softLoop(btg_server* s) {
         continue;
       }
       for (size_t i = 0; i < completed.size(); ++i) {
-        SrvReq* r;
+        SrvReq* r = nullptr;
         {
           std::lock_guard<std::mutex> lk(s->m);
-          r = s->slots[g][completed[i]];
-          s->slots[g].erase(completed[i]);
+          const auto it = s->slots[g].find(completed[i]);
+          if (it != s->slots[g].end()) {
+            r = it->second;
+            s->slots[g].erase(it);
+          }
         }
+        if (!r) continue;  // a slot the server never filled: nothing waits on it
         int rc = BTG_OK;
         if (killed[i] > 0) {  // over budget: trace the request again alone, with GPU retries
           btg::SoftJobH h = jobOf(r->req);
@@ -738,9 +751,14 @@ void softLoop(btg_server* s) {
             btg::openCreate(s->ctx->engine, s->rootCap, s->pixCap, s->jobsPerPass, true, s->queueCap, err);
         std::lock_guard<std::mutex> lk(s->m);
         s->pass[g] = op;
-        if (!op) {  // no pass left to run on: fail what waits
+        if (!op) {  // no pass left to run on: fail what waits and refuse what comes
           for (SrvReq* r : s->softQ) finishLocked(s, r, BTG_ERR_DEVICE, err);
           s->softQ.clear();
+          for (auto& sl : s->slots) {
+            for (auto& kv : sl) finishLocked(s, kv.second, BTG_ERR_DEVICE, err);
+            sl.clear();
+          }
+          s->softDown = "soft-shadow passes could not be rebuilt: " + err;
           return;
         }
       }
@@ -872,6 +890,10 @@ int btg_server_submit(btg_server* s, const btg_request* req, uint64_t* ticket) {
     s->err = "invalid request (kind, camera, or a soft request without exact = 1)";
     return BTG_ERR_ARG;
   }
+  if (req->kind == BTG_REQ_SOFT && !s->softDown.empty()) {
+    s->err = s->softDown;
+    return BTG_ERR_DEVICE;
+  }
   try {
     SrvReq* r = new SrvReq();
     r->id = s->nextId++;
@@ -896,18 +918,27 @@ int btg_server_submit(btg_server* s, const btg_request* req, uint64_t* ticket) {
 int btg_server_wait(btg_server* s, uint64_t ticket, double timeoutSeconds, btg_request_status* st) {
   if (!s || !st) return BTG_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->m);
-  auto it = s->reqs.find(ticket);
-  if (it == s->reqs.end()) {
+  if (s->reqs.find(ticket) == s->reqs.end()) {
     s->err = "unknown ticket";
     return BTG_ERR_ARG;
   }
-  SrvReq* r = it->second;
-  auto ready = [&] { return r->done; };
+  // The request is looked up again on every wake-up: another waiter on the
+  // same ticket may have released it meanwhile.
+  auto ready = [&] {
+    const auto i = s->reqs.find(ticket);
+    return i == s->reqs.end() || i->second->done;
+  };
   if (timeoutSeconds < 0)
     s->cvDone.wait(lk, ready);
   else
     s->cvDone.wait_for(lk, std::chrono::duration<double>(timeoutSeconds), ready);
   std::memset(st, 0, sizeof(*st));
+  const auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) {
+    s->err = "ticket released by another waiter";
+    return BTG_ERR_ARG;
+  }
+  SrvReq* r = it->second;
   if (!r->done) return BTG_OK;
   st->done = 1;
   st->status = r->status;

@@ -301,7 +301,8 @@ typedef struct {
 int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** out);
 int btg_server_submit(btg_server* s, const btg_request* req, uint64_t* ticket);
 /* Waits up to timeoutSeconds (< 0: no limit) for a ticket; status->done = 0 on
- * timeout. A finished ticket is released by this call. */
+ * timeout. A finished ticket is released by this call, once: another waiter on
+ * the same ticket then returns BTG_ERR_ARG. */
 int btg_server_wait(btg_server* s, uint64_t ticket, double timeoutSeconds, btg_request_status* status);
 const char* btg_server_error(btg_server* s);
 /* Finishes the requests in flight, then frees the server. */

@@ -183,3 +183,30 @@ def test_bad_requests_fail_alone(require_gpu, server_setup):
         assert np.array_equal(res["vis"], a["vis"])
     with pytest.raises(RuntimeError):
         srv.submit_soft(ids["room"], camera(sp["view"], 32, 24), 0, 24, sp["area_light"])
+
+
+def test_two_waiters_on_one_ticket(require_gpu, server_setup):
+    """Two threads wait on the same ticket: exactly one receives the result,
+    the other an error (the ticket is released once), and neither hangs."""
+    from beamtrace2007_amd.host import camera
+    srv, ids = server_setup
+    sp = load_scene("room")[3]
+    cam = camera(sp["view"], 48, 40)
+    t = srv.submit_soft(ids["room"], cam, 48, 40, sp["area_light"])
+    got, errors = [], []
+
+    def waiter():
+        try:
+            got.append(srv.wait(t, timeout=120))
+        except RuntimeError as e:
+            errors.append(str(e))
+
+    th = [threading.Thread(target=waiter) for _ in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(timeout=180)
+    assert not any(x.is_alive() for x in th)
+    assert len(got) == 1 and len(errors) == 1
+    assert got[0] is not None and got[0]["status"] == 0
+    assert np.array_equal(got[0]["vis"], soft_alone("room", cam, 48, 40, sp["area_light"])["vis"])
