@@ -951,7 +951,15 @@ int btg_server_wait(btg_server* s, uint64_t ticket, double timeoutSeconds, btg_r
   return BTG_OK;
 }
 
-const char* btg_server_error(btg_server* s) { return s ? s->err.c_str() : ""; }
+const char* btg_server_error(btg_server* s) {
+  if (!s) return "";
+  // the worker threads may replace s->err at any time: hand out a copy taken
+  // under the lock, valid until this thread's next call
+  thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(s->m);
+  copy = s->err;
+  return copy.c_str();
+}
 
 void btg_server_destroy(btg_server* s) {
   if (!s) return;

@@ -304,6 +304,8 @@ int btg_server_submit(btg_server* s, const btg_request* req, uint64_t* ticket);
  * timeout. A finished ticket is released by this call, once: another waiter on
  * the same ticket then returns BTG_ERR_ARG. */
 int btg_server_wait(btg_server* s, uint64_t ticket, double timeoutSeconds, btg_request_status* status);
+/* The server's last error message; the string stays valid until this
+ * thread's next btg_server_error call. */
 const char* btg_server_error(btg_server* s);
 /* Finishes the requests in flight, then frees the server. */
 void btg_server_destroy(btg_server* s);
