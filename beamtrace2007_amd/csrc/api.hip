@@ -89,7 +89,11 @@ double now() {
   return double(ts.tv_sec) + 1e-9 * double(ts.tv_nsec);
 }
 
-bool validCam(const btg_camera* c) { return c && c->W > 0 && c->H > 0 && c->halfW > 0 && c->halfH > 0; }
+// Image sides up to 2^15 keep pixel, tile and polygon counts inside int.
+constexpr int kMaxSide = 1 << 15;
+bool validCam(const btg_camera* c) {
+  return c && c->W > 0 && c->H > 0 && c->W <= kMaxSide && c->H <= kMaxSide && c->halfW > 0 && c->halfH > 0;
+}
 
 }  // namespace
 

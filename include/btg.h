@@ -101,7 +101,7 @@ int btg_get_options(const btg_ctx* ctx, btg_options* opt);
 int btg_set_options(btg_ctx* ctx, const btg_options* opt);
 
 /* Camera (src/render/render_types.h:15-39): image plane at distance 1 along
- * fwd, q in [-halfW, halfW] x [-halfH, halfH]. */
+ * fwd, q in [-halfW, halfW] x [-halfH, halfH]. W and H: 1..32768 pixels. */
 typedef struct {
   float eye[3], fwd[3], right[3], down[3];
   float halfW, halfH;

@@ -252,6 +252,9 @@ def test_invalid_arguments_are_errors_not_crashes(require_gpu):
         r.primary(cam, 16, 16, tile=0)
     with pytest.raises(RuntimeError):
         r.soft_shadows(cam, 0, 16, sp["area_light"])
+    for W, H in ((40000, 16), (16, 40000)):  # sides above 32768: refused before any allocation
+        with pytest.raises(RuntimeError):
+            r.primary(camera(sp["view"], W, H), W, H, tile=16)
     bad = np.zeros(1, dtype=__import__("beamtrace2007_amd.gpu", fromlist=["POLY_DTYPE"]).POLY_DTYPE)
     bad["tri"] = -1
     with pytest.raises(RuntimeError):
