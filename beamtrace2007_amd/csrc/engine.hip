@@ -1417,6 +1417,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
   cnt.zero();
   PHASE_INIT();
   const int n = int(min((unsigned long long)p.st->count[cur], p.capacity));
+  if (n == 0) return;  // an empty round (a pass draining, or an idle open pass)
   const Rec* q = p.tq[cur];
   bool active = false, exhausted = false;
   int leaves = 0;
@@ -1584,7 +1585,7 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
   cnt.zero();
   const int n = int(min((unsigned long long)p.st->splitCount, p.capacity));
   const int nxt = cur ^ 1;
-  for (;;) {
+  for (; n > 0;) {  // an empty round skips to the scheduling below
     int base = 0;
     if (lane() == 0) base = atomicAdd(&p.st->head[kHeadSplit], kWave);
     base = __shfl(base, 0, kWave);
@@ -2079,6 +2080,7 @@ __global__ void __launch_bounds__(kBlock) admitKernel(Params p, int cur) {
   Cnt cnt;
   cnt.zero();
   const int begin = p.st->admitBegin, n = p.st->admitCount;
+  if (n == 0) return;  // most rounds admit nothing: no queue atomics, no counter flush
   for (;;) {
     int base = 0;
     if (lane() == 0) base = atomicAdd(&p.st->head[kHeadAdmit], kWave);
