@@ -806,11 +806,23 @@ __device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5],
   }
 }
 
+// Pieces of one clip kept in LDS between the counting and the emitting pass:
+// up to kStageSlots per lane, 11 words each (corners 0..4, then n | list << 8;
+// pieces have at most 5 corners), word k of slot s of a lane at
+// col[(s * kStageWords + k) * kBlock].
+constexpr int kStageSlots = 7, kStageWords = 11;
+
 // Counting pass: records each list produces (pentagons become two) and the
 // total area of the hit pieces (summed in piece order, as the reference).
+// With a staging column it also keeps the pieces, in push order, so that the
+// emitting pass replays them instead of clipping again; a clip with more
+// pieces than slots is marked and clipped again.
 struct CountSink {
   int recs0 = 0, recs1 = 0;
   float hitArea = 0;
+  float* col = nullptr;
+  int staged = 0;
+  bool overflow = false;
   __device__ __forceinline__ void push(int l, const Pg<6>& q) {
     const int k = q.n == 5 ? 2 : 1;
     if (l == 1) {
@@ -819,6 +831,19 @@ struct CountSink {
     } else {
       recs0 += k;
     }
+    if (!col) return;
+    if (staged >= kStageSlots) {
+      overflow = true;
+      return;
+    }
+    float* w = col + staged * kStageWords * kBlock;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      w[v * kBlock] = q.x[v];
+      w[(5 + v) * kBlock] = q.y[v];
+    }
+    w[10 * kBlock] = __int_as_float(q.n | (l << 8));
+    ++staged;
   }
 };
 
@@ -1583,6 +1608,8 @@ template <int App>
 __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, int cur) {
   Cnt cnt;
   cnt.zero();
+  __shared__ float stageLds[kStageSlots * kStageWords * kBlock];
+  float* const stageCol = stageLds + threadIdx.x;
   const int n = int(min((unsigned long long)p.st->splitCount, p.capacity));
   const int nxt = cur ^ 1;
   for (; n > 0;) {  // an empty round skips to the scheduling below
@@ -1599,6 +1626,7 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
     Ln L[5];
     int nl = 0;
     int recs0 = 0;
+    int staged = -1;  // pieces kept in LDS by the counting pass (-1: clip again)
     int tri = -1;
     if (i < n) {
       r = p.sq[i];
@@ -1617,12 +1645,14 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
       tri = ti.tri;
       classifyTri(c, w, ti, L, nl);  // kClip, as found by the trace
       CountSink cs;
+      cs.col = stageCol;
       Ln Lc[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) Lc[k] = L[k];
       clipTri(c, w.b, Lc, nl, cs, cnt.v[sFive]);
       commit = cs.hitArea >= c.minArea;
       recs0 = cs.recs0;
+      staged = cs.overflow ? -1 : cs.staged;
       slots = commit ? cs.recs0 + (c.anyHit() ? 0 : cs.recs1) : 1;
       const int old = atomicAdd(&p.live[r.root], slots - 1);
       if (old + slots - 1 > p.budget) {
@@ -1657,8 +1687,24 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
         es.k1 = recs0;
         es.t = tri;
         es.cnt = &cnt;
-        uint32_t dummyFive = 0;
-        clipTri(c, w.b, L, nl, es, dummyFive);
+        if (staged >= 0) {  // replay the counting pass's pieces in their order
+          for (int k = 0; k < staged; ++k) {
+            const float* rd = stageCol + k * kStageWords * kBlock;
+            Pg<6> q;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+              q.x[v] = rd[v * kBlock];
+              q.y[v] = rd[(5 + v) * kBlock];
+            }
+            q.x[5] = q.y[5] = 0;
+            const int tag = __float_as_int(rd[10 * kBlock]);
+            q.n = tag & 0xff;
+            es.push(tag >> 8, q);
+          }
+        } else {
+          uint32_t dummyFive = 0;
+          clipTri(c, w.b, L, nl, es, dummyFive);
+        }
       }
     }
   }
