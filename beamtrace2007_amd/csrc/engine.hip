@@ -385,10 +385,14 @@ __device__ __forceinline__ void setAw(Beam& b, const Ctx& c) {
 
 // beamGeom (beam_core.h:200-221): corners (lanes past n repeat the last) and
 // |w| per axis; slivers below minArea are dropped.
+// `welded`: p is already the output of cleanPoly, whose last pass over the
+// vertices removed nothing, so a further cleanPoly would return it unchanged
+// and is skipped.
 template <int N>
-__device__ __forceinline__ bool beamGeom(const Ctx& c, const Pg<N>& p, int s0, int s1, int s2, Beam& b, float& dropped) {
+__device__ __forceinline__ bool beamGeom(const Ctx& c, const Pg<N>& p, int s0, int s1, int s2, Beam& b, float& dropped,
+                                         bool welded = false) {
   Pg<N> q = p;
-  cleanPoly(q, c.eps);
+  if (!welded) cleanPoly(q, c.eps);
   const float A = q.n >= 3 ? q.area() : 0.0f;
   if (q.n < 3 || A < c.minArea) {
     dropped = selAbs(p.area());
@@ -729,6 +733,10 @@ __device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5],
     p.y[i] = B.qy.v[i];
   }
   p.x[4] = p.x[5] = p.y[4] = p.y[5] = 0;
+  // p is welded (the output of cleanPoly) while it is the beam itself (every
+  // beam polygon comes out of beamGeom) or a welded split part; a pentagon's
+  // quad part and a pending triangle are not
+  bool welded = true;
   // pending triangle halves by start line 1..5 (slot 0 unused)
   float px[6][3], py[6][3];
   uint32_t pending = 0;
@@ -736,10 +744,12 @@ __device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5],
   for (;;) {
     int list = -1;  // the piece handed to the sink in this step
     Pg<6> q;
+    bool qWelded = false;
     bool pieceDone = false;
     if (li >= nl) {  // inside every line: the part of the beam the triangle covers
       list = 1;
       q = p;
+      qWelded = welded;
       pieceDone = true;
     } else {
       const Ln Lk = lineAt(L, li);
@@ -749,6 +759,7 @@ __device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5],
       } else if (!(cs & 1)) {
         list = 0;
         q = p;
+        qWelded = welded;
         pieceDone = true;
       } else {
         Pg<6> in, out;
@@ -757,6 +768,7 @@ __device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5],
         if (out.n >= 3) {
           list = 0;
           q = out;
+          qWelded = true;
         }
         cleanPoly(in, c.eps);
         if (in.n < 3) {
@@ -778,19 +790,22 @@ __device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5],
           pending |= 1u << lv;
           p = in;
           p.n = 4;
+          welded = false;
           li = lv;
         } else {
           p = in;
+          welded = true;
           ++li;
         }
       }
     }
-    if (list >= 0) sink.push(list, q);
+    if (list >= 0) sink.push(list, q, qWelded);
     if (pieceDone) {
       if (!pending) break;
       const int lv = 31 - __builtin_clz(pending);
       pending &= ~(1u << lv);
       p.n = 3;
+      welded = false;
 #pragma unroll
       for (int s = 1; s < 6; ++s) {
         if (s == lv) {
@@ -807,7 +822,8 @@ __device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5],
 }
 
 // Pieces of one clip kept in LDS between the counting and the emitting pass:
-// up to kStageSlots per lane, 11 words each (corners 0..4, then n | list << 8;
+// up to kStageSlots per lane, 11 words each (corners 0..4, then
+// n | list << 8 | welded << 9;
 // pieces have at most 5 corners), word k of slot s of a lane at
 // col[(s * kStageWords + k) * kBlock].
 constexpr int kStageSlots = 7, kStageWords = 11;
@@ -823,7 +839,7 @@ struct CountSink {
   float* col = nullptr;
   int staged = 0;
   bool overflow = false;
-  __device__ __forceinline__ void push(int l, const Pg<6>& q) {
+  __device__ __forceinline__ void push(int l, const Pg<6>& q, bool welded) {
     const int k = q.n == 5 ? 2 : 1;
     if (l == 1) {
       recs1 += k;
@@ -842,7 +858,7 @@ struct CountSink {
       w[v * kBlock] = q.x[v];
       w[(5 + v) * kBlock] = q.y[v];
     }
-    w[10 * kBlock] = __int_as_float(q.n | (l << 8));
+    w[10 * kBlock] = __int_as_float(q.n | (l << 8) | (welded ? 1 << 9 : 0));
     ++staged;
   }
 };
@@ -1557,24 +1573,24 @@ struct EmitSink {
   int k0, k1;  // next record of list 0 / list 1
   int t;
   Cnt* cnt;
-  __device__ __forceinline__ void push(int l, const Pg<6>& q0) {
+  __device__ __forceinline__ void push(int l, const Pg<6>& q0, bool welded) {
     if (q0.n == 5) {
       Pg<6> sub0, sub1;
       splitFive(q0, sub0, sub1);
       cnt->v[sFive]++;
-      one(l, sub0);
-      one(l, sub1);
+      one(l, sub0, false);
+      one(l, sub1, false);
     } else {
-      one(l, q0);
+      one(l, q0, welded);
     }
   }
-  __device__ __forceinline__ void one(int l, const Pg<6>& sub) {
+  __device__ __forceinline__ void one(int l, const Pg<6>& sub, bool welded) {
     const bool queued = !(l == 1 && c->anyHit());
     const int hit = l == 1 ? t : parent->hit;
     {
       Beam nb;
       float dropped = 0;
-      const bool ok = beamGeom(*c, sub, parent->s0, parent->s1, parent->s2, nb, dropped);
+      const bool ok = beamGeom(*c, sub, parent->s0, parent->s1, parent->s2, nb, dropped, welded);
       if (!ok) {
         cnt->dropped += double(dropped);
       } else if (!queued) {
@@ -1699,7 +1715,7 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
             q.x[5] = q.y[5] = 0;
             const int tag = __float_as_int(rd[10 * kBlock]);
             q.n = tag & 0xff;
-            es.push(tag >> 8, q);
+            es.push((tag >> 8) & 1, q, ((tag >> 9) & 1) != 0);
           }
         } else {
           uint32_t dummyFive = 0;

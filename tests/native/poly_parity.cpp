@@ -137,6 +137,10 @@ int main(int argc, char** argv) {
         btg::cleanPoly(gi, eps);
         btg::cleanPoly(go, eps);
         if (!same(gi, in, "cleanPoly(in)", k) || !same(go, out, "cleanPoly(out)", k)) return 1;
+        Poly in2 = in, out2 = out;
+        weld(in2, eps);
+        weld(out2, eps);
+        if (!same(gi, in2, "cleanPoly(in) twice", k) || !same(go, out2, "cleanPoly(out) twice", k)) return 1;
       }
     }
     {
@@ -145,6 +149,11 @@ int main(int argc, char** argv) {
       weld(w, eps);
       btg::cleanPoly(gw, eps);
       if (!same(gw, w, "cleanPoly", k)) return 1;
+      // welding is idempotent (the split kernel skips re-welding welded
+      // pieces): a second pass of the oracle's weld changes nothing
+      Poly w2 = w;
+      weld(w2, eps);
+      if (!same(gw, w2, "cleanPoly twice", k)) return 1;
     }
     {
       Poly rv = p;
