@@ -287,6 +287,26 @@ int btg_scene_remove(btg_ctx* ctx, int sceneId) {
   return btg::engineRemoveScene(ctx->engine, sceneId, ctx->err);
 }
 
+int btg_footprint(const btg_ctx* ctx, const btg_request* req, int64_t* bytes) {
+  if (!ctx || !req || !bytes || !validCam(&req->cam)) return BTG_ERR_ARG;
+  const int64_t W = req->cam.W, H = req->cam.H;
+  if (req->kind == BTG_REQ_SOFT) {
+    // roots: 64 per 8 x 8 pixel tile, partial tiles included (the soft passes' layout)
+    const int64_t roots = ((W + 7) / 8) * ((H + 7) / 8) * 64;
+    *bytes = W * H * btg::softPixelBytes(btg::engineOptions(ctx->engine).precision != 0) +
+             roots * btg::rootStateBytes();
+    return BTG_OK;
+  }
+  if (req->kind == BTG_REQ_POINT) {
+    // the primary pass's roots (16 x 16 tiles) and polygon buffer (polygon, root and tag)
+    const int64_t roots = ((W + 15) / 16) * ((H + 15) / 16);
+    const int64_t polys = std::max<int64_t>(int64_t(1) << 20, 2 * W * H);
+    *bytes = roots * btg::rootStateBytes() + polys * int64_t(sizeof(btg::OutPoly) + 8);
+    return BTG_OK;
+  }
+  return BTG_ERR_ARG;
+}
+
 int64_t btg_scene_bytes(const btg_ctx* ctx, int sceneId) {
   return ctx ? btg::engineSceneBytes(ctx->engine, sceneId) : -1;
 }
@@ -837,15 +857,17 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
   const btg::Options& opt = btg::engineOptions(ctx->engine);
   // per pass: three sub-beam queues (64 B records) and the shade queue (40 B)
   // in at most a quarter of hbmBytes (the largest power-of-two capacity that
-  // fits, and no more than the context's), then per pixel 40 B of images and
-  // per root ~164 B of state
+  // fits, and no more than the context's), then per pixel its images and per
+  // root its state (softPixelBytes, rootStateBytes: a full 8 x 8 tile has a
+  // root per pixel)
   int64_t queueCap = int64_t(1) << 16;
   while (queueCap * 2 <= opt.capacity && queueCap * 2 * (3 * 64 + 40) <= hbm / 4) queueCap *= 2;
   const int64_t queues = queueCap * (3 * 64 + 40);
   const int64_t perPass = hbm / 2 - queues;
   if (perPass < (int64_t(1) << 26))
     return failWith(ctx, BTG_ERR_ARG, "hbmBytes too small for two passes at the configured queue capacity");
-  const int64_t pixCap = std::min<int64_t>(int64_t(1) << 28, perPass / (40 + 164));
+  const int64_t pixCap = std::min<int64_t>(
+      int64_t(1) << 28, perPass / (btg::softPixelBytes(opt.precision != 0) + btg::rootStateBytes()));
   const int64_t rootCap = std::min<int64_t>(int64_t(1) << 30, pixCap + int64_t(jobs) * 16448);
   try {
     btg_server* s = new btg_server();

@@ -116,6 +116,11 @@ int engineAddScene(Engine* e, const btg_scene_desc& scene, float offset, float o
 int engineRemoveScene(Engine* e, int scene, std::string& err);
 // Device bytes a resident scene occupies (-1: no such scene).
 int64_t engineSceneBytes(const Engine* e, int scene);
+// Device bytes of one root's state (its query, live count, state, epoch and
+// exact sums) and of one soft-shadow pixel's images and hit point (guard: the
+// FP64-guarded mode's extra low word of the hit point).
+int64_t rootStateBytes();
+int64_t softPixelBytes(bool guard);
 
 // A soft-shadow job: resident scene, camera, light, host outputs (any may be
 // null: the image stays in device memory), W*H each.

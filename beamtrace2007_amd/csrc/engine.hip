@@ -2976,6 +2976,11 @@ int64_t engineSceneBytes(const Engine* e, int scene) {
                  sr.refs.size() * sizeof(LeafTri));
 }
 
+// roots: RootQ, live, state, epoch, missFx, gFx (as prepare allocates them)
+int64_t rootStateBytes() { return int64_t(sizeof(RootQ) + 3 * sizeof(int) + 2 * sizeof(unsigned long long)); }
+// pixels: pixTri, pixX, pixN, vis, E, Eu, and pixXlo when guarded
+int64_t softPixelBytes(bool guard) { return 4 + 12 + 12 + 4 + 4 + 4 + (guard ? 12 : 0); }
+
 void engineDestroy(Engine* e) { delete e; }
 
 Options& engineOptions(Engine* e) { return e->opt; }

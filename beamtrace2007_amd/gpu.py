@@ -95,7 +95,7 @@ EXPORTS = ["btg_device_query", "btg_create", "btg_destroy", "btg_last_error", "b
            "btg_soft", "btg_scene_add", "btg_scene_remove", "btg_scene_bytes", "btg_soft_batch", "btg_autotune", "btg_server_create", "btg_server_submit", "btg_server_wait",
            "btg_server_error", "btg_server_destroy", "btg_polys_count",
            "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point", "btg_raster", "btg_images",
-           "btg_create_file", "btg_scene_add_file"]
+           "btg_create_file", "btg_scene_add_file", "btg_footprint"]
 
 
 def lib() -> C.CDLL:
@@ -141,6 +141,7 @@ def lib() -> C.CDLL:
         L.btg_raster.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.c_int, vp, vp]
         L.btg_create_file.argtypes = [C.POINTER(vp), C.c_char_p]
         L.btg_scene_add_file.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
+        L.btg_footprint.argtypes = [vp, C.POINTER(btg_request), C.POINTER(C.c_int64)]
         L.btg_images.argtypes = [vp, C.POINTER(btg_camera), C.c_int, C.c_int, vp, vp, vp, C.POINTER(btg_run_info),
                                  C.POINTER(btg_run_info), C.POINTER(C.c_double)]
         _lib = L
@@ -438,6 +439,17 @@ class Server:
         for f in ("tri", "vis", "E", "Eu"):
             setattr(q, f, o[f].ctypes.data)
         return self._submit(q, o)
+
+    def footprint(self, kind: str, cam14, W: int, H: int) -> int:
+        """btg_footprint: device bytes a "soft" or "point" request holds while in flight."""
+        q = btg_request()
+        q.kind, q.scene, q.exact = (0 if kind == "soft" else 1), 0, 1
+        q.cam = make_camera(cam14, W, H)
+        b = C.c_int64(0)
+        rc = lib().btg_footprint(self._r._h, C.byref(q), C.byref(b))
+        if rc:
+            raise RuntimeError(f"btg_footprint failed ({rc})")
+        return int(b.value)
 
     def submit_point(self, cam14, W: int, H: int, light3) -> int:
         q = btg_request()

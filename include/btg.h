@@ -298,11 +298,21 @@ typedef struct {
   btg_polys* polys;        /* point: shadowed polygons, canonical order; the caller frees them */
 } btg_request_status;
 
+/* HBM footprint of a request: the device bytes it holds while in flight,
+ * resident scenes excluded. Soft shadows: its images and hit points per pixel
+ * plus the state of its roots (64 per 8 x 8 pixel tile), the quantities a
+ * server's admission counts against its passes (hbmBytes / 2 each, less the
+ * queues). Point shadows: the primary pass's tile roots and polygon buffer; the
+ * point pass then adds one root per primary hit polygon. Plays the part of the
+ * footprint query of a batching/serving front end (no counterpart in
+ * cuda/gpu_render.h, which serves one call at a time). */
+int btg_footprint(const btg_ctx* ctx, const btg_request* req, int64_t* bytes);
+
 int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** out);
 int btg_server_submit(btg_server* s, const btg_request* req, uint64_t* ticket);
-/* Waits up to timeoutSeconds (< 0: no limit) for a ticket; status->done = 0 on
- * timeout. A finished ticket is released by this call, once: another waiter on
- * the same ticket then returns BTG_ERR_ARG. */
+/* Waits up to timeoutSeconds (< 0: no limit; 0: a poll) for a ticket;
+ * status->done = 0 on timeout. A finished ticket is released by this call,
+ * once: another waiter on the same ticket then returns BTG_ERR_ARG. */
 int btg_server_wait(btg_server* s, uint64_t ticket, double timeoutSeconds, btg_request_status* status);
 /* The server's last error message; the string stays valid until this
  * thread's next btg_server_error call. */
