@@ -63,6 +63,10 @@ METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request 
 TRAFFIC = {("plant", 512): 485560317}
 # batch64: keyed by the schedule (queue capacity, budget, rounds per check, leaf cap)
 TRAFFIC_BATCH = {(1 << 27, 1024, 8, 8): 3374877286}
+# What bounds the trace kernel instead of HBM: VALU issue under divergence,
+# measured with rocprofv3 PMC counters for the same schedule
+# (profiles/trace_batch64_valu_busy.txt, profiles/trace_batch64_sq_counters.txt)
+ISSUE_BATCH = {(1 << 27, 1024, 8, 8): {"valu_busy": 0.787, "lane_utilisation": 0.229}}
 
 
 def parse():
@@ -304,6 +308,11 @@ def bench_batch(args) -> int:
                     "launches_per_step": trace_launches / args.steps,
                     "avg_launch_ms": round(trace_s / max(1, trace_launches) * 1e3, 4), "alg_bytes_per_step": int(b_trace)}
     value = px * args.steps / wall
+    sched = (capacity, tuned["budget"], tuned["rounds_per_check"], tuned["leaf_cap"])
+    issue = None if sched not in ISSUE_BATCH else dict(
+        ISSUE_BATCH[sched], kernel="traceKernel",
+        meaning="fraction of SIMD cycles issuing a VALU instruction, and of lanes active per VALU instruction "
+                "(rocprofv3 PMC, this schedule)")
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
@@ -322,7 +331,7 @@ def bench_batch(args) -> int:
         "retry_passes": infos[-1]["retry_passes"],
         "kernel_ms_per_step": {k: round(sum(i["kernel_s"][k] for i in infos) / args.steps * 1e3, 3)
                                for k in gpu.KERNELS},
-        "roofline": roofline, "host_kd_build_s": round(build_s, 3),
+        "roofline": roofline, "issue_bound": issue, "host_kd_build_s": round(build_s, 3),
         "cpu_baseline": None if args.no_cpu else cpu_batch_sample(scenes, jobs, draw),
         "precision": precision,
         "autotune": None if args.no_tune else {"chosen": tuned, "seconds": round(tune_s, 2), "trials": trials,
