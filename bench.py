@@ -60,13 +60,13 @@ METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request 
 # scaled by a 256 MiB copy: profiles/trace_plant512_hbm_traffic.txt and
 # profiles/trace_batch64_hbm_traffic.txt), valid for the schedule they were
 # measured with; other runs report null.
-TRAFFIC = {("plant", 512): 485560317}
+TRAFFIC = {("plant", 512): 486063220}
 # batch64: keyed by the schedule (queue capacity, budget, rounds per check, leaf cap)
-TRAFFIC_BATCH = {(1 << 27, 1024, 8, 8): 3374877286}
+TRAFFIC_BATCH = {(1 << 28, 1024, 8, 8): 5061659403}
 # What bounds the trace kernel instead of HBM: VALU issue under divergence,
 # measured with rocprofv3 PMC counters for the same schedule
 # (profiles/trace_batch64_valu_busy.txt, profiles/trace_batch64_sq_counters.txt)
-ISSUE_BATCH = {(1 << 27, 1024, 8, 8): {"valu_busy": 0.787, "lane_utilisation": 0.229}}
+ISSUE_BATCH = {(1 << 28, 1024, 8, 8): {"valu_busy": 0.808, "lane_utilisation": 0.230}}
 
 
 def parse():
