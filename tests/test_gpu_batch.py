@@ -189,3 +189,27 @@ def test_scenes_from_resident_files(require_gpu, tmp_path):
         b.add_scene(str(tmp_path / "missing.btgs"))
     a.close()
     b.close()
+
+
+def test_largest_queue_capacity_gives_identical_images(require_gpu):
+    """2^28 queue records (the benchmark's batch capacity on a 288 GB device:
+    62 GB of queues, record offsets past 2^32 bytes) against 2^22 records (64x
+    fewer roots in flight, many more rounds): identical images, bit for bit,
+    on a batch whose roots x budget fill the large queues."""
+    from beamtrace2007_amd.gpu import GpuRenderer, device_info
+    from beamtrace2007_amd.host import camera
+    if device_info()["mem_gb"] < 200:
+        pytest.skip("needs a device with room for 62 GB of queues")
+    s, _, _, sp = load_scene("plant")
+    jobs = [{"scene": 0, "cam": camera(sp["view"], 256, 256), "W": 256, "H": 256,
+             "light": scaled_light(sp["area_light"], k)} for k in (0.5, 1.0, 2.0, 4.0)]
+    outs = {}
+    for cap in (1 << 28, 1 << 22):
+        r = GpuRenderer(s)
+        r.set_options(capacity=cap)
+        print("capacity", cap, flush=True)
+        outs[cap], _ = r.soft_batch(jobs, exact=True)
+        r.close()
+    for a, b in zip(outs[1 << 28], outs[1 << 22]):
+        for f in ("tri", "vis", "E", "Eu"):
+            assert np.array_equal(a[f], b[f]), f
