@@ -281,3 +281,31 @@ def test_guarded_soft_shadows_match_guarded_oracle(require_gpu, name):
         r.set_options(**saved)
     o = oracle(name).soft(cam, W, H, sp["area_light"], exact=True, guard=True)
     check_soft(g, o)
+
+
+@pytest.mark.parametrize("name", ["room", "conference"])
+def test_primary_and_point_independent_of_leaf_cap(require_gpu, name):
+    """Yields (a sub-beam's record continues in the next round) and the pooled
+    nearest-hit tests restarting after an all-in hit leave polygons and
+    counters unchanged: leaf cap 1 (a yield after every leaf) and no cap equal
+    the default (primary passes at most 4) bit for bit."""
+    from beamtrace2007_amd.gpu import GpuRenderer, polys_to_dict
+    from beamtrace2007_amd.host import camera
+    s, a, k, sp = load_scene(name)
+    W, H = 160, 120
+    cam = camera(sp["view"], W, H)
+    r = GpuRenderer(s)
+    res = {}
+    for cap in (None, 1, 0):
+        if cap is not None:
+            r.set_options(leaf_cap=cap)
+        print(f"{name} leaf cap {cap}", flush=True)
+        hits, info = r.primary(cam, W, H, tile=16)
+        shadow, info2 = r.point_shadows(cam, W, H, hits, sp["point_light"])
+        res[cap] = (canon(polys_to_dict(hits)), canon(polys_to_dict(shadow)), info["stats"], info2["stats"])
+    r.close()
+    for cap in (1, 0):
+        assert np.array_equal(res[cap][0], res[None][0]) and np.array_equal(res[cap][1], res[None][1])
+        for key in ("kdSteps", "leafVisits", "triTests", "hits", "splits", "beams", "climbs"):
+            assert res[cap][2][key] == res[None][2][key], key
+            assert res[cap][3][key] == res[None][3][key], key
