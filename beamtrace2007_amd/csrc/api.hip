@@ -893,6 +893,11 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
     }
     btg::engineOptions(s->pointEngine) = opt;
     btg::engineOptions(s->fallback) = opt;
+    // The server's passes run beside the caller's threads: no graph capture
+    // (while a stream captures, other threads' legacy-stream calls fail on
+    // this runtime), rounds are launched directly.
+    btg::engineOptions(s->pointEngine).graphs = false;
+    btg::engineOptions(s->fallback).graphs = false;
     btg::engineOptions(s->pointEngine).capacity = std::min<int64_t>(opt.capacity, queueCap);
     btg::engineOptions(s->fallback).capacity = std::min<int64_t>(opt.capacity, queueCap);
     s->softThread = std::thread(softLoop, s);

@@ -2774,7 +2774,13 @@ int upload(Engine& e, int slot, const T* src, size_t n, const T*& dst, std::stri
     return BTG_ERR_MEMORY_CODE;
   }
   e.sceneMem[slot] = d;
-  if (n) BTG_TRY(hipMemcpy(d, src, sizeof(T) * n, hipMemcpyHostToDevice));
+  // On the context's own (non-blocking) stream, never the legacy stream: on
+  // this runtime a legacy-stream operation fails while any stream of the
+  // process is capturing a graph (another context's, on another thread).
+  if (n) {
+    BTG_TRY(hipMemcpyAsync(d, src, sizeof(T) * n, hipMemcpyHostToDevice, e.stream));
+    BTG_TRY(hipStreamSynchronize(e.stream));
+  }
   dst = static_cast<const T*>(d);
   return 0;
 }

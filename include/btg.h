@@ -95,7 +95,11 @@ typedef struct {
   int useGraphs;      /* 1 (default): each group of roundsPerCheck rounds is captured once per pass into a HIP graph
                          and replayed (no counterpart in the reference, which launches every kernel from the host,
                          cuda/wavefront.cu:641-676); 0: plain launches. Results are the same either way.
-                         profileKernels = 1 launches directly (it times every launch with HIP events). */
+                         profileKernels = 1 launches directly (it times every launch with HIP events).
+                         While a pass captures (microseconds per group), operations that other threads of
+                         the process issue on the legacy (null) stream fail on this runtime; a program
+                         that uses the legacy stream from other threads meanwhile sets 0. The library
+                         itself never uses the legacy stream, and a server's passes never capture. */
 } btg_options;
 int btg_get_options(const btg_ctx* ctx, btg_options* opt);
 int btg_set_options(btg_ctx* ctx, const btg_options* opt);
