@@ -46,3 +46,21 @@ def test_create_rejects_bad_scene_without_touching_the_device():
     h = ctypes.c_void_p()
     assert L.btg_create(ctypes.byref(h), ctypes.byref(d)) < 0
     assert b"incomplete" in L.btg_last_error(None)
+
+
+def test_entry_points_reject_null_arguments_without_a_device():
+    """Argument checks come before any device work: null contexts, servers
+    and outputs are errors (negative codes), never crashes."""
+    from beamtrace2007_amd import gpu
+    L = gpu.lib()
+    q = gpu.btg_request()
+    q.cam = gpu.make_camera([0, 0, 0, 0, 0, 1, 1, 0, 0, 0, -1, 0, 0.5, 0.5], 8, 8)
+    b = ctypes.c_int64(0)
+    assert L.btg_footprint(None, ctypes.byref(q), ctypes.byref(b)) < 0
+    t = ctypes.c_uint64(0)
+    assert L.btg_server_submit(None, ctypes.byref(q), ctypes.byref(t)) < 0
+    st = gpu.btg_request_status()
+    assert L.btg_server_wait(None, 1, 0.0, ctypes.byref(st)) < 0
+    assert L.btg_server_error(None) == b""
+    L.btg_server_destroy(None)
+    assert L.btg_scene_bytes(None, 0) == -1
