@@ -1102,28 +1102,20 @@ struct PhaseClock {
 
 // Occupancy of the trace and split kernels: at least N waves per SIMD
 // (-DBTG_TRACE_WAVES_PER_EU=N, -DBTG_SPLIT_WAVES_PER_EU=N) caps their
-// registers at 512 / N. Two waves per SIMD keep the primary and point trace
-// kernels and the split kernel unspilled. The float soft-shadow trace kernel
-// runs four (128 registers, 55 values spilled to scratch): 1.4% faster on
-// the 64-job batch than three without spills (10.09 s against 10.24 s per
-// step); the guarded one stays at three.
+// registers at 512 / N. Two waves per SIMD keep the trace kernel in 224
+// registers with nothing spilled; the split kernel spills a little at two
+// waves but runs 1.6x faster than with one.
 #ifndef BTG_TRACE_WAVES_PER_EU
 #define BTG_TRACE_WAVES_PER_EU 2
 #endif
 #ifndef BTG_SOFT_TRACE_WAVES_PER_EU
-#define BTG_SOFT_TRACE_WAVES_PER_EU 4
-#endif
-#ifndef BTG_GUARD_TRACE_WAVES_PER_EU
-#define BTG_GUARD_TRACE_WAVES_PER_EU 3
+#define BTG_SOFT_TRACE_WAVES_PER_EU 3
 #endif
 #ifndef BTG_SPLIT_WAVES_PER_EU
 #define BTG_SPLIT_WAVES_PER_EU 2
 #endif
 #ifdef BTG_TRACE_WAVES_PER_EU
-#define TRACE_OCCUPANCY                                                                          \
-  __attribute__((amdgpu_waves_per_eu(App == kSoft    ? BTG_SOFT_TRACE_WAVES_PER_EU               \
-                                     : App == kSoftG ? BTG_GUARD_TRACE_WAVES_PER_EU              \
-                                                     : BTG_TRACE_WAVES_PER_EU)))
+#define TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(isSoft(App) ? BTG_SOFT_TRACE_WAVES_PER_EU : BTG_TRACE_WAVES_PER_EU)))
 #else
 #define TRACE_OCCUPANCY
 #endif
