@@ -203,7 +203,7 @@ def main() -> int:
         "config": {"workload": f"{args.workload} {W}x{H}: {desc}", "mode": "plain (one job per call)",
                    "tris": s.num_tris, "kd_nodes": len(s.kd_arrays()["nodes"]), "exact": True,
                    "budget": opts["budget"], "capacity": opts["capacity"], "leaf_cap": opts["leaf_cap"]},
-        "device": dev["name"], "arch": dev["arch"],
+        "device": device_label(dev), "arch": dev["arch"],
         "beams_per_s": round(st["beams"] * args.steps / wall, 1),
         "rounds": infos[-1]["rounds"], "killed_roots": infos[-1]["killed_roots"],
         "kernel_ms_per_step": kernel_ms, "alg_bytes_per_step": int(b_job),
@@ -215,6 +215,12 @@ def main() -> int:
     out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded))
     print(json.dumps(out), flush=True)
     return 0
+
+
+def device_label(dev: dict) -> str:
+    """The device's marketing name, or (where the runtime reports none) its
+    compute units, memory and architecture."""
+    return dev["name"] or f"{dev['compute_units']}-CU {dev['mem_gb']:.0f} GiB {dev['arch'].split(':')[0]}"
 
 
 def batch_capacity(mem_gb: float) -> int:
@@ -336,7 +342,7 @@ def bench_batch(args) -> int:
         "config": {"workload": f"batch64: {WORKLOADS['batch64'][2]}", "mode": "batched (64 jobs per call)",
                    "jobs": len(jobs), "pixels_per_step": px, "tris": {n: scenes[n].num_tris for n in scenes},
                    "draw_seed": 64, "scene_seed": 2007},
-        "device": dev["name"], "arch": dev["arch"],
+        "device": device_label(dev), "arch": dev["arch"],
         "schedule": dict(tuned, capacity=capacity),
         "plain": None if plain_s is None else {"value": round(px / plain_s, 1), "unit": "pixels/s",
                                                "seconds": round(plain_s, 3), "form": "one softShadows call per job"},
