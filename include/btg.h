@@ -273,7 +273,9 @@ int btg_images(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, const fl
 typedef struct btg_server btg_server;
 
 typedef struct {
-  int64_t hbmBytes;    /* device memory for work in flight, split between two passes (0: 32 GiB) */
+  int64_t hbmBytes;    /* device memory for work in flight, split between two passes (0: 128 GiB, or 45% of the
+                          device where that is less); a quarter of it holds the passes' queues, which set how
+                          many roots run at once */
   int roundsPerStep;   /* wavefront rounds between host checks (0: 8) */
   int jobsPerPass;     /* request slots of one pass (0: 512) */
 } btg_server_config;

@@ -14,6 +14,11 @@ from conftest import load_scene
 pytestmark = pytest.mark.gpu
 
 
+# The suite's other modules keep renderers (and their queues) alive: servers
+# here take 32 GiB rather than the default 128 GiB.
+SERVER_HBM = 32 << 30
+
+
 def scaled_light(light9, s):
     L = np.array(light9, np.float32).copy()
     L[3:] = L[3:] * np.float32(s)
@@ -46,7 +51,7 @@ def server_setup():
     from beamtrace2007_amd.gpu import GpuRenderer, Server
     r = GpuRenderer(load_scene("room")[0])
     plant = r.add_scene(load_scene("plant")[0])
-    srv = Server(r)
+    srv = Server(r, hbm_bytes=SERVER_HBM)
     yield srv, {"room": 0, "plant": plant}
     srv.close()
     r.close()
@@ -143,7 +148,7 @@ def test_guarded_requests_equal_guarded_alone(require_gpu):
     from beamtrace2007_amd.host import camera
     r = GpuRenderer(load_scene("room")[0])
     r.set_options(precision=1)
-    srv = Server(r)
+    srv = Server(r, hbm_bytes=SERVER_HBM)
     sp = load_scene("room")[3]
     plan = [(40, 30, 1.0), (17, 9, 0.5), (64, 48, 2.0)]
     tickets = []
