@@ -9,6 +9,7 @@
 #include <string>
 #include <system_error>
 #include <thread>
+#include <vector>
 #include <unordered_map>
 #include <vector>
 
@@ -159,6 +160,29 @@ bool validScene(const btg_scene_desc* d, std::string& why) {
     if (i > 0 && d->parent[i] >= uint64_t(d->numNodes)) {
       why = "kd parent out of range";
       return false;
+    }
+  }
+  // Children always follow their parent (left = i + 1, right > i + 1), so one
+  // forward pass gives every reachable node its depth. The restart trail holds
+  // one bit per inner level (64 bits): inner nodes deeper than 63 are refused.
+  // Each child's parent link must name the node that references it (the climb
+  // walks those links), which also rules out a node with two parents.
+  std::vector<int32_t> depth(size_t(d->numNodes), -1);
+  depth[0] = 0;
+  for (int64_t i = 0; i < d->numNodes; ++i) {
+    const uint32_t data = nodes[i].data;
+    if ((data & 3u) == 3u || depth[size_t(i)] < 0) continue;
+    if (depth[size_t(i)] > 63) {
+      why = "kd-tree deeper than 63 inner levels";
+      return false;
+    }
+    const uint32_t kids[2] = {uint32_t(i + 1), data >> 2};
+    for (uint32_t c : kids) {
+      if (d->parent[c] != uint32_t(i) || depth[c] >= 0) {
+        why = "kd parent links do not match the children";
+        return false;
+      }
+      depth[c] = depth[size_t(i)] + 1;
     }
   }
   return true;

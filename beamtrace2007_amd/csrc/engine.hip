@@ -33,7 +33,6 @@
 #include <chrono>
 #include <cstring>
 
-#include <rocprim/device/device_scan.hpp>
 
 #include "engine.h"
 
@@ -2476,7 +2475,7 @@ class Engine {
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
-  DevBuf rasterImg, rasterRows, rasterTmp;  // device rasterization: image, rows and offsets, scan scratch
+  DevBuf rasterImg, rasterRows;  // device rasterization: image; row counts, span offsets and tile sums
   WState* hostState = nullptr;  // pinned
   uint64_t phaseCycles[8] = {};
   // per-launch timing (Options::profile): event pairs and the kernel kind
@@ -2526,7 +2525,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &rasterTmp, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs};
+                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -3330,20 +3329,32 @@ int openTryAdd(OpenPass* op, const SoftJobH& h, int* slot, std::string& err) {
   J.tileBase = 0;  // the job's primary rays are launched on their own
   const int j = int(op->jobs.size());
   op->jobs.push_back(J);  // capacity reserved: the element stays put until the copy has read it
+  op->jobLeftHost[j] = int(roots);
+  // The job joins the pass (host lists, root and pixel ranges) only once its
+  // uploads and primary rays are queued: a failure here leaves the pass as it
+  // was, so it can still drain and end.
+  hipError_t he = hipMemcpyAsync(static_cast<SoftJob*>(e.jobs.ptr) + j, &op->jobs[size_t(j)], sizeof(SoftJob),
+                                 hipMemcpyHostToDevice, e.stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(static_cast<int*>(op->jobLeft.ptr) + j, &op->jobLeftHost[j], 4, hipMemcpyHostToDevice,
+                        e.stream);
+  if (he == hipSuccess) {
+    Params q = p;
+    q.jobs = static_cast<const SoftJob*>(e.jobs.ptr) + j;
+    q.numJobs = 1;
+    const int tiles = ((h.cam.W + 15) / 16) * ((h.cam.H + 15) / 16);
+    primaryRayKernel<<<unsigned(tiles), kBlock, 0, e.stream>>>(q);
+    he = hipGetLastError();
+  }
+  if (he != hipSuccess) {
+    op->jobs.pop_back();
+    op->jobLeftHost[j] = 0;
+    err = std::string("adding a job to an open pass: ") + hipGetErrorString(he);
+    return BTG_ERR_DEVICE_CODE;
+  }
   op->hostJobs.push_back(h);
   op->roots.push_back(int(roots));
   op->finished.push_back(0);
-  op->jobLeftHost[j] = int(roots);
-  BTG_TRY(hipMemcpyAsync(static_cast<SoftJob*>(e.jobs.ptr) + j, &op->jobs[size_t(j)], sizeof(SoftJob),
-                         hipMemcpyHostToDevice, e.stream));
-  BTG_TRY(hipMemcpyAsync(static_cast<int*>(op->jobLeft.ptr) + j, &op->jobLeftHost[j], 4, hipMemcpyHostToDevice,
-                         e.stream));
-  Params q = p;
-  q.jobs = static_cast<const SoftJob*>(e.jobs.ptr) + j;
-  q.numJobs = 1;
-  const int tiles = ((h.cam.W + 15) / 16) * ((h.cam.H + 15) / 16);
-  primaryRayKernel<<<unsigned(tiles), kBlock, 0, e.stream>>>(q);
-  BTG_TRY(hipGetLastError());
   p.R += int(roots);
   p.numJobs = j + 1;
   op->pixUsed += pix;
@@ -3588,35 +3599,114 @@ __device__ __forceinline__ void rasterEdges(const OutPoly& pg, float halfW, floa
   r.y1 = y1 < H - 1 ? y1 : H - 1;
 }
 
+// Per-polygon row counts are scanned into 64-bit span offsets in three
+// launches over tiles of kScanTile polygons (counts and tile sums, one block
+// scanning the tile sums, per-tile offsets). A polygon covers at most 32768
+// rows and a list at most 2^30 polygons, so 64-bit offsets cannot wrap.
+constexpr int kScanThreads = 256, kScanItems = 4, kScanTile = kScanThreads * kScanItems;
+
+// Exclusive scan of one value per thread across the block (wave64 shuffles,
+// then the waves' totals through LDS); *total receives the block's sum.
+// Every thread of the block must call it.
+__device__ __forceinline__ uint64_t blockExclusiveScan(uint64_t v, uint64_t* lds, uint64_t* total) {
+  const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6, waves = int(blockDim.x) >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) lds[wave] = x;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int w = 0; w < waves; ++w) {
+    const uint64_t s = lds[w];
+    before += w < wave ? s : 0;
+    all += s;
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
 // Rows a polygon covers (0 if it is filtered out): valid output of its pass
 // (tag == epoch of its root, when the epoch filter is given), a hit polygon
-// unless this is the shadow pass, 3 or 4 corners.
-__global__ void rasterRowsKernel(const OutPoly* polys, const int* root, const int* tag, const int* epoch, int n,
-                                 float halfW, float halfH, int W, int H, int shadow, uint32_t* rows) {
-  const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const OutPoly pg = polys[i];
-  bool ok = pg.n >= 3 && pg.n <= 4 && (shadow || pg.tri >= 0);
-  if (ok && epoch) ok = tag[i] == epoch[root[i]];
-  uint32_t c = 0;
-  if (ok) {
-    RasterEdges r;
-    rasterEdges(pg, halfW, halfH, W, H, r);
-    if (r.y1 >= r.y0) c = uint32_t(r.y1 - r.y0 + 1);
+// unless this is the shadow pass, 3 or 4 corners. Also each tile's row sum.
+__global__ void __launch_bounds__(kScanThreads) rasterRowsKernel(const OutPoly* polys, const int* root, const int* tag,
+                                                                 const int* epoch, int n, float halfW, float halfH,
+                                                                 int W, int H, int shadow, uint32_t* rows,
+                                                                 uint64_t* tileSum) {
+  __shared__ uint64_t lds[kScanThreads / 64];
+  const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    if (i >= n) break;
+    const OutPoly pg = polys[i];
+    bool ok = pg.n >= 3 && pg.n <= 4 && (shadow || pg.tri >= 0);
+    if (ok && epoch) ok = tag[i] == epoch[root[i]];
+    uint32_t c = 0;
+    if (ok) {
+      RasterEdges r;
+      rasterEdges(pg, halfW, halfH, W, H, r);
+      if (r.y1 >= r.y0) c = uint32_t(r.y1 - r.y0 + 1);
+    }
+    rows[i] = c;
+    s += c;
   }
-  rows[i] = c;
+  uint64_t total;
+  blockExclusiveScan(s, lds, &total);
+  if (threadIdx.x == 0) tileSum[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile sums in place by one block; tileSum[tiles]
+// receives the number of spans.
+__global__ void __launch_bounds__(1024) rasterTileScanKernel(uint64_t* tileSum, int tiles) {
+  __shared__ uint64_t lds[1024 / 64];
+  uint64_t carry = 0;
+  for (int c = 0; c < tiles; c += int(blockDim.x)) {
+    const int i = c + int(threadIdx.x);
+    const uint64_t v = i < tiles ? tileSum[i] : 0;
+    uint64_t total;
+    const uint64_t ex = blockExclusiveScan(v, lds, &total);
+    if (i < tiles) tileSum[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) tileSum[tiles] = carry;
+}
+
+// First span of each polygon: the tile's offset plus the scan within the tile.
+__global__ void __launch_bounds__(kScanThreads) rasterOffsKernel(const uint32_t* rows, const uint64_t* tileSum, int n,
+                                                                 uint64_t* offs) {
+  __shared__ uint64_t lds[kScanThreads / 64];
+  const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanItems;
+  uint32_t r[kScanItems];
+  uint64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    r[k] = base + k < n ? rows[base + k] : 0u;
+    s += r[k];
+  }
+  uint64_t total;
+  uint64_t ex = blockExclusiveScan(s, lds, &total) + tileSum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) offs[base + k] = ex;
+    ex += r[k];
+  }
 }
 
 // One (polygon, row) span per thread: the polygon is the last one whose first
 // span offset is <= the item (binary search over the exclusive scan).
-__global__ void rasterSpansKernel(const OutPoly* polys, const uint32_t* offs, int n, uint32_t total, float halfW,
+__global__ void rasterSpansKernel(const OutPoly* polys, const uint64_t* offs, int n, uint32_t total, float halfW,
                                   float halfH, int W, int H, int shadow, int32_t* ids, uint8_t* mask) {
   const uint32_t j = uint32_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= total) return;
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (offs[mid] <= j)
+    if (offs[mid] <= uint64_t(j))
       lo = mid;
     else
       hi = mid - 1;
@@ -3624,7 +3714,9 @@ __global__ void rasterSpansKernel(const OutPoly* polys, const uint32_t* offs, in
   const OutPoly pg = polys[lo];
   RasterEdges r;
   rasterEdges(pg, halfW, halfH, W, H, r);
-  const int y = r.y0 + int(j - offs[lo]);
+  const uint64_t k = uint64_t(j) - offs[lo];
+  if (r.y1 < r.y0 || k > uint64_t(r.y1 - r.y0)) return;  // not with a consistent scan: never leave the rows
+  const int y = r.y0 + int(k);
   const double sy = y + 0.5;
   double xl = -1e300, xr = 1e300;
   bool empty = false;
@@ -3671,49 +3763,65 @@ __global__ void compactPolys(const OutPoly* polys, const int* root, const int* t
   dst[atomicAdd(count, 1)] = polys[i];
 }
 
+// Device bytes of the scan scratch for n polygons: row counts (4 B each),
+// span offsets (8 B each) and tile sums (8 B per tile of kScanTile, one more).
+size_t rasterScratchBytes(int64_t n) {
+  const int64_t m = n > 0 ? n : 1;
+  const int64_t tiles = (m + kScanTile - 1) / kScanTile;
+  return size_t(((m * 4 + 7) & ~int64_t(7)) + m * 8 + (tiles + 1) * 8);
+}
+
 // Rasterizes n device polygons (epoch filter optional) into the device image
-// `img` (ids or mask, W x H) and copies it to the host.
+// `img` (ids or mask, W x H) on e.stream, with `scratch` (rasterScratchBytes(n)
+// bytes of device memory) for the scan; allocates nothing. Returns after the
+// span launch is queued (the span count is read back to size it).
+int rasterInto(Engine& e, const HostCam& cam, const OutPoly* polys, const int* root, const int* tag, const int* epoch,
+               int n, bool shadow, void* img, void* scratch, std::string& err) {
+  const size_t np = size_t(cam.W) * size_t(cam.H);
+  const int fillBlocks = int(std::min<size_t>((np + 255) / 256, 4096));
+  if (shadow)
+    fillByte<<<fillBlocks, 256, 0, e.stream>>>(static_cast<uint8_t*>(img), np);
+  else
+    fillInt<<<fillBlocks, 256, 0, e.stream>>>(static_cast<int32_t*>(img), np, -1);
+  BTG_TRY(hipGetLastError());
+  if (n <= 0) return 0;
+  const int tiles = (n + kScanTile - 1) / kScanTile;
+  uint32_t* rows = static_cast<uint32_t*>(scratch);
+  uint64_t* offs = reinterpret_cast<uint64_t*>(static_cast<char*>(scratch) + ((int64_t(n) * 4 + 7) & ~int64_t(7)));
+  uint64_t* tileSum = offs + n;
+  rasterRowsKernel<<<tiles, kScanThreads, 0, e.stream>>>(polys, root, tag, epoch, n, cam.halfW, cam.halfH, cam.W,
+                                                         cam.H, shadow ? 1 : 0, rows, tileSum);
+  BTG_TRY(hipGetLastError());
+  rasterTileScanKernel<<<1, 1024, 0, e.stream>>>(tileSum, tiles);
+  BTG_TRY(hipGetLastError());
+  rasterOffsKernel<<<tiles, kScanThreads, 0, e.stream>>>(rows, tileSum, n, offs);
+  BTG_TRY(hipGetLastError());
+  uint64_t total = 0;
+  BTG_TRY(hipMemcpyAsync(&total, tileSum + tiles, 8, hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipStreamSynchronize(e.stream));
+  if (total > 0xffffffffull) {
+    err = "too many polygon rows to rasterize (" + std::to_string(total) + " spans; at most 2^32 - 1)";
+    return BTG_ERR_LIMIT_CODE;
+  }
+  if (total > 0) {
+    rasterSpansKernel<<<unsigned((total + 255) / 256), 256, 0, e.stream>>>(
+        polys, offs, n, uint32_t(total), cam.halfW, cam.halfH, cam.W, cam.H, shadow ? 1 : 0,
+        static_cast<int32_t*>(img), static_cast<uint8_t*>(img));
+    BTG_TRY(hipGetLastError());
+  }
+  return 0;
+}
+
+// Rasterizes n device polygons (epoch filter optional) into the context's
+// device image and copies it to the host.
 int rasterDevice(Engine& e, const HostCam& cam, const OutPoly* polys, const int* root, const int* tag,
                  const int* epoch, int n, bool shadow, void* hostOut, std::string& err) {
   const size_t np = size_t(cam.W) * size_t(cam.H);
   int rc;
   if ((rc = e.ensure(e.rasterImg, np * (shadow ? 1 : 4), err)) ||
-      (rc = e.ensure(e.rasterRows, size_t(std::max(n, 1)) * 8 + 16, err)))
+      (rc = e.ensure(e.rasterRows, rasterScratchBytes(n), err)))
     return rc;
-  const int fillBlocks = int(std::min<size_t>((np + 255) / 256, 4096));
-  if (shadow)
-    fillByte<<<fillBlocks, 256, 0, e.stream>>>(static_cast<uint8_t*>(e.rasterImg.ptr), np);
-  else
-    fillInt<<<fillBlocks, 256, 0, e.stream>>>(static_cast<int32_t*>(e.rasterImg.ptr), np, -1);
-  BTG_TRY(hipGetLastError());
-  if (n > 0) {
-    uint32_t* rows = static_cast<uint32_t*>(e.rasterRows.ptr);
-    uint32_t* offs = rows + n;
-    rasterRowsKernel<<<(n + 255) / 256, 256, 0, e.stream>>>(polys, root, tag, epoch, n, cam.halfW, cam.halfH, cam.W,
-                                                            cam.H, shadow ? 1 : 0, rows);
-    BTG_TRY(hipGetLastError());
-    size_t tmpBytes = 0;
-    BTG_TRY(rocprim::exclusive_scan(nullptr, tmpBytes, rows, offs, uint32_t(0), size_t(n), rocprim::plus<uint32_t>(),
-                                    e.stream));
-    if ((rc = e.ensure(e.rasterTmp, tmpBytes, err))) return rc;
-    BTG_TRY(rocprim::exclusive_scan(e.rasterTmp.ptr, tmpBytes, rows, offs, uint32_t(0), size_t(n),
-                                    rocprim::plus<uint32_t>(), e.stream));
-    uint32_t last[2] = {0, 0};
-    BTG_TRY(hipMemcpyAsync(&last[0], rows + (n - 1), 4, hipMemcpyDeviceToHost, e.stream));
-    BTG_TRY(hipMemcpyAsync(&last[1], offs + (n - 1), 4, hipMemcpyDeviceToHost, e.stream));
-    BTG_TRY(hipStreamSynchronize(e.stream));
-    const uint64_t total = uint64_t(last[0]) + uint64_t(last[1]);
-    if (total > 0xffffffffull) {
-      err = "too many polygon rows to rasterize";
-      return BTG_ERR_LIMIT_CODE;
-    }
-    if (total > 0) {
-      rasterSpansKernel<<<unsigned((total + 255) / 256), 256, 0, e.stream>>>(
-          polys, offs, n, uint32_t(total), cam.halfW, cam.halfH, cam.W, cam.H, shadow ? 1 : 0,
-          static_cast<int32_t*>(e.rasterImg.ptr), static_cast<uint8_t*>(e.rasterImg.ptr));
-      BTG_TRY(hipGetLastError());
-    }
-  }
+  if ((rc = rasterInto(e, cam, polys, root, tag, epoch, n, shadow, e.rasterImg.ptr, e.rasterRows.ptr, err))) return rc;
   BTG_TRY(hipMemcpyAsync(hostOut, e.rasterImg.ptr, np * (shadow ? 1 : 4), hipMemcpyDeviceToHost, e.stream));
   BTG_TRY(hipStreamSynchronize(e.stream));
   return 0;

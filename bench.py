@@ -27,7 +27,8 @@ PCIe in the timed region). The line also reports:
                 against MI355X's 8 TB/s HBM3E;
   cpu_baseline  the reference's own CPU build (oracle/_ref, compiled from the
                 reference sources) on a bounded sample of the same work, all
-                host threads (up to 16), its default options (mailbox on).
+                host cores the process may use (affinity and CPU quota) and
+                on one thread, its default options (mailbox on).
 """
 from __future__ import annotations
 
@@ -392,7 +393,7 @@ def batch_precision(r, scenes, jobs, px) -> dict | None:
     r.set_options(**saved)
     errs = {0: [], 1: []}
     eerrs = {0: [], 1: []}
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, _ = host_cores()
     refs = {}
     for j, o0, o1 in zip(sample, res[0], res[1]):
         name = j["name"]
@@ -425,44 +426,97 @@ def batch_precision(r, scenes, jobs, px) -> dict | None:
     return out
 
 
-def cpu_batch_sample(scenes, jobs, draw, budget_s: float = 20.0) -> dict | None:
-    """The reference CPU build on a bounded sample of the batch: rows at
-    fixed fractions of each job's height (1/2, 1/4, 3/4, 1/8, ...), one
-    fraction across all 64 jobs at a time, until about budget_s seconds of
-    tracing."""
+def host_cores() -> tuple[int, dict]:
+    """Threads for the CPU baseline: every core this process may run on (its
+    affinity mask, further limited by a cgroup CPU quota where one is set),
+    with what the machine reports beside it."""
+    n_os = os.cpu_count() or 1
+    try:
+        n_aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n_aff = n_os
+    quota = None
+    for path in ("/sys/fs/cgroup/cpu.max", "/sys/fs/cgroup/cpu/cpu.cfs_quota_us"):
+        try:
+            with open(path) as f:
+                parts = f.read().split()
+            if path.endswith("cpu.max") and parts and parts[0] != "max":
+                quota = float(parts[0]) / float(parts[1])
+            elif path.endswith("quota_us") and parts and int(parts[0]) > 0:
+                with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as g:
+                    quota = int(parts[0]) / int(g.read().split()[0])
+            break
+        except (OSError, ValueError, IndexError, ZeroDivisionError):
+            continue
+    n = n_aff if quota is None else max(1, min(n_aff, int(quota + 0.999)))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, {"os_cpu_count": n_os, "affinity": n_aff, "cgroup_quota": quota, "model": model}
+
+
+def cpu_batch_sample(scenes, jobs, draw, budget_s: float = 20.0, single_s: float = 8.0) -> dict | None:
+    """The reference CPU build on a bounded sample of the batch, on every
+    host core and on one: each job traces every k-th row of its image (k set
+    so that a call has several rows per thread, rows spread over the image's
+    height), jobs in draw order until about budget_s seconds of tracing; the
+    one-thread figure traces a subset of the same rows (every 4th of the
+    sampled rows of the first jobs) for about single_s seconds. Rows are
+    handed to the reference's own row workers (src/render/render.cpp:386-421
+    hands out rows the same way)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from refpy import Ref, ref_available
 
     if not ref_available("f"):
         return None
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, cores = host_cores()
     refs = {}
-    px = rows = 0
-    secs = 0.0
-    fracs = [0.5, 0.25, 0.75, 0.125, 0.375, 0.625, 0.875]
-    for f in fracs:
+
+    def ref_of(name):
+        if name not in refs:
+            a = scenes[name].arrays()
+            refs[name] = Ref("f").load(a["pos"], a["tris"])
+        return refs[name]
+
+    def run(nthreads, rows_per_job, budget):
+        px = rows = 0
+        secs = 0.0
+        njobs = 0
         for j in jobs:
-            name = j["name"]
-            if name not in refs:
-                a = scenes[name].arrays()
-                refs[name] = Ref("f").load(a["pos"], a["tris"])
-            sp = scenes[name].spec()
-            y = min(j["H"] - 1, int(j["H"] * f))
-            res = refs[name].soft_rows(sp["view"], j["W"], j["H"], j["light"], y, y + 1, exact=True, mailbox=True,
-                                       trail=False, threads=threads)
+            sp = scenes[j["name"]].spec()
+            stride = max(1, j["H"] // rows_per_job)
+            y0 = stride // 2
+            res = ref_of(j["name"]).soft_rows(sp["view"], j["W"], j["H"], j["light"], y0, j["H"], exact=True,
+                                              mailbox=True, trail=False, threads=nthreads, stride=stride)
             secs += float(res["seconds"][0] + res["seconds"][1])
-            px += j["W"]
-            rows += 1
-            if secs >= budget_s:
+            n = res["vis"].shape[0]
+            rows += n
+            px += n * j["W"]
+            njobs += 1
+            if secs >= budget:
                 break
-        if secs >= budget_s:
-            break
+        return px, rows, secs, njobs
+
+    px, rows, secs, njobs = run(threads, max(8, 4 * threads), budget_s)
+    px1, rows1, secs1, njobs1 = run(1, 4, single_s)
     for rf in refs.values():
         rf.free()
-    print(f"bench.py: CPU reference sample {rows} rows, {secs:.1f} s", file=sys.stderr, flush=True)
-    return {"value": round(px / secs, 1), "unit": "pixels/s", "cores": threads, "kind": "reference",
-            "sample": f"{rows} rows at fixed height fractions across the 64 jobs ({px} px, {secs:.2f} s: primary rays "
-                      "+ shadow beams, mailbox on, frame stack)"}
+    allc, one = px / secs, px1 / secs1
+    print(f"bench.py: CPU reference {threads} threads: {rows} rows of {njobs} jobs, {secs:.1f} s, {allc:.0f} px/s; "
+          f"1 thread: {rows1} rows of {njobs1} jobs, {secs1:.1f} s, {one:.0f} px/s ({cores['model']})",
+          file=sys.stderr, flush=True)
+    return {"value": round(allc, 1), "unit": "pixels/s", "cores": threads, "kind": "reference",
+            "single_thread": {"value": round(one, 1), "rows": rows1, "jobs": njobs1, "seconds": round(secs1, 2)},
+            "speedup_over_one_thread": round(allc / one, 2), "host": cores,
+            "sample": f"about {max(8, 4 * threads)} evenly spaced rows of each of the first {njobs} jobs of the batch "
+                      f"({rows} rows, {px} px, {secs:.2f} s on {threads} threads: primary rays + shadow beams, "
+                      "mailbox on, frame stack, the reference's row workers)"}
 
 
 def bench_serve(args) -> dict | None:
@@ -550,7 +604,7 @@ def cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded=None) -> dict:
     stride = max(1, args.cpu_stride)
     y0, y1 = 0, H
     rows = (H + stride - 1) // stride
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, cores = host_cores()
     a = s.arrays()
     if ref_available("f"):
         ref = Ref("f").load(a["pos"], a["tris"])
@@ -558,7 +612,7 @@ def cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded=None) -> dict:
                             stride=stride)
         secs = float(cpu["seconds"][0] + cpu["seconds"][1])
         res["cpu_baseline"] = {"value": round(rows * W / secs, 1), "unit": "pixels/s", "cores": threads,
-                               "kind": "reference",
+                               "kind": "reference", "host": cores,
                                "sample": f"every {stride}th row of the same {W}x{H} frame ({rows * W} px, "
                                          f"{secs:.2f} s: primary rays + shadow beams, mailbox on, frame stack)"}
         ref.free()

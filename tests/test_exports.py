@@ -64,3 +64,72 @@ def test_entry_points_reject_null_arguments_without_a_device():
     assert L.btg_server_error(None) == b""
     L.btg_server_destroy(None)
     assert L.btg_scene_bytes(None, 0) == -1
+
+
+_keep = []
+
+
+def _chain_tree(inner_levels: int):
+    """A kd-tree whose every inner node has a leaf on its left and the next
+    inner node on its right (depth-first order), so its depth is
+    inner_levels; all leaves are empty."""
+    import numpy as np
+    from beamtrace2007_amd.host import Scene
+    n = 2 * inner_levels + 1
+    nodes = np.zeros(n, dtype=[("split", "<f4"), ("data", "<u4"), ("offset", "<u4")])
+    parent = np.full(n, 0xFFFFFFFF, np.uint32)
+    for lev in range(inner_levels):
+        i = 2 * lev
+        nodes[i] = (0.0, ((i + 2) << 2) | (lev % 3), 0)  # right child i + 2, axis lev % 3
+        nodes[i + 1] = (0.0, 3, 0)                       # empty leaf
+        parent[i + 1] = parent[i + 2] = i
+    nodes[n - 1] = (0.0, 3, 0)
+    s = Scene.generate("analytic")
+    _keep.append(s)  # the arrays view the scene's memory
+    return s.arrays(), nodes, parent
+
+
+def _create_with(arrays, nodes, parent):
+    import numpy as np
+    from beamtrace2007_amd import gpu
+    L = gpu.lib()
+    d = gpu.btg_scene_desc()
+    box = np.zeros((len(nodes), 6), np.float32)
+    idx = np.zeros(1, np.uint32)
+    d.pos, d.numVerts = arrays["pos"].ctypes.data, len(arrays["pos"])
+    d.tri3, d.triN, d.numTris = arrays["tris"].ctypes.data, arrays["triN"].ctypes.data, len(arrays["tris"])
+    d.nodes, d.numNodes = nodes.ctypes.data, len(nodes)
+    d.triIndices, d.numRefs = idx.ctypes.data, 0
+    d.parent, d.nodeBox = parent.ctypes.data, box.ctypes.data
+    h = ctypes.c_void_p()
+    rc = L.btg_create(ctypes.byref(h), ctypes.byref(d))
+    msg = L.btg_last_error(None)
+    if rc == 0:
+        L.btg_destroy(h)
+    return rc, msg
+
+
+def test_create_rejects_kd_trees_deeper_than_the_restart_trail():
+    """The restart trail keeps one bit per inner level in 64 bits: a tree with
+    64 inner levels is refused before any device work, one with 63 passes
+    the host checks (and then needs a device)."""
+    from beamtrace2007_amd import gpu
+    arrays, nodes, parent = _chain_tree(65)
+    rc, msg = _create_with(arrays, nodes, parent)
+    assert rc < 0 and b"deeper than 63" in msg
+    arrays, nodes, parent = _chain_tree(64)
+    rc, msg = _create_with(arrays, nodes, parent)
+    assert rc == 0 or b"deeper" not in msg
+    if gpu.device_info() is None:
+        assert rc < 0 and b"device" in msg
+
+
+def test_create_rejects_parent_links_that_do_not_match_the_children():
+    arrays, nodes, parent = _chain_tree(5)
+    parent[4] = 0  # node 4 is node 2's right child
+    rc, msg = _create_with(arrays, nodes, parent)
+    assert rc < 0 and b"parent links" in msg
+    arrays, nodes, parent = _chain_tree(5)
+    nodes[2]["data"] = (1 << 2) | 0  # node 2's right child points back at node 1 (not after its left child)
+    rc, msg = _create_with(arrays, nodes, parent)
+    assert rc < 0 and b"child index" in msg

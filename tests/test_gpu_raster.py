@@ -131,3 +131,24 @@ def test_building_frame_at_full_size(require_gpu):
     assert np.mean((mask == 0) | (ids >= 0)) >= 0.9999
     assert 0.01 < np.mean(mask) < 0.99
     r.close()
+
+
+def test_span_count_past_32_bits_is_refused(require_gpu):
+    """Row counts are scanned in 64 bits: 131073 polygons covering every row of
+    a 1 x 32768 image need 2^32 + 32768 spans, which is refused with an error
+    (a 32-bit scan would wrap to 32768 and pass); one polygon fewer than the
+    limit's worth of rows on a small image rasterizes normally."""
+    from beamtrace2007_amd.gpu import POLY_DTYPE
+    r = renderer("room")
+    cam = [0, 0, 0, 0, 0, -1, 1, 0, 0, 0, -1, 0, 0.5, 0.5]
+    n = 131073
+    polys = np.zeros(n, POLY_DTYPE)
+    polys["x"] = [-1.0, 1.0, 1.0, -1.0]
+    polys["y"] = [-1.0, -1.0, 1.0, 1.0]
+    polys["n"] = 4
+    polys["tri"] = np.arange(n)
+    print("raster 131073 full-height polygons on 1 x 32768", flush=True)
+    with pytest.raises(RuntimeError, match="too many polygon rows"):
+        r.raster(cam, 1, 32768, polys, shadow=False)
+    ids = r.raster(cam, 3, 5, polys[:4000], shadow=False)
+    assert np.all(ids == 3999)
