@@ -1416,6 +1416,14 @@ float orc_light_vis(const orc_scene* s, const float* x3, const float* ns3, int t
 }
 
 orc_polys* orc_primary(const orc_scene* s, const float* cam14, int W, int H, int cull, int tile, double* stats) {
+  return orc_primary_tiles(s, cam14, W, H, cull, tile, 0, tile > 0 ? (H + tile - 1) / tile : 0, stats);
+}
+
+// Tiled primary visibility restricted to the tile rows [ty0, ty1) (a strip of
+// a large frame; every tile root is traced alone, so a strip's polygons equal
+// the whole frame's polygons of those tiles).
+orc_polys* orc_primary_tiles(const orc_scene* s, const float* cam14, int W, int H, int cull, int tile, int ty0,
+                             int ty1, double* stats) {
   const Cam c = camOf(cam14, W, H);
   Tracer tr(*s);
   Query q;
@@ -1433,7 +1441,9 @@ orc_polys* orc_primary(const orc_scene* s, const float* cam14, int W, int H, int
     P->v = tr.out;
   } else {
     const int tilesX = (W + tile - 1) / tile, tilesY = (H + tile - 1) / tile;
-    for (int i = 0; i < tilesX * tilesY; ++i) {
+    ty0 = std::max(0, ty0);
+    ty1 = std::min(tilesY, ty1);
+    for (int i = ty0 * tilesX; i < ty1 * tilesX; ++i) {
       const int tx = i % tilesX, ty = i / tilesX;
       const int x0 = tx * tile, y0 = ty * tile, x1 = std::min(W, x0 + tile), y1 = std::min(H, y0 + tile);
       float a, b, cc, d;

@@ -51,6 +51,9 @@ void orc_soft(const orc_scene* s, const float* cam14, int W, int H, const float*
  * every output beam for the whole image. */
 typedef struct orc_polys orc_polys;
 orc_polys* orc_primary(const orc_scene* s, const float* cam14, int W, int H, int cull, int tile, double* stats);
+/* The same for the tiles of tile rows [ty0, ty1) only (a strip of a frame). */
+orc_polys* orc_primary_tiles(const orc_scene* s, const float* cam14, int W, int H, int cull, int tile, int ty0,
+                             int ty1, double* stats);
 /* Point-light shadows of the primary hit polygons (beamPointShadows,
  * src/render/render.cpp:209-247). */
 orc_polys* orc_point(const orc_scene* s, const float* cam14, int W, int H, const float* px4, const float* py4,

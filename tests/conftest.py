@@ -48,3 +48,28 @@ def require_gpu():
     if info is None:
         pytest.fail("no HIP device visible to libbtg.so")
     return info
+
+
+VIS_TOL = 1e-5   # absolute, visible fraction
+E_TOL = 1e-5     # relative, exact Lambert integral
+
+
+def check_soft(g, o, vis_tol: float = VIS_TOL, e_tol: float = E_TOL):
+    """GPU soft shadows against the oracle: primary hit ids and the
+    unoccluded irradiance bit-exact, visible fraction within vis_tol absolute
+    (in practice bit-exact: piece areas are summed exactly), exact irradiance
+    within e_tol relative (the double acos of device libm against glibc)."""
+    assert np.array_equal(g["tri"], o["tri"])
+    assert np.array_equal(g["Eu"], o["Eu"])
+    dv = np.abs(g["vis"].astype(np.float64) - o["vis"])
+    assert dv.max() <= vis_tol, f"max |dvis| {dv.max()}"
+    assert np.mean(g["vis"] == o["vis"]) > 0.9999
+    rel = np.abs(g["E"].astype(np.float64) - o["E"]) / np.maximum(np.abs(o["E"]), 1e-30)
+    assert np.all((rel <= e_tol) | (np.abs(o["E"]) < 1e-30)), f"max rel dE {rel.max()}"
+
+
+def canonical_equal(gpu_polys, oracle_polys) -> bool:
+    """Polygon lists equal bit for bit in canonical order."""
+    from beamtrace2007_amd.gpu import polys_to_dict
+    from refpy import canon
+    return bool(np.array_equal(canon(polys_to_dict(gpu_polys)), canon(oracle_polys)))

@@ -16,6 +16,12 @@ handed to the reference's Scene::finalize and KdTree::build):
   * point shadows of the tiled primary hits (digest, count, counters);
   * the rasterized hit ids and shadow mask of the tiled run.
 Only data is stored; nothing of the reference's source.
+
+`python tests/golden/make_golden.py large` writes tests/golden/large.npz for
+the benchmark scenes (blob, atrium, conference hall, the full 2M-triangle
+building): scene and kd-tree digests, soft shadows at 40 x 30 (tri, vis, E,
+Eu, counters), and tiled primary visibility (16 x 16 tiles) and point
+shadows at 64 x 48 (digests, counts, counters).
 """
 from __future__ import annotations
 
@@ -112,5 +118,51 @@ def main() -> None:
         print(f"{name}: {int(out['num_tris'])} tris, {os.path.getsize(path)} bytes", flush=True)
 
 
+LARGE = ["blob", "atrium", "conference", "building"]
+LARGE_SOFT = (40, 30)
+LARGE_PRIM = (64, 48)
+
+
+def large() -> None:
+    from beamtrace2007_amd.host import camera
+    out = {}
+    for name in LARGE:
+        s = Scene.generate(name)
+        a = s.arrays()
+        sp = s.spec()
+        ref = Ref("f").load(a["pos"], a["tris"])
+        ra = ref.scene_arrays()
+        rk = ref.kd_arrays()
+        out[f"{name}_num_tris"] = np.int64(len(ra["tris"]))
+        out[f"{name}_scene_digest"] = np.array(digest(ra["tris"], np.asarray(ra["triN"], np.float32),
+                                                      np.asarray(ra["triArea"], np.float32)))
+        out[f"{name}_kd_digest"] = np.array(kd_digest(rk["split"], rk["data"], rk["offset"], rk["triIndices"],
+                                                      rk["parent"], rk["nodeBox"]))
+        out[f"{name}_kd_nodes"] = np.int64(len(rk["split"]))
+        view, light9, point3 = sp["view"], sp["area_light"], sp["point_light"]
+        W, H = LARGE_SOFT
+        soft = ref.soft(view, W, H, light9, exact=True, mailbox=False, trail=True, threads=8)
+        for f in ("tri", "vis", "E", "Eu"):
+            out[f"{name}_soft_{f}"] = soft[f]
+        out[f"{name}_soft_stats"] = stats_vec(soft["stats"])
+        W, H = LARGE_PRIM
+        p = ref.primary(view, W, H, tile=16)
+        c = canon({k: p[k] for k in ("x", "y", "n", "tri", "area")})
+        out[f"{name}_prim16_digest"] = np.array(digest(c))
+        out[f"{name}_prim16_count"] = np.int64(len(c))
+        out[f"{name}_prim16_stats"] = stats_vec(p["stats"])
+        pt = ref.point(view, W, H, p, point3)
+        cp = canon({k: pt[k] for k in ("x", "y", "n", "tri", "area")})
+        out[f"{name}_point_digest"] = np.array(digest(cp))
+        out[f"{name}_point_count"] = np.int64(len(cp))
+        out[f"{name}_point_stats"] = stats_vec(pt["stats"])
+        ref.free()
+        print(f"{name}: {int(out[f'{name}_num_tris'])} tris, {int(out[f'{name}_kd_nodes'])} kd nodes, "
+              f"{len(c)} primary and {len(cp)} shadow polygons", flush=True)
+    path = os.path.join(HERE, "large.npz")
+    np.savez_compressed(path, **out)
+    print(f"large: {os.path.getsize(path)} bytes", flush=True)
+
+
 if __name__ == "__main__":
-    main()
+    large() if sys.argv[1:] == ["large"] else main()

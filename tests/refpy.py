@@ -229,6 +229,8 @@ class Oracle:
                                    dp, C.c_int]
             L.orc_primary.restype = vp
             L.orc_primary.argtypes = [vp, fp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
+            L.orc_primary_tiles.restype = vp
+            L.orc_primary_tiles.argtypes = [vp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]
             L.orc_point.restype = vp
             L.orc_point.argtypes = [vp, fp, C.c_int, C.c_int, fp, fp, i32p, i32p, fp, i64, fp, dp]
             L.orc_polys_count.restype = i64
@@ -278,9 +280,15 @@ class Oracle:
         self.L.orc_polys_free(h)
         return {"x": x, "y": y, "n": nn, "tri": tri, "area": area}
 
-    def primary(self, cam14, W, H, cull=False, tile=0):
+    def primary(self, cam14, W, H, cull=False, tile=0, tile_rows=None):
+        """tile_rows=(ty0, ty1): only the tiles of those tile rows (tile > 0)."""
         st = np.zeros(14)
-        p = self._polys(self.L.orc_primary(self.h, _f(cam14), W, H, int(cull), tile, st.ctypes.data_as(dp)))
+        if tile_rows is None:
+            h = self.L.orc_primary(self.h, _f(cam14), W, H, int(cull), tile, st.ctypes.data_as(dp))
+        else:
+            h = self.L.orc_primary_tiles(self.h, _f(cam14), W, H, int(cull), tile, int(tile_rows[0]),
+                                         int(tile_rows[1]), st.ctypes.data_as(dp))
+        p = self._polys(h)
         p["stats"] = stats_dict(st)
         return p
 

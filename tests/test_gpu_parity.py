@@ -9,15 +9,12 @@ order that depends on the traversal; Eu bit-exact.
 import numpy as np
 import pytest
 
-from conftest import load_scene
+from conftest import VIS_TOL, check_soft, load_scene
 from refpy import Oracle, canon
 
 pytestmark = pytest.mark.gpu
 
 SCENES = ["analytic", "room", "random_tris", "sphere", "plant"]
-VIS_TOL = 1e-5   # absolute, visible fraction
-E_TOL = 1e-5     # relative, exact integral
-
 _renderers, _oracles = {}, {}
 
 
@@ -34,16 +31,6 @@ def oracle(name):
         s, a, k, sp = load_scene(name)
         _oracles[name] = Oracle(a, k)
     return _oracles[name]
-
-
-def check_soft(g, o):
-    assert np.array_equal(g["tri"], o["tri"])
-    assert np.array_equal(g["Eu"], o["Eu"])
-    dv = np.abs(g["vis"].astype(np.float64) - o["vis"])
-    assert dv.max() <= VIS_TOL
-    assert np.mean(g["vis"] == o["vis"]) > 0.9999
-    rel = np.abs(g["E"].astype(np.float64) - o["E"]) / np.maximum(np.abs(o["E"]), 1e-30)
-    assert np.all((rel <= E_TOL) | (np.abs(o["E"]) < 1e-30))
 
 
 @pytest.mark.parametrize("name", SCENES)

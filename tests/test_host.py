@@ -114,3 +114,19 @@ def test_unknown_scene_is_an_error():
     from beamtrace2007_amd.host import Scene
     with pytest.raises(RuntimeError, match="unknown scene"):
         Scene.generate("nope")
+
+
+@pytest.mark.parametrize("name", ["blob", "atrium", "conference", "building"])
+def test_benchmark_scene_kdtrees_match_reference_digests(name):
+    """The benchmark scenes, the full 2M-triangle building included: finalized
+    scene and SAH kd-tree equal the reference build's (tests/golden/large.npz)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "large.npz"))
+    s, a, k, sp = load_scene(name)
+    assert a["tris"].shape[0] == int(g[f"{name}_num_tris"])
+    assert digest(a["tris"], a["triN"], a["triArea"]) == str(g[f"{name}_scene_digest"])
+    n = k["nodes"]
+    assert len(n) == int(g[f"{name}_kd_nodes"])
+    assert digest(n["split"], n["data"], n["offset"], k["triIndices"], k["parent"], k["nodeBox"]) == \
+        str(g[f"{name}_kd_digest"])
+    assert k["depth"] <= 63
