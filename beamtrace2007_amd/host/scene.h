@@ -76,10 +76,6 @@ struct KdTree {
   void build(const Scene& s, const KdParams& p = KdParams());
 };
 
-// Bounds of triangle t clipped to `box` in double, rounded outward, clamped to
-// `box` and to `whole` (kdtree.cpp:21-77).
-Box3 clipTriBounds(const Scene& s, uint32_t t, const Box3& box, const Box3& whole);
-
 struct View {
   V3 eye, target, up{0, 1, 0};
   float fovY = 60;
