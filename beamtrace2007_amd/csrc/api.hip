@@ -589,6 +589,16 @@ int btg_raster(btg_ctx* ctx, const btg_camera* cam, const btg_poly* polys, int64
   }
 }
 
+int btg_raster_device_noalloc(btg_ctx* ctx, const btg_camera* cam, const btg_poly* polysDev, int64_t count,
+                              int shadowPass, int32_t* idsDev, uint8_t* maskDev) {
+  if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");
+  if (!validCam(cam) || count < 0 || (count > 0 && !polysDev) || (shadowPass ? !maskDev : !idsDev))
+    return failWith(ctx, BTG_ERR_ARG, "invalid camera, polygons or output");
+  static_assert(BTG_RASTER_NOALLOC_MAX_POLYS == btg::kNoAllocMaxPolys, "allocation-free limit");
+  return btg::engineRasterNoAlloc(ctx->engine, camOf(*cam), reinterpret_cast<const btg::OutPoly*>(polysDev), count,
+                                  shadowPass != 0, idsDev, maskDev, ctx->err);
+}
+
 int btg_images(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, const float* light3, int32_t* ids,
                uint8_t* mask, btg_run_info* primInfo, btg_run_info* pointInfo, double* rasterSeconds) {
   if (!ctx) return failWith(nullptr, BTG_ERR_ARG, "null context");

@@ -94,6 +94,11 @@ class Engine;
 // shadow = true writes mask (1 where any polygon covers the centre).
 int engineRaster(Engine* e, const HostCam& cam, const OutPoly* polys, int64_t n, bool shadow, int32_t* ids,
                  uint8_t* mask, std::string& err);
+// btg_raster of device polygons into a device image with the per-device
+// static scratch (no allocation); n <= kNoAllocMaxPolys.
+int engineRasterNoAlloc(Engine* e, const HostCam& cam, const OutPoly* polysDev, int64_t n, bool shadow,
+                        int32_t* idsDev, uint8_t* maskDev, std::string& err);
+constexpr int64_t kNoAllocMaxPolys = int64_t(1) << 20;
 // Primary visibility and (light != null) point shadows of one frame with both
 // rasterizations on the device: the primary polygons feed the ID image and
 // the point pass without a host round trip. rasterMs: wall time of the two

@@ -32,9 +32,17 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
-
+#include <mutex>
 
 #include "engine.h"
+
+// Scan scratch of the allocation-free rasterization (btg_raster_device_noalloc):
+// row counts (4 B), span offsets (8 B) and tile sums for kNoAllocMaxPolys
+// polygons, 12 MiB + 8 KiB. Every device the library runs on holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives on that device (noAllocScratchOn).
+static __device__ __align__(256) unsigned char noAllocScratch[btg::kNoAllocMaxPolys * 12 +
+                                                                (btg::kNoAllocMaxPolys / 1024 + 1) * 8];
 
 namespace btg {
 
@@ -2476,6 +2484,7 @@ class Engine {
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
   DevBuf rasterImg, rasterRows;  // device rasterization: image; row counts, span offsets and tile sums
+  void* noAllocScratch = nullptr;  // this device's static rasterization scratch (null: unavailable)
   WState* hostState = nullptr;  // pinned
   uint64_t phaseCycles[8] = {};
   // per-launch timing (Options::profile): event pairs and the kernel kind
@@ -2900,6 +2909,34 @@ int rebuildArena(Engine& e, std::string& err) {
 
 }  // namespace
 
+namespace {
+
+// This device's address of noAllocScratch: looked up once per device, with
+// that device current, and kept for the process (null if the lookup failed).
+void* noAllocScratchOn(int dev) {
+  static std::mutex m;
+  static std::vector<void*> addr;
+  static std::vector<char> looked;
+  std::lock_guard<std::mutex> g(m);
+  if (dev < 0) return nullptr;
+  if (size_t(dev) >= addr.size()) {
+    addr.resize(size_t(dev) + 1, nullptr);
+    looked.resize(size_t(dev) + 1, 0);
+  }
+  if (!looked[size_t(dev)]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(noAllocScratch)) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+    }
+    addr[size_t(dev)] = p;
+    looked[size_t(dev)] = 1;
+  }
+  return addr[size_t(dev)];
+}
+
+}  // namespace
+
 Engine* engineCreate(const btg_scene_desc& d, float offset, float offsetGuard, std::string& err) {
   Engine* e = new Engine();
   int dev = 0;
@@ -2909,6 +2946,7 @@ Engine* engineCreate(const btg_scene_desc& d, float offset, float offsetGuard, s
     delete e;
     return nullptr;
   }
+  e->noAllocScratch = noAllocScratchOn(dev);
   // Queue capacity by device size: the largest power of two whose three
   // sub-beam queues and shade queue (3 x 64 + 40 B per record) fit in 11% of
   // device memory, 2^20..2^28 (2^27 on a 288 GB MI355X: 31 GB), so that a
@@ -3841,6 +3879,28 @@ int engineRaster(Engine* e, const HostCam& cam, const OutPoly* polys, int64_t n,
     BTG_TRY(hipMemcpyAsync(e->prim.ptr, polys, size_t(n) * sizeof(OutPoly), hipMemcpyHostToDevice, e->stream));
   return rasterDevice(*e, cam, static_cast<const OutPoly*>(e->prim.ptr), nullptr, nullptr, nullptr, int(n), shadow,
                       shadow ? static_cast<void*>(mask) : static_cast<void*>(ids), err);
+}
+
+int engineRasterNoAlloc(Engine* e, const HostCam& cam, const OutPoly* polysDev, int64_t n, bool shadow,
+                        int32_t* idsDev, uint8_t* maskDev, std::string& err) {
+  if (cam.W <= 0 || cam.H <= 0 || n < 0 || (n > 0 && !polysDev) || (shadow ? !maskDev : !idsDev)) {
+    err = "invalid raster arguments";
+    return BTG_ERR_ARG_CODE;
+  }
+  if (n > kNoAllocMaxPolys || rasterScratchBytes(n) > sizeof(noAllocScratch)) {
+    err = "allocation-free rasterization takes at most " + std::to_string(kNoAllocMaxPolys) + " polygons (got " +
+          std::to_string(n) + ")";
+    return BTG_ERR_LIMIT_CODE;
+  }
+  if (!e->noAllocScratch) {
+    err = "the static rasterization scratch has no address on this device";
+    return BTG_ERR_DEVICE_CODE;
+  }
+  const int rc = rasterInto(*e, cam, polysDev, nullptr, nullptr, nullptr, int(n), shadow,
+                            shadow ? static_cast<void*>(maskDev) : static_cast<void*>(idsDev), e->noAllocScratch, err);
+  if (rc) return rc;
+  BTG_TRY(hipStreamSynchronize(e->stream));
+  return 0;
 }
 
 int engineImages(Engine* e, const HostCam& cam, bool cull, int tile, const F3* light, int32_t* ids, uint8_t* mask,

@@ -94,7 +94,8 @@ assert POLY_DTYPE.itemsize == 44
 EXPORTS = ["btg_device_query", "btg_create", "btg_destroy", "btg_last_error", "btg_get_options", "btg_set_options",
            "btg_soft", "btg_scene_add", "btg_scene_remove", "btg_scene_bytes", "btg_soft_batch", "btg_autotune", "btg_server_create", "btg_server_submit", "btg_server_wait",
            "btg_server_error", "btg_server_destroy", "btg_polys_count",
-           "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point", "btg_raster", "btg_images",
+           "btg_polys_data", "btg_polys_free", "btg_primary", "btg_point", "btg_raster", "btg_raster_device_noalloc",
+           "btg_images",
            "btg_create_file", "btg_scene_add_file", "btg_footprint"]
 
 
@@ -139,6 +140,7 @@ def lib() -> C.CDLL:
         L.btg_point.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.POINTER(C.c_float), C.c_int,
                                 C.POINTER(vp), C.POINTER(btg_run_info)]
         L.btg_raster.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.c_int, vp, vp]
+        L.btg_raster_device_noalloc.argtypes = [vp, C.POINTER(btg_camera), vp, C.c_int64, C.c_int, vp, vp]
         L.btg_create_file.argtypes = [C.POINTER(vp), C.c_char_p]
         L.btg_scene_add_file.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
         L.btg_footprint.argtypes = [vp, C.POINTER(btg_request), C.POINTER(C.c_int64)]
@@ -388,6 +390,16 @@ class GpuRenderer:
         if rc:
             self._err("btg_raster", rc)
         return out
+
+    def raster_device_noalloc(self, cam14, W: int, H: int, polys_dev: int, count: int, out_dev: int,
+                              shadow: bool = False) -> None:
+        """btg_raster_device_noalloc: rasterizes `count` polygons at device address
+        polys_dev into the device image at out_dev (int32 ids, or uint8 mask when
+        shadow), allocating nothing (at most BTG_RASTER_NOALLOC_MAX_POLYS)."""
+        rc = lib().btg_raster_device_noalloc(self._h, C.byref(make_camera(cam14, W, H)), polys_dev or None, int(count),
+                                             int(shadow), None if shadow else out_dev, out_dev if shadow else None)
+        if rc:
+            self._err("btg_raster_device_noalloc", rc)
 
     def images(self, cam14, W: int, H: int, light3=None, cull: bool = False, tile: int = 16):
         """Primary IDs and (light3 given) the point-shadow mask of one frame, both

@@ -244,6 +244,21 @@ int btg_point(btg_ctx* ctx, const btg_camera* cam, const btg_poly* prim, int64_t
 int btg_raster(btg_ctx* ctx, const btg_camera* cam, const btg_poly* polys, int64_t count, int shadowPass,
                int32_t* ids, uint8_t* mask);
 
+/* btg_raster over device memory with no allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * polysDev: `count` btg_poly in device memory; idsDev (shadowPass = 0, W*H
+ * int32) or maskDev (shadowPass = 1, W*H bytes) in device memory, written
+ * with exactly btg_raster's result. The scan scratch is a static device array
+ * sized for BTG_RASTER_NOALLOC_MAX_POLYS polygons (12 MiB), one per device,
+ * held until the process ends; count above that maximum is BTG_ERR_LIMIT. All
+ * calls on one device share that array: calls that overlap on one device race
+ * (serialise them). Synchronous on the context's stream; device buffers the
+ * caller wrote on other streams must be complete before the call. */
+#define BTG_RASTER_NOALLOC_MAX_POLYS (1 << 20)
+int btg_raster_device_noalloc(btg_ctx* ctx, const btg_camera* cam, const btg_poly* polysDev, int64_t count,
+                              int shadowPass, int32_t* idsDev, uint8_t* maskDev);
+
 /* One frame of primary visibility and (light3 != NULL) point shadows with
  * both rasterizations on the device (bt_render's primary / point output,
  * apps/bt_render.cpp:228-247, without the CPU post-pass): ids[W*H] as
