@@ -89,8 +89,12 @@ def parse():
                     help="sub-beam records per queue (0: batch64 sizes it from device memory, others use the "
                          "library default)")
     ap.add_argument("--leaf-cap", type=int, default=-1, help="leaves per sub-beam per trace launch (-1: default)")
-    ap.add_argument("--qps", type=float, default=3.0, help="serve: request arrival rate")
-    ap.add_argument("--requests", type=int, default=60, help="serve: requests in the stream")
+    ap.add_argument("--qps", default="3,0",
+                    help="serve: request arrival rates, comma-separated (0: every request at once, the saturation "
+                         "capacity)")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests in the list")
+    ap.add_argument("--no-plain-serve", dest="plain_serve", action="store_false",
+                    help="serve: skip the plain form on the same request list")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (the known-size copy that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and print their size")
@@ -520,8 +524,18 @@ def cpu_batch_sample(scenes, jobs, draw, budget_s: float = 20.0, single_s: float
 
 
 def bench_serve(args) -> dict | None:
-    """A Poisson stream of mixed requests through the request server: p50/p99
-    latency (submit -> results in host memory) and pixels/s over the stream."""
+    """The served mixed stream on the conference hall through the request
+    server, and the plain form on the same request list.
+
+    One seeded request list (args.requests requests: 70% exact soft shadows,
+    30% point shadows, sides uniform 256..1024) is served at each rate of
+    args.qps: a rate > 0 is a Poisson arrival stream at that rate (latency
+    submit -> results in host memory, p50/p99), 0 submits every request at
+    once (saturation: the server's capacity in pixels/s). The plain form then
+    renders the same list one call at a time on the same context (soft:
+    softShadows with the images downloaded; point: primary + point shadows
+    with the polygon lists downloaded, as a served point request returns
+    them)."""
     import threading
 
     from beamtrace2007_amd import gpu
@@ -535,61 +549,103 @@ def bench_serve(args) -> dict | None:
     sc.build_kd(0.4)
     sp = sc.spec()
     r = gpu.GpuRenderer(sc)
-    # warm-up outside the stream: one small request of each kind
-    srv = gpu.Server(r)
-    for t in (srv.submit_soft(0, camera(sp["view"], 64, 64), 64, 64, sp["area_light"]),
-              srv.submit_point(camera(sp["view"], 64, 64), 64, 64, sp["point_light"])):
-        srv.wait(t)
-    draw = served_draw(args.requests, args.qps, seed=5)
-    tickets = [None] * len(draw)
-    t0 = time.perf_counter()
-
-    def submitter():
-        for i, d in enumerate(draw):
-            delay = t0 + d["t"] - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            cam = camera(sp["view"], d["side"], d["side"])
-            if d["kind"] == "soft":
-                tickets[i] = srv.submit_soft(0, cam, d["side"], d["side"], sp["area_light"])
-            else:
-                tickets[i] = srv.submit_point(cam, d["side"], d["side"], sp["point_light"])
-
-    th = threading.Thread(target=submitter)
-    th.start()
-    lat = {"soft": [], "point": []}
-    queue = []
-    last_report = time.perf_counter()
-    for i, d in enumerate(draw):
-        while tickets[i] is None:
-            time.sleep(0.001)
-        res = srv.wait(tickets[i])
-        if res["status"] != 0:
-            print(f"bench.py: request {i} failed: {res.get('error')}", file=sys.stderr)
-            srv.close()
-            return None
-        lat[d["kind"]].append(res["latency_s"])
-        queue.append(res["queue_s"])
-        if time.perf_counter() - last_report > 10:
-            print(f"bench.py: served {i + 1}/{len(draw)}", file=sys.stderr, flush=True)
-            last_report = time.perf_counter()
-    th.join()
-    span = time.perf_counter() - t0
-    srv.close()
-    r.close()
-    allv = np.array(lat["soft"] + lat["point"])
-    px = sum(d["side"] ** 2 for d in draw)
+    rates = [float(q) for q in str(args.qps).split(",") if q.strip()]
+    base = served_draw(args.requests, 1.0, seed=5)  # kinds and sides do not depend on the rate
+    px_of = [d["side"] ** 2 for d in base]
+    total_px = sum(px_of)
 
     def pct(v, q):
         return round(float(np.percentile(np.array(v), q)) * 1e3, 2) if len(v) else None
 
-    return {"requests": len(draw), "qps": args.qps, "arrival_seed": 5, "seconds": round(span, 3),
-            "pixels": px, "pixels_per_s": round(px / span, 1),
-            "latency_ms": {"p50": pct(allv, 50), "p99": pct(allv, 99), "soft_p50": pct(lat["soft"], 50),
-                           "soft_p99": pct(lat["soft"], 99), "point_p50": pct(lat["point"], 50),
-                           "point_p99": pct(lat["point"], 99)},
-            "queue_ms": {"p50": pct(queue, 50), "p99": pct(queue, 99)},
-            "mix": {"soft": len(lat["soft"]), "point": len(lat["point"])},
+    def serve(rate):
+        srv = gpu.Server(r)
+        for t in (srv.submit_soft(0, camera(sp["view"], 64, 64), 64, 64, sp["area_light"]),
+                  srv.submit_point(camera(sp["view"], 64, 64), 64, 64, sp["point_light"])):
+            srv.wait(t)  # warm-up outside the measurement
+        arrivals = [d["t"] / rate for d in base] if rate > 0 else [0.0] * len(base)
+        tickets = [None] * len(base)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, d in enumerate(base):
+                delay = t0 + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                cam = camera(sp["view"], d["side"], d["side"])
+                if d["kind"] == "soft":
+                    tickets[i] = srv.submit_soft(0, cam, d["side"], d["side"], sp["area_light"])
+                else:
+                    tickets[i] = srv.submit_point(cam, d["side"], d["side"], sp["point_light"])
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat = {"soft": [], "point": []}
+        queue = []
+        last = time.perf_counter()
+        for i, d in enumerate(base):
+            while tickets[i] is None:
+                time.sleep(0.0005)
+            res = srv.wait(tickets[i])
+            if res["status"] != 0:
+                print(f"bench.py: request {i} failed: {res.get('error')}", file=sys.stderr)
+                th.join()
+                srv.close()
+                return None
+            lat[d["kind"]].append(res["latency_s"])
+            queue.append(res["queue_s"])
+            if time.perf_counter() - last > 10:
+                print(f"bench.py: rate {rate:g}: served {i + 1}/{len(base)}", file=sys.stderr, flush=True)
+                last = time.perf_counter()
+        th.join()
+        span = time.perf_counter() - t0
+        srv.close()
+        allv = lat["soft"] + lat["point"]
+        print(f"bench.py: rate {rate:g}: {len(base)} requests in {span:.1f} s, {total_px / span:.0f} px/s, p50 "
+              f"{pct(allv, 50)} ms, p99 {pct(allv, 99)} ms", file=sys.stderr, flush=True)
+        return {"qps": rate if rate > 0 else "saturation", "seconds": round(span, 3),
+                "pixels_per_s": round(total_px / span, 1),
+                "offered_pixels_per_s": round(total_px / arrivals[-1], 1) if rate > 0 else None,
+                "latency_ms": {"p50": pct(allv, 50), "p99": pct(allv, 99), "soft_p50": pct(lat["soft"], 50),
+                               "soft_p99": pct(lat["soft"], 99), "point_p50": pct(lat["point"], 50),
+                               "point_p99": pct(lat["point"], 99)},
+                "queue_ms": {"p50": pct(queue, 50), "p99": pct(queue, 99)}}
+
+    points = []
+    for rate in rates:
+        p = serve(rate)
+        if p is None:
+            r.close()
+            return None
+        points.append(p)
+    plain = None
+    if args.plain_serve:
+        for d in base[:2]:  # warm-up
+            cam = camera(sp["view"], 64, 64)
+            r.soft_shadows(cam, 64, 64, sp["area_light"], exact=True)
+        lat = []
+        t0 = time.perf_counter()
+        for d in base:
+            t1 = time.perf_counter()
+            cam = camera(sp["view"], d["side"], d["side"])
+            if d["kind"] == "soft":
+                r.soft_shadows(cam, d["side"], d["side"], sp["area_light"], exact=True, download=True)
+            else:
+                hits, _ = r.primary(cam, d["side"], d["side"], tile=16)
+                r.point_shadows(cam, d["side"], d["side"], hits, sp["point_light"])
+            lat.append(time.perf_counter() - t1)
+        span = time.perf_counter() - t0
+        print(f"bench.py: plain form on the same requests {span:.1f} s, {total_px / span:.0f} px/s",
+              file=sys.stderr, flush=True)
+        plain = {"seconds": round(span, 3), "pixels_per_s": round(total_px / span, 1),
+                 "call_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                 "form": "one call per request, in arrival order (soft: softShadows, images downloaded; point: "
+                         "primary + pointShadows, polygon lists downloaded)"}
+    r.close()
+    sat = next((p for p in points if p["qps"] == "saturation"), None)
+    return {"requests": len(base), "arrival_seed": 5, "pixels": total_px,
+            "mix": {"soft": sum(d["kind"] == "soft" for d in base), "point": sum(d["kind"] == "point" for d in base)},
+            "points": points, "plain": plain,
+            "served_over_plain": round(sat["pixels_per_s"] / plain["pixels_per_s"], 3) if sat and plain else None,
             "config": WORKLOADS["serve"][2]}
 
 
