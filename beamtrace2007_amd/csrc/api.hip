@@ -73,6 +73,7 @@ void fillInfo(const btg::RunInfo& r, double wall, btg_run_info* info) {
     t.fiveSplits = s[btg::sFive];
     t.climbs = s[btg::sClimbs];
     t.rays = s[btg::sRays];
+    t.mailboxSkips = s[btg::sMail];
   };
   put(r.stats, info->stats);
   info->stats.droppedArea = r.dropped;
@@ -352,6 +353,7 @@ int btg_get_options(const btg_ctx* ctx, btg_options* o) {
   o->leafCap = x.leafCap;
   o->precision = x.precision;
   o->useGraphs = x.graphs ? 1 : 0;
+  o->mailbox = x.mailbox ? 1 : 0;
   return BTG_OK;
 }
 
@@ -359,7 +361,7 @@ int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   if (!ctx || !o) return BTG_ERR_ARG;
   if (o->budget < 1 || o->primaryBudget < 1 || o->capacity < 1024 || o->capacity > (int64_t(1) << 31) ||
       o->roundsPerCheck < 1 || o->leafCap < 0 || o->precision < 0 || o->precision > 1 || o->useGraphs < 0 ||
-      o->useGraphs > 1)
+      o->useGraphs > 1 || o->mailbox < 0 || o->mailbox > 1)
     return failWith(ctx, BTG_ERR_ARG, "invalid options");
   btg::Options& x = btg::engineOptions(ctx->engine);
   x.budget = o->budget;
@@ -370,6 +372,7 @@ int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   x.leafCap = o->leafCap;
   x.precision = o->precision;
   x.graphs = o->useGraphs != 0;
+  x.mailbox = o->mailbox != 0;
   return BTG_OK;
 }
 

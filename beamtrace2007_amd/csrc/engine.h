@@ -39,13 +39,14 @@ struct DevScene {
   const Box* nodeBox = nullptr;
   const uint32_t* parent = nullptr;
   const LeafTri* refs = nullptr;   // in leaf order (kd triIndices order)
+  const int32_t* refTri = nullptr; // the triangle id of each leaf reference (the mailbox's 4-byte read)
   Box bounds;                      // kd root box
   int numTris = 0;
 };
 
 // Per-call run statistics (counters of src/util/stats.h:10-42 that the
 // engine produces).
-enum StatId { sKd, sLeaf, sTests, sHits, sSplits, sBeams, sRoots, sPresplit, sFive, sClimbs, sRays, sNum };
+enum StatId { sKd, sLeaf, sTests, sHits, sSplits, sBeams, sRoots, sPresplit, sFive, sClimbs, sRays, sMail, sNum };
 
 struct RunInfo {
   double primaryMs = 0, traceMs = 0;
@@ -65,6 +66,7 @@ struct Options {
   int roundsPerCheck = 8;
   int leafCap = 8;
   int precision = 0;  // soft shadows: 0 float build (bit-exact), 1 FP64-guarded
+  bool mailbox = false;  // skip exact repeats of a triangle test by one sub-beam (results unchanged)
   bool profile = false;
   bool graphs = true;  // replay each group of rounds from a captured HIP graph
 };

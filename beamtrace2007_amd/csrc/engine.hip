@@ -51,6 +51,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kPrimaryLeafCap = 4;  // leaf cap of primary passes (at most; see prepare)
+constexpr int kMbox = 8;            // mailbox entries per lane (a power of two)
 
 // kSoftG: soft shadows in the FP64-guarded mode (double hit point, the
 // double build's apex offset, the apex held as a float pair).
@@ -179,6 +180,7 @@ struct Params {
   float* Eu;
   int tile, tilesW, cull;
   int leafCap;  // leaves a sub-beam may visit per trace launch before it yields (0: no cap)
+  int mailbox;  // 1: per-lane mailbox of the triangles the lane's sub-beam has passed unchanged
   F3 pointLight;
   const OutPoly* prim;
   OutPoly* out;
@@ -1307,9 +1309,18 @@ __device__ __forceinline__ C4 shflC4(const C4& v, int src) {
 // to the triangle's plane (setStatus) and the owner returns kContinue with
 // the cursor past it, so the rest of the leaf is tested against the new hit,
 // again as the sequential loop does.
+// Mailbox (Params::mailbox): each lane keeps, in its column of `mboxLds`
+// (kMbox direct-mapped entries, stride kBlock), triangles its current sub-beam
+// has been tested against with the result Keep. A Keep leaves the beam
+// unchanged, so testing the same triangle again (it is listed in several
+// leaves when it straddles a split plane) would give Keep again: such a pair is
+// skipped without loading the triangle, counted as a mailbox skip instead of a
+// test (as src/beam/beam_core.h:884-912 counts its skips). The column is
+// cleared whenever the lane's beam changes: a new record, or (primary) a
+// nearer hit. Results are the same with and without it.
 template <int App>
 __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const Stepper& ss, bool want,
-                        Cnt& cnt) {
+                        Cnt& cnt, int32_t* mboxLds) {
   const int k = want ? int(ss.leafCount) - r.cursor : 0;
   int incl = k;
 #pragma unroll
@@ -1338,6 +1349,8 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
     }
     const int owner = lo;
     const int oExcl = __shfl(excl, owner, kWave), oCursor = __shfl(r.cursor, owner, kWave);
+    const int oIncl = __shfl(incl, owner, kWave);
+    int32_t* const ocol = mboxLds + ((int(threadIdx.x) & ~(kWave - 1)) + owner);
     const int oOff = __shfl(int(ss.leafOffset), owner, kWave);
     const bool live = t < total && ((open >> owner) & 1ull);
     Ctx oc;
@@ -1369,7 +1382,13 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
     int status = 0, tri = -1;  // 0 skipped (uncounted), 1 Keep, 2 AllIn, 3 Clip
     float tA0 = 0, tA1 = 0, tA2 = 0, tC = 0;  // an AllIn triangle's plane (primary)
     F3 tN = f3(0, 0, 0);
-    if (live) {
+    bool boxed = false;
+    if (live && p.mailbox) {
+      tri = p.sv.refTri[uint32_t(oOff) + uint32_t(oCursor + (t - oExcl))];
+      boxed = ocol[(tri & (kMbox - 1)) * kBlock] == tri;
+      if (boxed) cnt.v[sMail]++;
+    }
+    if (live && !boxed) {
       const uint32_t idx = uint32_t(oCursor + (t - oExcl));
       const LeafTri ref = p.sv.refs[uint32_t(oOff) + idx];
       tri = ref.tri;
@@ -1394,6 +1413,13 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
       }
     }
     const uint64_t hitB = __ballot(status >= 2), cntB = __ballot(status >= 1), allInB = __ballot(status == 2);
+    if (p.mailbox && status == 1) {  // a Keep before the owner's first other result: the beam passed it unchanged
+      const int a = oExcl - base > 0 ? oExcl - base : 0;
+      const int b = oIncl - base < kWave ? oIncl - base : kWave;
+      const uint64_t below = b == kWave ? ~0ull : ((1ull << b) - 1ull);
+      const uint64_t hm = hitB & below & ~((1ull << a) - 1ull);
+      if (hm == 0 || lane() < __builtin_ctzll(hm)) ocol[(tri & (kMbox - 1)) * kBlock] = tri;
+    }
     int src = lane();
     bool took = false;
     if (!resolved) {
@@ -1447,6 +1473,11 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
         w.hitN = hN;
         r.hit = firstTri;
         r.cursor = first + 1;
+        if (p.mailbox) {  // a nearer hit: earlier Keeps were against the farther one
+          int32_t* const own = mboxLds + threadIdx.x;
+#pragma unroll
+          for (int k = 0; k < kMbox; ++k) own[k * kBlock] = -1;
+        }
         return kContinue;
       }
       emitOutput<App>(p, c, w.b, r.root, firstTri, cnt);
@@ -1474,6 +1505,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
   Ctx c;
   Stepper ss;
   __shared__ uint32_t pathLds[kPathLevels * kBlock];
+  __shared__ int32_t mboxLds[kMbox * kBlock];
   const Path path{pathLds + threadIdx.x};
   for (;;) {
     const uint64_t idle = __ballot(!active);
@@ -1505,6 +1537,10 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
             ss.validFrom = r.depth;
             active = true;
             leaves = 0;
+            if (p.mailbox) {
+#pragma unroll
+              for (int k = 0; k < kMbox; ++k) mboxLds[k * kBlock + threadIdx.x] = -1;
+            }
           }
         }
       }
@@ -1519,7 +1555,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
     PHASE_MARK(kPhRest);
     if (kCoopTest) {
       const bool want = active && res == kContinue;  // a leaf with triangles pending
-      const int cr = coopTest<App>(p, c, r, w, ss, want, cnt);
+      const int cr = coopTest<App>(p, c, r, w, ss, want, cnt, mboxLds);
       if (want) res = cr;
     } else if (active) {
       while (res == kContinue) {
@@ -2479,7 +2515,7 @@ class Engine {
   hipEvent_t ev[4] = {};
   int cus = 0;
   // scene storage
-  void* sceneMem[5] = {};
+  void* sceneMem[6] = {};
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
@@ -2599,6 +2635,7 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
   // long as their slowest lane: they yield after at most kPrimaryLeafCap
   // leaves (measured on the conference hall: 45 ms against 50 ms at 8).
   p.leafCap = app == kPrimary && e.opt.leafCap > 0 ? std::min(e.opt.leafCap, kPrimaryLeafCap) : e.opt.leafCap;
+  p.mailbox = e.opt.mailbox ? 1 : 0;
   return 0;
 }
 
@@ -2865,11 +2902,13 @@ int rebuildArena(Engine& e, std::string& err) {
   std::vector<Box> bx;
   std::vector<uint32_t> par;
   std::vector<LeafTri> lt;
+  std::vector<int32_t> rt;
   nv.reserve(size_t(tris) * 2);
   kn.reserve(nodes);
   bx.reserve(nodes);
   par.reserve(nodes);
   lt.reserve(refs);
+  rt.reserve(refs);
   for (const SceneRec& sr : e.scenes) {
     if (!sr.live) continue;
     nv.insert(nv.end(), sr.triNV.begin(), sr.triNV.end());
@@ -2887,6 +2926,7 @@ int rebuildArena(Engine& e, std::string& err) {
       r.tri += int32_t(sr.triBase);
       for (int k = 0; k < 3; ++k) r.vi[k] += sr.vertBase;
       lt.push_back(r);
+      rt.push_back(r.tri);
     }
   }
   for (void*& m : e.sceneMem)
@@ -2900,6 +2940,7 @@ int rebuildArena(Engine& e, std::string& err) {
   if (!rc) rc = upload(e, 2, bx.data(), bx.size(), sv.nodeBox, err);
   if (!rc) rc = upload(e, 3, par.data(), par.size(), sv.parent, err);
   if (!rc) rc = upload(e, 4, lt.data(), lt.size(), sv.refs, err);
+  if (!rc) rc = upload(e, 5, rt.data(), rt.size(), sv.refTri, err);
   if (rc) return rc;
   const SceneRec& s0 = e.scenes[0];
   sv.bounds = s0.bounds;
@@ -3016,7 +3057,7 @@ int64_t engineSceneBytes(const Engine* e, int scene) {
   if (scene < 0 || size_t(scene) >= e->scenes.size() || !e->scenes[size_t(scene)].live) return -1;
   const SceneRec& sr = e->scenes[size_t(scene)];
   return int64_t(sr.triNV.size() * sizeof(float4) + sr.nodes.size() * (sizeof(KdNodeD) + sizeof(Box) + 4) +
-                 sr.refs.size() * sizeof(LeafTri));
+                 sr.refs.size() * (sizeof(LeafTri) + sizeof(int32_t)));
 }
 
 // roots: RootQ, live, state, epoch, missFx, gFx (as prepare allocates them)

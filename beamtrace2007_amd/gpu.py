@@ -36,7 +36,7 @@ class btg_scene_desc(C.Structure):
 class btg_options(C.Structure):
     _fields_ = [("budget", C.c_int), ("primaryBudget", C.c_int), ("capacity", C.c_int64),
                 ("roundsPerCheck", C.c_int), ("profileKernels", C.c_int), ("leafCap", C.c_int),
-                ("precision", C.c_int), ("useGraphs", C.c_int)]
+                ("precision", C.c_int), ("useGraphs", C.c_int), ("mailbox", C.c_int)]
 
 
 class btg_camera(C.Structure):
@@ -321,13 +321,14 @@ class GpuRenderer:
         lib().btg_get_options(self._h, C.byref(o))
         return {"budget": o.budget, "primary_budget": o.primaryBudget, "capacity": o.capacity,
                 "rounds_per_check": o.roundsPerCheck, "profile": bool(o.profileKernels), "leaf_cap": o.leafCap,
-                "precision": o.precision, "graphs": bool(o.useGraphs)}
+                "precision": o.precision, "graphs": bool(o.useGraphs), "mailbox": bool(o.mailbox)}
 
     def set_options(self, **kw) -> None:
         cur = self.options
         cur.update(kw)
         o = btg_options(cur["budget"], cur["primary_budget"], cur["capacity"], cur["rounds_per_check"],
-                        int(cur["profile"]), int(cur["leaf_cap"]), int(cur["precision"]), int(cur["graphs"]))
+                        int(cur["profile"]), int(cur["leaf_cap"]), int(cur["precision"]), int(cur["graphs"]),
+                        int(cur["mailbox"]))
         rc = lib().btg_set_options(self._h, C.byref(o))
         if rc:
             self._err("btg_set_options", rc)

@@ -15,7 +15,9 @@
 // Differences: roots over budget are traced again on the GPU (no CPU
 // fallback), so lastFallbackSeconds is 0 and lastOverflowPixels counts the
 // roots that took a second GPU pass; polygon lists come back in a canonical
-// order.
+// order; useMailbox turns on the library's mailbox (btg_options.mailbox) for
+// all three applications (soft shadows: with SoftOptions::mailbox), whose
+// results equal those without it.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -75,12 +77,13 @@ TraceStats toStats(const btg_trace_stats& s) {
   return t;
 }
 
-void applyKnobs(btg_ctx* ctx, int budget, int primaryBudget, size_t capacity) {
+void applyKnobs(btg_ctx* ctx, int budget, int primaryBudget, size_t capacity, bool mailbox) {
   btg_options o;
   if (btg_get_options(ctx, &o) != BTG_OK) die("btg_get_options", ctx);
   o.budget = budget;
   o.primaryBudget = primaryBudget;
   o.capacity = int64_t(capacity);
+  o.mailbox = mailbox ? 1 : 0;
   if (btg_set_options(ctx, &o) != BTG_OK) die("btg_set_options", ctx);
 }
 
@@ -135,7 +138,9 @@ GpuRenderer::~GpuRenderer() {
 }
 
 void GpuRenderer::softShadows(const Camera& cam, const AreaLight& light, const SoftOptions& opt, SoftResult& res) {
-  applyKnobs(impl_->ctx, wfBudget, wfPrimaryBudget, wfCapacity);
+  // Post Office mailboxing when both the renderer and the call ask for it, as
+  // the reference's GPU soft shadows do (cuda/gpu_render.cu:612)
+  applyKnobs(impl_->ctx, wfBudget, wfPrimaryBudget, wfCapacity, useMailbox && opt.mailbox);
   const size_t npix = size_t(cam.W) * size_t(cam.H);
   res.W = cam.W;
   res.H = cam.H;
@@ -166,7 +171,7 @@ void GpuRenderer::softShadows(const Camera& cam, const AreaLight& light, const S
 }
 
 void GpuRenderer::primary(const Camera& cam, bool cull, std::vector<OutBeam>& hitBeams, RenderStats& rs, int tile) {
-  applyKnobs(impl_->ctx, wfBudget, wfPrimaryBudget, wfCapacity);
+  applyKnobs(impl_->ctx, wfBudget, wfPrimaryBudget, wfCapacity, useMailbox);
   const auto t0 = std::chrono::steady_clock::now();
   btg_camera c = toCam(cam);
   btg_polys* out = nullptr;
@@ -189,7 +194,7 @@ void GpuRenderer::primary(const Camera& cam, bool cull, std::vector<OutBeam>& hi
 void GpuRenderer::pointShadows(const Camera& cam, const std::vector<OutBeam>& primHits, const Vec3& light,
                                std::vector<OutBeam>& shadowPolys, RenderStats& rs) {
   if (primHits.empty()) return;
-  applyKnobs(impl_->ctx, wfBudget, wfPrimaryBudget, wfCapacity);
+  applyKnobs(impl_->ctx, wfBudget, wfPrimaryBudget, wfCapacity, useMailbox);
   const auto t0 = std::chrono::steady_clock::now();
   btg_camera c = toCam(cam);
   const float l3[3] = {light.x, light.y, light.z};

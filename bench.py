@@ -89,7 +89,9 @@ def parse():
                     help="sub-beam records per queue (0: batch64 sizes it from device memory, others use the "
                          "library default)")
     ap.add_argument("--leaf-cap", type=int, default=-1, help="leaves per sub-beam per trace launch (-1: default)")
-    ap.add_argument("--qps", default="3,0",
+    ap.add_argument("--mailbox", type=int, default=-1, choices=(-1, 0, 1),
+                    help="GPU mailboxing of repeated triangle tests (-1: library default, off)")
+    ap.add_argument("--qps", default="10,0",
                     help="serve: request arrival rates, comma-separated (0: every request at once, the saturation "
                          "capacity)")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the list")
@@ -164,6 +166,8 @@ def main() -> int:
         kw["capacity"] = args.capacity
     if args.leaf_cap >= 0:
         kw["leaf_cap"] = args.leaf_cap
+    if args.mailbox >= 0:
+        kw["mailbox"] = bool(args.mailbox)
     r.set_options(**kw)
     opts = r.options
     for _ in range(args.warmup):
@@ -270,6 +274,8 @@ def bench_batch(args) -> int:
         kw["budget"] = args.budget
     if args.leaf_cap >= 0:
         kw["leaf_cap"] = args.leaf_cap
+    if args.mailbox >= 0:
+        kw["mailbox"] = bool(args.mailbox)
     r.set_options(**kw)
     jobs = []
     for d in draw:
@@ -319,6 +325,7 @@ def bench_batch(args) -> int:
         print(f"bench.py: batched step {k + 1}: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     capacity = r.options["capacity"]
+    mailbox = r.options["mailbox"]
     precision = None if args.no_cpu else batch_precision(r, scenes, jobs, px)
     r.close()
     st = infos[-1]["stats"]
@@ -348,7 +355,7 @@ def bench_batch(args) -> int:
                    "jobs": len(jobs), "pixels_per_step": px, "tris": {n: scenes[n].num_tris for n in scenes},
                    "draw_seed": 64, "scene_seed": 2007},
         "device": device_label(dev), "arch": dev["arch"],
-        "schedule": dict(tuned, capacity=capacity),
+        "schedule": dict(tuned, capacity=capacity, mailbox=mailbox),
         "plain": None if plain_s is None else {"value": round(px / plain_s, 1), "unit": "pixels/s",
                                                "seconds": round(plain_s, 3), "form": "one softShadows call per job"},
         "batched_over_plain": None if plain_s is None else round(plain_s / (wall / args.steps), 3),

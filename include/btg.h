@@ -100,6 +100,11 @@ typedef struct {
                          the process issue on the legacy (null) stream fail on this runtime; a program
                          that uses the legacy stream from other threads meanwhile sets 0. The library
                          itself never uses the legacy stream, and a server's passes never capture. */
+  int mailbox;        /* 1: Post Office mailboxing on the GPU (GpuRenderer::useMailbox, cuda/gpu_render.h:48-50;
+                         src/beam/beam_core.h:772-781, 884-912): a sub-beam skips a triangle it has already been
+                         tested against with the result Keep while unchanged (a triangle straddling a split plane
+                         is listed in several leaves); counted in mailboxSkips instead of triTests. Results are the
+                         same as with 0 (default). */
 } btg_options;
 int btg_get_options(const btg_ctx* ctx, btg_options* opt);
 int btg_set_options(btg_ctx* ctx, const btg_options* opt);

@@ -296,3 +296,46 @@ def test_primary_and_point_independent_of_leaf_cap(require_gpu, name):
         for key in ("kdSteps", "leafVisits", "triTests", "hits", "splits", "beams", "climbs"):
             assert res[cap][2][key] == res[None][2][key], key
             assert res[cap][3][key] == res[None][3][key], key
+
+
+@pytest.mark.parametrize("name", ["plant", "blob", "conference"])
+def test_mailbox_changes_nothing_but_the_test_count(require_gpu, name):
+    """Post Office mailboxing on the GPU (btg_options.mailbox): a sub-beam skips
+    triangles it has already passed unchanged. Images, polygons and every
+    counter but triTests are identical with and without it, and each skip
+    replaces exactly one test (triTests + mailboxSkips is unchanged)."""
+    from beamtrace2007_amd.gpu import polys_to_dict
+    from beamtrace2007_amd.host import camera
+    from refpy import canon
+    s, a, k, sp = load_scene(name)
+    r = renderer(name)
+    W, H = 64, 48
+    cam = camera(sp["view"], W, H)
+    saved = r.options
+    runs = {}
+    try:
+        for mb in (False, True):
+            r.set_options(mailbox=mb)
+            print(f"{name} mailbox {mb}", flush=True)
+            soft = r.soft_shadows(cam, W, H, sp["area_light"], exact=True)
+            prim, pinfo = r.primary(cam, W, H, tile=16)
+            pt, qinfo = r.point_shadows(cam, W, H, prim, sp["point_light"])
+            runs[mb] = (soft, prim, pinfo, pt, qinfo)
+    finally:
+        r.set_options(**saved)
+    (s0, p0, pi0, q0, qi0), (s1, p1, pi1, q1, qi1) = runs[False], runs[True]
+    for f in ("tri", "vis", "E", "Eu"):
+        assert np.array_equal(s0[f], s1[f]), f
+    assert np.array_equal(canon(polys_to_dict(p0)), canon(polys_to_dict(p1)))
+    assert np.array_equal(canon(polys_to_dict(q0)), canon(polys_to_dict(q1)))
+    skips = 0
+    for i0, i1 in ((s0["info"], s1["info"]), (pi0, pi1), (qi0, qi1)):
+        if i0["killed_roots"] or i1["killed_roots"]:
+            continue  # roots traced twice: counters are not comparable
+        a0, a1 = i0["stats"], i1["stats"]
+        assert a0["mailboxSkips"] == 0
+        assert a1["triTests"] + a1["mailboxSkips"] == a0["triTests"]
+        for key in ("kdSteps", "leafVisits", "hits", "splits", "beams", "climbs"):
+            assert a0[key] == a1[key], key
+        skips += a1["mailboxSkips"]
+    print(f"{name}: {skips} mailbox skips", flush=True)
