@@ -1331,7 +1331,7 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
   const int total = __shfl(incl, kWave - 1, kWave);
   const int excl = incl - k;
   bool resolved = !(want && k > 0);
-  int first = -1, firstTri = -1, counted = 0;
+  int first = -1, firstTri = -1, counted = 0, skipped = 0;
   bool firstAllIn = false;
   float hA0 = 0, hA1 = 0, hA2 = 0, hC = 0;  // the first AllIn's plane (primary)
   F3 hN = f3(0, 0, 0);
@@ -1386,7 +1386,6 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
     if (live && p.mailbox) {
       tri = p.sv.refTri[uint32_t(oOff) + uint32_t(oCursor + (t - oExcl))];
       boxed = ocol[(tri & (kMbox - 1)) * kBlock] == tri;
-      if (boxed) cnt.v[sMail]++;
     }
     if (live && !boxed) {
       const uint32_t idx = uint32_t(oCursor + (t - oExcl));
@@ -1413,6 +1412,7 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
       }
     }
     const uint64_t hitB = __ballot(status >= 2), cntB = __ballot(status >= 1), allInB = __ballot(status == 2);
+    const uint64_t boxB = p.mailbox ? __ballot(boxed) : 0ull;
     if (p.mailbox && status == 1) {  // a Keep before the owner's first other result: the beam passed it unchanged
       const int a = oExcl - base > 0 ? oExcl - base : 0;
       const int b = oIncl - base < kWave ? oIncl - base : kWave;
@@ -1440,6 +1440,7 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
           resolved = true;
         }
         counted += __popcll(cntB & upto);
+        skipped += __popcll(boxB & upto);
       }
       if (!resolved && incl <= base + kWave) resolved = true;  // all of this owner's tests done: Keep
     }
@@ -1461,6 +1462,7 @@ __device__ int coopTest(const Params& p, const Ctx& c, Rec& r, Work& w, const St
   }
   if (!want) return kContinue;
   cnt.v[sTests] += uint32_t(counted);
+  cnt.v[sMail] += uint32_t(skipped);
   if (first >= 0) {
     if (firstAllIn) {
       if (App == kPrimary) {  // testOne's nearest-hit update
