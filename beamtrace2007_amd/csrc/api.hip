@@ -353,7 +353,7 @@ int btg_get_options(const btg_ctx* ctx, btg_options* o) {
   o->leafCap = x.leafCap;
   o->precision = x.precision;
   o->useGraphs = x.graphs ? 1 : 0;
-  o->mailbox = x.mailbox ? 1 : 0;
+  o->mailbox = x.mailbox;
   return BTG_OK;
 }
 
@@ -372,7 +372,7 @@ int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   x.leafCap = o->leafCap;
   x.precision = o->precision;
   x.graphs = o->useGraphs != 0;
-  x.mailbox = o->mailbox != 0;
+  x.mailbox = o->mailbox;
   return BTG_OK;
 }
 
@@ -419,7 +419,8 @@ int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact,
   if (!jobs || numJobs <= 0) return failWith(ctx, BTG_ERR_ARG, "the tuning sample needs at least one job");
   btg::Options& opt = btg::engineOptions(ctx->engine);
   const btg::Options start = opt;
-  // Coordinate descent over the schedule knobs, one knob at a time, each
+  // Coordinate descent over the schedule knobs (leaf cap, rounds between
+  // host checks, mailboxing), one knob at a time, each
   // value timed on the sample (GPU time of the beam kernels and primary
   // rays); a value replaces the incumbent only when it is more than 2%
   // faster, since a scaled-down sample cannot resolve closer calls, and the
@@ -435,7 +436,8 @@ int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact,
     std::vector<int> values;
   };
   const Knob knobs[] = {{&btg::Options::leafCap, {4, 16, 2, 32, 8}},
-                        {&btg::Options::roundsPerCheck, {4, 8, 16, 32}}};
+                        {&btg::Options::roundsPerCheck, {4, 8, 16, 32}},
+                        {&btg::Options::mailbox, {1, 0}}};
   std::vector<btg::SoftJobH> h(static_cast<size_t>(numJobs));
   for (int j = 0; j < numJobs; ++j) {
     if (!validCam(&jobs[j].cam)) return failWith(ctx, BTG_ERR_ARG, "job " + std::to_string(j) + ": invalid camera");
@@ -457,7 +459,7 @@ int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact,
   int rc = timeOf(cur, bestSec);  // also warms the code and buffers up
   if (!rc) rc = timeOf(cur, bestSec);
   auto record = [&](const btg::Options& o, double sec) {
-    if (trials && n < maxTrials) trials[n] = btg_autotune_trial{o.leafCap, o.budget, o.roundsPerCheck, sec};
+    if (trials && n < maxTrials) trials[n] = btg_autotune_trial{o.leafCap, o.budget, o.roundsPerCheck, o.mailbox, sec};
     ++n;
   };
   if (!rc) record(cur, bestSec);
@@ -491,6 +493,7 @@ int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact,
   best->leafCap = cur.leafCap;
   best->precision = start.precision;
   best->useGraphs = start.graphs ? 1 : 0;
+  best->mailbox = cur.mailbox;
   return BTG_OK;
 }
 

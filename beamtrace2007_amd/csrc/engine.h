@@ -66,7 +66,7 @@ struct Options {
   int roundsPerCheck = 8;
   int leafCap = 8;
   int precision = 0;  // soft shadows: 0 float build (bit-exact), 1 FP64-guarded
-  bool mailbox = false;  // skip exact repeats of a triangle test by one sub-beam (results unchanged)
+  int mailbox = 0;  // 1: skip exact repeats of a triangle test by one sub-beam (results unchanged)
   bool profile = false;
   bool graphs = true;  // replay each group of rounds from a captured HIP graph
 };

@@ -2637,7 +2637,7 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
   // long as their slowest lane: they yield after at most kPrimaryLeafCap
   // leaves (measured on the conference hall: 45 ms against 50 ms at 8).
   p.leafCap = app == kPrimary && e.opt.leafCap > 0 ? std::min(e.opt.leafCap, kPrimaryLeafCap) : e.opt.leafCap;
-  p.mailbox = e.opt.mailbox ? 1 : 0;
+  p.mailbox = e.opt.mailbox;
   return 0;
 }
 

@@ -49,7 +49,8 @@ class btg_area_light(C.Structure):
 
 
 class btg_autotune_trial(C.Structure):
-    _fields_ = [("leafCap", C.c_int), ("budget", C.c_int), ("roundsPerCheck", C.c_int), ("seconds", C.c_double)]
+    _fields_ = [("leafCap", C.c_int), ("budget", C.c_int), ("roundsPerCheck", C.c_int), ("mailbox", C.c_int),
+                ("seconds", C.c_double)]
 
 
 class btg_server_config(C.Structure):
@@ -280,8 +281,9 @@ class GpuRenderer:
                                 C.byref(n))
         if rc:
             self._err("btg_autotune", rc)
-        opts = {"budget": best.budget, "rounds_per_check": best.roundsPerCheck, "leaf_cap": best.leafCap}
-        tr = [{"leaf_cap": t.leafCap, "budget": t.budget, "rounds_per_check": t.roundsPerCheck,
+        opts = {"budget": best.budget, "rounds_per_check": best.roundsPerCheck, "leaf_cap": best.leafCap,
+                "mailbox": bool(best.mailbox)}
+        tr = [{"leaf_cap": t.leafCap, "budget": t.budget, "rounds_per_check": t.roundsPerCheck, "mailbox": bool(t.mailbox),
                "seconds": t.seconds} for t in trials[:min(n.value, 64)]]
         return opts, tr
 

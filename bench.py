@@ -58,16 +58,24 @@ WORKLOADS = {
 }
 METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request latency"
 # Measured HBM bytes per trace-kernel launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-# scaled by a 256 MiB copy: profiles/trace_plant512_hbm_traffic.txt and
-# profiles/trace_batch64_hbm_traffic.txt), valid for the schedule they were
-# measured with; other runs report null.
-TRAFFIC = {("plant", 512): 486063220}
-# batch64: keyed by the schedule (queue capacity, budget, rounds per check, leaf cap)
-TRAFFIC_BATCH = {(1 << 28, 1024, 8, 8): 5061659403}
-# What bounds the trace kernel instead of HBM: VALU issue under divergence,
-# measured with rocprofv3 PMC counters for the same schedule
-# (profiles/trace_batch64_valu_busy.txt, profiles/trace_batch64_sq_counters.txt)
-ISSUE_BATCH = {(1 << 28, 1024, 8, 8): {"valu_busy": 0.808, "lane_utilisation": 0.230}}
+# scaled by a 256 MiB copy: the *_hbm_traffic.txt summaries under profiles/) and
+# the trace kernel's issue figures (rocprofv3 SQ counters: *_valu_busy.txt,
+# *_sq_counters.txt), valid only for the library build and schedule they were
+# measured with: keyed by the SHA-256 prefix of lib/libbtg.so and the schedule,
+# so that any kernel change or another schedule reports null, never a stale figure.
+TRAFFIC = {}
+TRAFFIC_BATCH = {}
+ISSUE_BATCH = {}
+
+
+def library_hash() -> str:
+    import hashlib
+    path = os.environ.get("BTG_LIBRARY") or os.path.join(ROOT, "beamtrace2007_amd", "lib", "libbtg.so")
+    try:
+        with open(path, "rb") as f:
+            return hashlib.sha256(f.read()).hexdigest()[:16]
+    except OSError:
+        return ""
 
 
 def parse():
@@ -190,7 +198,7 @@ def main() -> int:
     if trace_s > 0:
         achieved = b_trace * args.steps / trace_s / 1e9
         defaults = args.budget <= 0 and args.capacity <= 0 and args.leaf_cap < 0
-        traffic = TRAFFIC.get((args.workload, res)) if defaults else None
+        traffic = TRAFFIC.get((library_hash(), args.workload, res)) if defaults else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                     "kernel": "traceKernel", "launches_per_step": trace_launches / args.steps,
@@ -301,7 +309,7 @@ def bench_batch(args) -> int:
     t1 = time.perf_counter()
     if args.no_tune:
         o = r.options
-        tuned, trials = {k: o[k] for k in ("budget", "rounds_per_check", "leaf_cap")}, []
+        tuned, trials = {k: o[k] for k in ("budget", "rounds_per_check", "leaf_cap", "mailbox")}, []
     else:
         tuned, trials = r.autotune(sample, exact=True, max_seconds=15.0)
     tune_s = time.perf_counter() - t1
@@ -335,13 +343,14 @@ def bench_batch(args) -> int:
     roofline = None
     if trace_s > 0:
         achieved = b_trace * args.steps / trace_s / 1e9
-        traffic = TRAFFIC_BATCH.get((capacity, tuned["budget"], tuned["rounds_per_check"], tuned["leaf_cap"]))
+        traffic = TRAFFIC_BATCH.get((library_hash(), capacity, tuned["budget"], tuned["rounds_per_check"],
+                                     tuned["leaf_cap"], mailbox))
         roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "kernel": "traceKernel",
                     "launches_per_step": trace_launches / args.steps,
                     "avg_launch_ms": round(trace_s / max(1, trace_launches) * 1e3, 4), "alg_bytes_per_step": int(b_trace)}
     value = px * args.steps / wall
-    sched = (capacity, tuned["budget"], tuned["rounds_per_check"], tuned["leaf_cap"])
+    sched = (library_hash(), capacity, tuned["budget"], tuned["rounds_per_check"], tuned["leaf_cap"], mailbox)
     issue = None if sched not in ISSUE_BATCH else dict(
         ISSUE_BATCH[sched], kernel="traceKernel",
         meaning="fraction of SIMD cycles issuing a VALU instruction, and of lanes active per VALU instruction "
@@ -355,7 +364,7 @@ def bench_batch(args) -> int:
                    "jobs": len(jobs), "pixels_per_step": px, "tris": {n: scenes[n].num_tris for n in scenes},
                    "draw_seed": 64, "scene_seed": 2007},
         "device": device_label(dev), "arch": dev["arch"],
-        "schedule": dict(tuned, capacity=capacity, mailbox=mailbox),
+        "schedule": dict(tuned, capacity=capacity, mailbox=mailbox), "library_sha256_16": library_hash(),
         "plain": None if plain_s is None else {"value": round(px / plain_s, 1), "unit": "pixels/s",
                                                "seconds": round(plain_s, 3), "form": "one softShadows call per job"},
         "batched_over_plain": None if plain_s is None else round(plain_s / (wall / args.steps), 3),

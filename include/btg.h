@@ -206,7 +206,7 @@ int btg_soft_batch(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exac
  * and, if `trials` is given, up to maxTrials timed settings. Results do not
  * depend on these knobs, only speed does. */
 typedef struct {
-  int leafCap, budget, roundsPerCheck;
+  int leafCap, budget, roundsPerCheck, mailbox;
   double seconds;
 } btg_autotune_trial;
 int btg_autotune(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exact, double maxSeconds,
