@@ -49,7 +49,14 @@ namespace btg {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kBlock = 256;
+// Threads per workgroup of the wavefront kernels (beams per workgroup), 4
+// waves; -DBTG_BLOCK=128 or 512 builds the other sizes (the LDS columns of the
+// trace and split kernels scale with it).
+#ifndef BTG_BLOCK
+#define BTG_BLOCK 256
+#endif
+constexpr int kBlock = BTG_BLOCK;
+constexpr int kRayBlock = 256;  // primary rays: one 16 x 16 pixel tile per workgroup
 constexpr int kPrimaryLeafCap = 4;  // leaf cap of primary passes (at most; see prepare)
 constexpr int kMbox = 8;            // mailbox entries per lane (a power of two)
 
@@ -835,7 +842,10 @@ __device__ __forceinline__ void clipTri(const Ctx& c, const Beam& B, Ln (&L)[5],
 // n | list << 8 | welded << 9;
 // pieces have at most 5 corners), word k of slot s of a lane at
 // col[(s * kStageWords + k) * kBlock].
-constexpr int kStageSlots = 7, kStageWords = 11;
+#ifndef BTG_STAGE_SLOTS
+#define BTG_STAGE_SLOTS 7
+#endif
+constexpr int kStageSlots = BTG_STAGE_SLOTS, kStageWords = 11;
 
 // Counting pass: records each list produces (pentagons become two) and the
 // total area of the hit pieces (summed in piece order, as the reference).
@@ -2414,7 +2424,7 @@ __device__ void hitPointD(const LeafTri& t, F3 o, F3 d, F3& X, F3& Xlo) {
 
 // The soft-shadow primary pass (cuda/gpu_render.cu:222-244): one ray through
 // each pixel centre; X = o + d t, N = the triangle normal facing the eye.
-__global__ void __launch_bounds__(kBlock) primaryRayKernel(Params p) {
+__global__ void __launch_bounds__(kRayBlock) primaryRayKernel(Params p) {
   Cnt cnt;
   cnt.zero();
   // block -> (job, 16 x 16 tile): the job with the largest tileBase <= block
@@ -3146,7 +3156,7 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
   if ((rc = zeroStats(*e, err))) return rc;
   BTG_TRY(hipEventRecord(e->ev[0], e->stream));
   const int lp = e->beginLaunch(3);
-  primaryRayKernel<<<unsigned(tiles), kBlock, 0, e->stream>>>(p);
+  primaryRayKernel<<<unsigned(tiles), kRayBlock, 0, e->stream>>>(p);
   e->endLaunch(lp);
   BTG_TRY(hipGetLastError());
   BTG_TRY(hipEventRecord(e->ev[1], e->stream));
@@ -3424,7 +3434,7 @@ int openTryAdd(OpenPass* op, const SoftJobH& h, int* slot, std::string& err) {
     q.jobs = static_cast<const SoftJob*>(e.jobs.ptr) + j;
     q.numJobs = 1;
     const int tiles = ((h.cam.W + 15) / 16) * ((h.cam.H + 15) / 16);
-    primaryRayKernel<<<unsigned(tiles), kBlock, 0, e.stream>>>(q);
+    primaryRayKernel<<<unsigned(tiles), kRayBlock, 0, e.stream>>>(q);
     he = hipGetLastError();
   }
   if (he != hipSuccess) {
