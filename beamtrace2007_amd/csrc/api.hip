@@ -360,7 +360,7 @@ int btg_get_options(const btg_ctx* ctx, btg_options* o) {
 int btg_set_options(btg_ctx* ctx, const btg_options* o) {
   if (!ctx || !o) return BTG_ERR_ARG;
   if (o->budget < 1 || o->primaryBudget < 1 || o->capacity < 1024 || o->capacity > (int64_t(1) << 31) ||
-      o->roundsPerCheck < 1 || o->leafCap < 0 || o->precision < 0 || o->precision > 1 || o->useGraphs < 0 ||
+      o->roundsPerCheck < 1 || o->leafCap < 0 || o->precision < 0 || o->precision > 2 || o->useGraphs < 0 ||
       o->useGraphs > 1 || o->mailbox < 0 || o->mailbox > 1)
     return failWith(ctx, BTG_ERR_ARG, "invalid options");
   btg::Options& x = btg::engineOptions(ctx->engine);

@@ -181,6 +181,7 @@ struct Params {
   const int* pixTri;
   const float* pixX;
   const float* pixXlo;  // guarded soft: low parts of the double hit points (null otherwise)
+  int* pixRef;          // FP64 soft: the hit's leaf reference per pixel (null otherwise)
   const float* pixN;
   float* vis;
   float* E;
@@ -2422,6 +2423,120 @@ __device__ void hitPointD(const LeafTri& t, F3 o, F3 d, F3& X, F3& Xlo) {
   Xlo = f3(float(px - double(X.x)), float(py - double(X.y)), float(pz - double(X.z)));
 }
 
+#include "trace_fp64.h"
+
+// The job of a batch pixel: the last job whose first pixel is <= k.
+__device__ int jobOfPixel(const Params& p, int k) {
+  int lo = 0, hi = p.numJobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (p.jobs[mid].pixBase <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// One pixel of FP64 soft shadows: beamLightVisibility (render.cpp:279-303)
+// with lightBeamSetup and lightCenterG (soft_core.h:38-84) in double; false
+// when the tracer's pools overflowed (nothing written).
+template <class TR>
+__device__ bool fp64Pixel(const Params& p, const SoftJob& J, int pix, TR& tr, Cnt& cnt) {
+  using fp64::V;
+  const int T = p.pixTri[pix];
+  if (T < 0) {
+    p.vis[pix] = -1;
+    p.E[pix] = 0;
+    p.Eu[pix] = 0;
+    return true;
+  }
+  const V x = fp64::vmk(double(p.pixX[3 * pix]) + double(p.pixXlo[3 * pix]),
+                        double(p.pixX[3 * pix + 1]) + double(p.pixXlo[3 * pix + 1]),
+                        double(p.pixX[3 * pix + 2]) + double(p.pixXlo[3 * pix + 2]));
+  const LeafTri ref = p.sv.refs[uint32_t(p.pixRef[pix])];
+  const V nd = TR::normalOf(fp64::vmk(ref.v[0], ref.v[1], ref.v[2]), fp64::vmk(ref.v[3], ref.v[4], ref.v[5]),
+                            fp64::vmk(ref.v[6], ref.v[7], ref.v[8]));
+  const V nf = fp64::vmk(p.pixN[3 * pix], p.pixN[3 * pix + 1], p.pixN[3 * pix + 2]);  // faces the eye
+  const V ns = fp64::dot(nd, nf) < 0 ? fp64::neg(nd) : nd;
+  const V c = fp64::vf(J.lc), U = fp64::vf(J.lU), Vv = fp64::vf(J.lV);
+  const V uv = fp64::crs(U, Vv);
+  const double A = fp64::len(uv);
+  const V nL = fp64::nrmz(uv);
+  const bool front = fp64::dot(x - c, nL) > 0;
+  double G0 = 0;
+  if (front) {
+    const V w = c - x;
+    const double r2 = fp64::dot(w, w), rl = sqrt(r2);
+    const V wn = fp64::vmk(w.x / rl, w.y / rl, w.z / rl);
+    const double ci = fp64::dot(ns, wn), cl = -fp64::dot(nL, wn);
+    G0 = (ci > 0 ? ci : 0) * (cl > 0 ? cl : 0) / r2;
+  }
+  p.Eu[pix] = float(A * G0);
+  p.vis[pix] = 0;
+  p.E[pix] = 0;
+  if (!front) return true;
+  tr.Q = fp64::Plane::make(x + ns * double(J.offset), c, U, Vv);
+  const double hzc = fp64::dot(ns, c - x);
+  const V hu = U * 0.5, hv = Vv * 0.5;
+  const V corners[4] = {(c - hu) - hv, (c + hu) - hv, (c + hu) + hv, (c - hu) + hv};
+  fp64::Poly quad;
+  quad.n = 0;
+  for (int i = 0; i < 4; ++i) {
+    double qx, qy;
+    tr.Q.coords(corners[i], qx, qy);
+    quad.push(qx, qy);
+  }
+  if (quad.area() < 0) quad.reverse();
+  const double lightArea = quad.area();
+  const fp64::Line hz{fp64::dot(ns, tr.Q.u), fp64::dot(ns, tr.Q.v), hzc};
+  int above = 0;
+  for (int i = 0; i < quad.n; ++i) above += hz.at(quad.x[i], quad.y[i]) > 0;
+  if (above == 0) return true;
+  fp64::Poly root = quad;
+  if (above != quad.n) {
+    fp64::Poly rest;
+    fp64::splitPoly(quad, hz, 0, root, rest);
+  }
+  if (!(root.n >= 3 && root.area() > 0)) return true;
+  tr.sv = &p.sv;
+  tr.exclude = T;
+  tr.x = x;
+  tr.ns = ns;
+  tr.exact = p.exact != 0;
+  tr.cnt = &cnt;
+  tr.trace(root, J.node0, J.bounds);
+  if (tr.overflow) return false;
+  double vis = tr.missArea / lightArea;
+  vis = vis < 0 ? 0.0 : (vis > 1 ? 1.0 : vis);
+  p.vis[pix] = float(vis);
+  p.E[pix] = p.exact ? float(tr.G) : float(A * G0 * vis);
+  return true;
+}
+
+// FP64 soft shadows over `count` pixels (list[i], or i without a list), one
+// pixel per thread with pools of Pn beams and Kn work items in private
+// memory. A pixel whose pools overflow is appended to `retry` (traced again
+// with larger pools); without a retry list it is counted in *failed.
+template <int Pn, int Kn>
+__global__ void __launch_bounds__(64) softFp64Kernel(Params p, const int* list, int count, int* retry,
+                                                     int* retryCount, int* failed) {
+  Cnt cnt;
+  cnt.zero();
+  const int i = int(blockIdx.x) * 64 + int(threadIdx.x);
+  if (i < count) {
+    const int pix = list ? list[i] : i;
+    fp64::Tracer<Pn, Kn> tr;
+    if (!fp64Pixel(p, p.jobs[jobOfPixel(p, pix)], pix, tr, cnt)) {
+      if (retry)
+        retry[atomicAdd(retryCount, 1)] = pix;
+      else
+        atomicAdd(failed, 1);
+    }
+  }
+  flushCnt(p, cnt);
+}
+
 // The soft-shadow primary pass (cuda/gpu_render.cu:222-244): one ray through
 // each pixel centre; X = o + d t, N = the triangle normal facing the eye.
 __global__ void __launch_bounds__(kRayBlock) primaryRayKernel(Params p) {
@@ -2452,8 +2567,10 @@ __global__ void __launch_bounds__(kRayBlock) primaryRayKernel(Params p) {
     float* X = const_cast<float*>(p.pixX);
     float* N = const_cast<float*>(p.pixN);
     triOut[k] = -1;
+    if (p.pixRef) p.pixRef[k] = -1;
     if (rayNearest(p.sv, J.node0, J.bounds, J.eye, d, th_tri, th, th_ref, cnt)) {
       triOut[k] = th_tri;
+      if (p.pixRef) p.pixRef[k] = int(th_ref);
       F3 hp = J.eye + d * th;
       if (p.pixXlo) {
         F3 lo;
@@ -2531,6 +2648,7 @@ class Engine {
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
+  DevBuf pixRef, fpList;  // FP64 soft shadows: hit leaf reference per pixel; pixels to trace again, counters
   DevBuf rasterImg, rasterRows;  // device rasterization: image; row counts, span offsets and tile sums
   void* noAllocScratch = nullptr;  // this device's static rasterization scratch (null: unavailable)
   WState* hostState = nullptr;  // pinned
@@ -2582,7 +2700,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs};
+                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs, &pixRef, &fpList};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -3081,6 +3199,42 @@ void engineDestroy(Engine* e) { delete e; }
 
 Options& engineOptions(Engine* e) { return e->opt; }
 
+namespace {
+
+// FP64 soft shadows of every pixel of a batch (after its primary pass): pools
+// of 64 beams first, then 384 for the pixels that need more.
+int softFp64(Engine& e, Params& p, size_t np, std::string& err) {
+  int* retry = static_cast<int*>(e.fpList.ptr);
+  int* counters = retry + np;  // [0] pixels to trace again, [1] pixels that overflowed twice
+  clearInt<<<1, 64, 0, e.stream>>>(counters);
+  clearInt<<<1, 64, 0, e.stream>>>(counters + 1);
+  int lt = e.beginLaunch(1);
+  softFp64Kernel<64, 64><<<unsigned((np + 63) / 64), 64, 0, e.stream>>>(p, nullptr, int(np), retry, counters,
+                                                                        counters + 1);
+  e.endLaunch(lt);
+  BTG_TRY(hipGetLastError());
+  int again = 0;
+  BTG_TRY(hipMemcpyAsync(&again, counters, 4, hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipStreamSynchronize(e.stream));
+  if (again > 0) {
+    lt = e.beginLaunch(1);
+    softFp64Kernel<384, 256><<<unsigned((again + 63) / 64), 64, 0, e.stream>>>(p, retry, again, nullptr, nullptr,
+                                                                              counters + 1);
+    e.endLaunch(lt);
+    BTG_TRY(hipGetLastError());
+    int failed = 0;
+    BTG_TRY(hipMemcpyAsync(&failed, counters + 1, 4, hipMemcpyDeviceToHost, e.stream));
+    BTG_TRY(hipStreamSynchronize(e.stream));
+    if (failed > 0) {
+      err = std::to_string(failed) + " pixel(s) need more than 384 live beams in FP64 mode";
+      return BTG_ERR_LIMIT_CODE;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
 int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo& info, std::string& err) {
   if (n <= 0) {
     err = "a batch needs at least one job";
@@ -3139,6 +3293,11 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
       (guard && (rc = e->ensure(e->pixXlo, np * 12, err))))
     return rc;
   if (guard) p.pixXlo = static_cast<const float*>(e->pixXlo.ptr);
+  const bool fp64Mode = e->opt.precision == 2;
+  if (fp64Mode) {
+    if ((rc = e->ensure(e->pixRef, np * 4, err)) || (rc = e->ensure(e->fpList, (np + 4) * 4, err))) return rc;
+    p.pixRef = static_cast<int*>(e->pixRef.ptr);
+  }
   BTG_TRY(hipMemcpyAsync(e->jobs.ptr, jt.data(), jt.size() * sizeof(SoftJob), hipMemcpyHostToDevice, e->stream));
   p.jobs = static_cast<const SoftJob*>(e->jobs.ptr);
   p.numJobs = n;
@@ -3169,7 +3328,11 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
     BTG_TRY(hipGetLastError());
     return 0;
   };
-  if ((rc = runAll(*e, p, e->opt.budget, info, fin, err))) return rc;
+  if (fp64Mode) {
+    if ((rc = softFp64(*e, p, np, err))) return rc;
+  } else if ((rc = runAll(*e, p, e->opt.budget, info, fin, err))) {
+    return rc;
+  }
   BTG_TRY(hipEventRecord(e->ev[3], e->stream));
   BTG_TRY(hipEventSynchronize(e->ev[3]));
   float ms0 = 0, ms1 = 0;

@@ -1,0 +1,845 @@
+// FP64 soft shadows (btg_options.precision = 2), included by engine.hip.
+//
+// The reference's double build (Real = double, src/core/real.h:20-35) on the
+// device: one pixel per thread, the depth-first BeamCore with the restart trail
+// and without mailboxes (src/beam/beam_core.h:569-1003), its geometry
+// (src/beam/beam_geom.h) and soft-shadow setup (src/render/soft_core.h) in
+// double with the double build's tolerances. The scene data are the float
+// scene's (vertices, kd-tree: exact in double); the receiver point is the
+// guarded primary pass's double hit point and the normal is the triangle's,
+// computed in double as Scene::finalize does (src/scene/scene.cpp:18-33).
+// The tree is the float build's (the double build makes its own): kd
+// decisions are conservative, so the visible fraction is the same up to the
+// double rounding of the piece areas, about 1e-12 of the light.
+//
+// Every beam of a pixel lives in the thread's private memory (bounded pools:
+// a pixel that needs more is traced again with larger pools; results do not
+// depend on the pool sizes). For an any-hit query with the light plane as the
+// far plane, a beam carries only its corners and ray-direction signs: the far
+// plane is the same for all of them and |w| is recomputed from the corners.
+namespace fp64 {
+
+constexpr double kEps = 1e-9;      // fuzzy split epsilon, relative to the root's extent
+constexpr double kKd = 1e-12;      // kd decision margin
+constexpr double kDeg = 1e-10;     // edge-on triangles
+constexpr double kCopCos = 1e-9;   // coplanar: cosine
+constexpr double kCopRel = 1e-8;   // coplanar: offset
+
+struct V {
+  double x, y, z;
+};
+__device__ __forceinline__ V vmk(double x, double y, double z) { return V{x, y, z}; }
+__device__ __forceinline__ V vf(F3 a) { return V{double(a.x), double(a.y), double(a.z)}; }
+__device__ __forceinline__ V operator+(V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V operator-(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V operator*(V a, double s) { return V{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V neg(V a) { return V{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V crs(V a, V b) {
+  return V{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double len(V a) { return sqrt(dot(a, a)); }
+__device__ __forceinline__ V nrmz(V a) {
+  const double l = len(a);
+  return l > 0 ? V{a.x / l, a.y / l, a.z / l} : a;
+}
+__device__ __forceinline__ double at(V a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+__device__ __forceinline__ double absS(double v) { return v < 0 ? -v : v; }
+
+// Four corner lanes (R4, src/core/simd4.h).
+struct L4 {
+  double v[4];
+};
+__device__ __forceinline__ L4 affine(const L4& x, double a, const L4& y, double b, double c) {
+  L4 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) r.v[i] = x.v[i] * a + y.v[i] * b + c;
+  return r;
+}
+__device__ __forceinline__ L4 mul(const L4& a, double s) {
+  L4 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) r.v[i] = a.v[i] * s;
+  return r;
+}
+// lhs < rhs - (|lhs| + |rhs|) k on every lane; the same with > and +.
+__device__ __forceinline__ bool allLess(const L4& a, const L4& b, double k) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (!(a.v[i] < b.v[i] - (fabs(a.v[i]) + fabs(b.v[i])) * k)) return false;
+  return true;
+}
+__device__ __forceinline__ bool allGreater(const L4& a, const L4& b, double k) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (!(a.v[i] > b.v[i] + (fabs(a.v[i]) + fabs(b.v[i])) * k)) return false;
+  return true;
+}
+
+// PolyN<8> (beam_geom.h:16-52).
+struct Poly {
+  double x[8], y[8];
+  int n;
+  __device__ void push(double px, double py) {
+    if (n < 8) {
+      x[n] = px;
+      y[n] = py;
+      ++n;
+    }
+  }
+  __device__ double area() const {
+    double a = 0;
+    for (int i = 0; i < n; ++i) {
+      const int j = i + 1 == n ? 0 : i + 1;
+      a += x[i] * y[j] - x[j] * y[i];
+    }
+    return a * 0.5;
+  }
+  __device__ void reverse() {
+    for (int i = 0, j = n - 1; i < j; ++i, --j) {
+      const double tx = x[i], ty = y[i];
+      x[i] = x[j];
+      y[i] = y[j];
+      x[j] = tx;
+      y[j] = ty;
+    }
+  }
+  __device__ double extent() const {
+    double lx = INFINITY, hx = -INFINITY, ly = INFINITY, hy = -INFINITY;
+    for (int i = 0; i < n; ++i) {
+      lx = x[i] < lx ? x[i] : lx;
+      hx = x[i] > hx ? x[i] : hx;
+      ly = y[i] < ly ? y[i] : ly;
+      hy = y[i] > hy ? y[i] : hy;
+    }
+    const double ex = hx - lx, ey = hy - ly;
+    return ex > ey ? ex : ey;
+  }
+};
+
+struct Line {
+  double a, b, c;
+  __device__ double at(double px, double py) const { return a * px + b * py + c; }
+};
+
+// makeLine (beam_geom.h:65-76).
+__device__ Line makeLine(double a, double b, double c, double ext) {
+  Line L{0, 0, 0};
+  const double nrm = sqrt(a * a + b * b);
+  if (!(nrm > 0) || nrm * ext * 1e9 < fabs(c)) {
+    L.c = c > 0 ? 1.0 : (c < 0 ? -1.0 : 0.0);
+    return L;
+  }
+  L.a = a / nrm;
+  L.b = b / nrm;
+  L.c = c / nrm;
+  return L;
+}
+
+// splitPoly (beam_geom.h:82-114), capacity 8.
+__device__ bool splitPoly(const Poly& p, const Line& L, double eps, Poly& in, Poly& out) {
+  in.n = out.n = 0;
+  double d[8];
+  int cls[8];
+  int nin = 0, nout = 0;
+  for (int i = 0; i < p.n; ++i) {
+    d[i] = L.at(p.x[i], p.y[i]);
+    cls[i] = d[i] > eps ? 1 : (d[i] < -eps ? -1 : 0);
+  }
+  for (int i = 0; i < p.n; ++i) {
+    const int j = i + 1 == p.n ? 0 : i + 1;
+    nin += (cls[i] >= 0) + (cls[i] * cls[j] < 0);
+    nout += (cls[i] <= 0) + (cls[i] * cls[j] < 0);
+  }
+  if (nin > 8 || nout > 8) return false;
+  for (int i = 0; i < p.n; ++i) {
+    const int j = i + 1 == p.n ? 0 : i + 1;
+    if (cls[i] >= 0) in.push(p.x[i], p.y[i]);
+    if (cls[i] <= 0) out.push(p.x[i], p.y[i]);
+    if (cls[i] * cls[j] < 0) {
+      const bool iLow = p.x[i] < p.x[j] || (p.x[i] == p.x[j] && p.y[i] < p.y[j]);
+      const int a = iLow ? i : j, b = iLow ? j : i;
+      const double t = d[a] / (d[a] - d[b]);
+      const double X = p.x[a] + t * (p.x[b] - p.x[a]);
+      const double Y = p.y[a] + t * (p.y[b] - p.y[a]);
+      in.push(X, Y);
+      out.push(X, Y);
+    }
+  }
+  return true;
+}
+
+// cleanPoly (beam_geom.h:121-150).
+__device__ void cleanPoly(Poly& p, double epsSplit) {
+  const double e = epsSplit * 1e-3;
+  bool changed = true;
+  while (changed && p.n >= 3) {
+    changed = false;
+    for (int i = 0; i < p.n && p.n >= 3; ++i) {
+      const int h = i == 0 ? p.n - 1 : i - 1;
+      const int j = i + 1 == p.n ? 0 : i + 1;
+      const double ex = p.x[j] - p.x[h], ey = p.y[j] - p.y[h];
+      const double dx = p.x[i] - p.x[h], dy = p.y[i] - p.y[h];
+      const double l = sqrt(ex * ex + ey * ey);
+      const double d2 = dx * dx + dy * dy;
+      const double cr = fabs(ex * dy - ey * dx);
+      if (d2 <= e * e || cr <= e * l) {
+        for (int k = i; k + 1 < p.n; ++k) {
+          p.x[k] = p.x[k + 1];
+          p.y[k] = p.y[k + 1];
+        }
+        --p.n;
+        changed = true;
+        --i;
+      }
+    }
+  }
+  if (p.n < 3) p.n = 0;
+}
+
+// classifyPoly (beam_core.h:68-78): bit 0 some vertex inside, bit 1 outside.
+__device__ int sides(const Poly& p, const Line& L, double eps) {
+  int c = 0;
+  for (int i = 0; i < p.n; ++i) {
+    const double d = L.at(p.x[i], p.y[i]);
+    if (d > eps)
+      c |= 1;
+    else if (d < -eps)
+      c |= 2;
+  }
+  return c;
+}
+
+// splitFive (beam_core.h:80-89): pentagon -> quad (0..3) + triangle (0, 3, 4).
+__device__ void splitFive(const Poly& p, Poly& a, Poly& b) {
+  a.n = b.n = 0;
+  for (int i = 0; i < 4; ++i) a.push(p.x[i], p.y[i]);
+  b.push(p.x[0], p.y[0]);
+  b.push(p.x[3], p.y[3]);
+  b.push(p.x[4], p.y[4]);
+}
+
+// BeamPlane (beam_types.h:14-52).
+struct Plane {
+  V O, p0, u, v, n, d;
+  __device__ V rel(V p) const { return p - O; }
+  __device__ double rel(double s, int a) const { return s - at(O, a); }
+  __device__ V point(double qx, double qy) const { return (p0 + u * qx) + v * qy; }
+  __device__ void coords(V p, double& qx, double& qy) const {
+    const V r = p - p0;
+    qx = dot(r, u);
+    qy = dot(r, v);
+  }
+  __device__ static Plane make(V O, V p0, V U, V Vv) {
+    Plane P;
+    P.O = O;
+    P.p0 = p0;
+    const V d = p0 - O;
+    V n = nrmz(crs(U, Vv));
+    if (dot(n, d) < 0) n = neg(n);
+    P.u = nrmz(U);
+    P.v = crs(n, P.u);
+    P.n = n;
+    P.d = d;
+    return P;
+  }
+};
+
+// A sub-beam: corners on the query plane (lanes past n repeat the last) and
+// the ray-direction sign per axis.
+struct Beam {
+  L4 qx, qy;
+  int n;
+  int sgn[3];
+};
+
+// The working form of a beam (CoreBeam, beam_core.h:30-40): |w| per axis.
+struct Aw {
+  L4 aw[3];
+};
+
+// CoreTriInfo (beam_core.h:42-51).
+struct Tri {
+  bool ok;
+  Line edge[3];
+  double A[3], C;
+  V r[3], n;
+};
+
+struct Work {
+  int beam;
+  uint32_t node;
+  int depth;
+  uint64_t trail;
+  Box box;
+};
+
+// One pixel's query, depth-first. P: beams per pixel, K: pending work items,
+// S: pieces per clip list.
+template <int P, int K>
+struct Tracer {
+  static constexpr int S = 16;
+  const DevScene* sv;
+  Plane Q;
+  int exclude;
+  double eps, minArea, ext, farCq;
+  V x, ns;     // shading point and normal (exact Lambert integral of the miss pieces)
+  bool exact;  // integrate the miss pieces
+  double missArea, G;
+  bool overflow;
+  Cnt* cnt;
+  Beam pool[P];
+  int freeL[P], nFree, nPool;
+  int cur[P], nCur, nxt[P], nNxt;
+  Work work[K];
+  int nWork;
+  Poly lists[2][S];
+  int nList[2];
+
+  __device__ void setup(double extent) {
+    ext = extent > 0 ? extent : 1.0;
+    eps = kEps * ext;
+    minArea = eps * eps;
+    farCq = dot(Q.n, Q.d);
+    missArea = G = 0;
+    overflow = false;
+    nFree = nPool = nCur = nNxt = nWork = 0;
+  }
+
+  __device__ int alloc() {
+    if (nFree > 0) return freeL[--nFree];
+    if (nPool >= P) {
+      overflow = true;
+      return -1;
+    }
+    return nPool++;
+  }
+  __device__ void release(int b) {
+    if (nFree < P) freeL[nFree++] = b;
+  }
+  __device__ void pushWork(const Work& w) {
+    if (nWork >= K) {
+      overflow = true;
+      return;
+    }
+    work[nWork++] = w;
+  }
+  __device__ void pushNext(int b) {
+    if (nNxt >= P) {
+      overflow = true;
+      return;
+    }
+    nxt[nNxt++] = b;
+  }
+
+  __device__ Aw awOf(const Beam& b) const {
+    Aw r;
+    for (int a = 0; a < 3; ++a) {
+      const L4 w = affine(b.qx, at(Q.u, a), b.qy, at(Q.v, a), at(Q.d, a));
+      for (int i = 0; i < 4; ++i) {
+        const double ws = w.v[i] * double(b.sgn[a]);
+        r.aw[a].v[i] = 0.0 > ws ? 0.0 : ws;
+      }
+    }
+    return r;
+  }
+  // The light plane as every beam's far plane (setStatus, beam_core.h:225-240).
+  __device__ double farC() const { return absS(farCq); }
+
+  // decide (beam_core.h:139-164).
+  __device__ int decide(const Beam& b, const Aw& w, int a, double Ns, const Box& bx) const {
+    const L4& awa = w.aw[a];
+    for (int k = 1; k <= 2; ++k) {
+      const int c = a + k >= 3 ? a + k - 3 : a + k;
+      const double sg = double(b.sgn[c]);
+      const double entry = b.sgn[c] > 0 ? double(comp(bx.lo, c)) : double(comp(bx.hi, c));
+      const double exit = b.sgn[c] > 0 ? double(comp(bx.hi, c)) : double(comp(bx.lo, c));
+      const double nmin = sg * Q.rel(entry, c), nmax = sg * Q.rel(exit, c);
+      const L4 lhs = mul(w.aw[c], Ns);
+      if (allLess(lhs, mul(awa, nmin), kKd)) return 1;     // far
+      if (allGreater(lhs, mul(awa, nmax), kKd)) return 0;  // near
+    }
+    const double fc = farC();
+    if (allGreater(mul(L4{{fc, fc, fc, fc}}, Ns), mul(awa, fc), kKd)) return 0;
+    return 2;  // both
+  }
+
+  // overlaps (beam_core.h:166-195).
+  __device__ bool overlaps(const Beam& b, const Aw& w, const Box& bx) const {
+    double nmin[3], nmax[3];
+    for (int c = 0; c < 3; ++c) {
+      const double sg = double(b.sgn[c]);
+      nmin[c] = sg * Q.rel(b.sgn[c] > 0 ? double(comp(bx.lo, c)) : double(comp(bx.hi, c)), c);
+      nmax[c] = sg * Q.rel(b.sgn[c] > 0 ? double(comp(bx.hi, c)) : double(comp(bx.lo, c)), c);
+      if (nmax[c] < 0) return false;
+    }
+    for (int c = 0; c < 3; ++c)
+      for (int e = 0; e < 3; ++e) {
+        if (c == e) continue;
+        if (allLess(mul(w.aw[e], nmax[c]), mul(w.aw[c], nmin[e]), kKd)) return false;
+      }
+    const double fc = farC();
+    for (int c = 0; c < 3; ++c)
+      if (allGreater(mul(L4{{fc, fc, fc, fc}}, nmin[c]), mul(w.aw[c], fc), kKd)) return false;
+    return true;
+  }
+
+  // beamGeom (beam_core.h:200-221): false (and the area dropped) for a sliver.
+  __device__ bool geom(const Poly& p, const int sgn[3], Beam& b) {
+    Poly c = p;
+    cleanPoly(c, eps);
+    const double A = c.n >= 3 ? c.area() : 0.0;
+    if (c.n < 3 || A < minArea) return false;
+    const int l3 = c.n > 3 ? 3 : 2;
+    b.qx = L4{{c.x[0], c.x[1], c.x[2], c.x[l3]}};
+    b.qy = L4{{c.y[0], c.y[1], c.y[2], c.y[l3]}};
+    b.n = c.n;
+    for (int a = 0; a < 3; ++a) b.sgn[a] = sgn[a];
+    return true;
+  }
+
+  // computeTriInfo + triSetup (beam_core.h:242-286) of a leaf triangle.
+  __device__ Tri setupTri(const LeafTri& t) const {
+    Tri ti;
+    ti.ok = false;
+    V p[3];
+    for (int k = 0; k < 3; ++k) {
+      p[k] = vmk(double(t.v[3 * k]), double(t.v[3 * k + 1]), double(t.v[3 * k + 2]));
+      ti.r[k] = Q.rel(p[k]);
+    }
+    ti.n = normalOf(p[0], p[1], p[2]);
+    V m[3];
+    const int E[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+    for (int k = 0; k < 3; ++k) {
+      const int i = E[k][0], j = E[k][1];
+      m[k] = t.vi[i] < t.vi[j] ? crs(ti.r[i], ti.r[j]) : neg(crs(ti.r[j], ti.r[i]));
+    }
+    const double D = dot(ti.r[2], m[0]);
+    const double scale = len(m[0]) * len(ti.r[2]);
+    if (!(absS(D) > kDeg * scale)) return ti;
+    const double s = D > 0 ? 1.0 : -1.0;
+    for (int k = 0; k < 3; ++k) ti.edge[k] = makeLine(s * dot(m[k], Q.u), s * dot(m[k], Q.v), s * dot(m[k], Q.d), ext);
+    ti.A[0] = dot(ti.n, Q.u);
+    ti.A[1] = dot(ti.n, Q.v);
+    ti.A[2] = dot(ti.n, Q.d);
+    ti.C = dot(ti.n, ti.r[0]);
+    ti.ok = true;
+    return ti;
+  }
+
+  // closerLine (beam_core.h:302-317) against the light plane.
+  __device__ Line farLine(const Tri& T) const {
+    const double AF[3] = {0, 0, farCq}, CF = farCq;
+    const double cosang = dot(T.n, Q.n);
+    const double aC = absS(T.C), aF = absS(CF);
+    const double scaleC = aC > aF ? aC : aF;
+    const double ca = absS(cosang);
+    const double diff = T.C - cosang * CF;
+    if (ca > 1 - kCopCos && absS(diff) <= kCopRel * scaleC) return Line{0, 0, -1.0};
+    const double sg = T.C * CF > 0 ? 1.0 : -1.0;
+    return makeLine(sg * (CF * T.A[0] - T.C * AF[0]), sg * (CF * T.A[1] - T.C * AF[1]),
+                    sg * (CF * T.A[2] - T.C * AF[2]), ext);
+  }
+
+  // separated (beam_core.h:321-342).
+  __device__ bool separated(const Beam& b, const Tri& ti) const {
+    const L4 wx = affine(b.qx, Q.u.x, b.qy, Q.v.x, Q.d.x);
+    const L4 wy = affine(b.qx, Q.u.y, b.qy, Q.v.y, Q.d.y);
+    const L4 wz = affine(b.qx, Q.u.z, b.qy, Q.v.z, Q.d.z);
+    L4 nx, ny, nz, nl;
+    for (int i = 0; i < 4; ++i) {
+      const int j = (i + 1) & 3;
+      nx.v[i] = wy.v[i] * wz.v[j] - wz.v[i] * wy.v[j];
+      ny.v[i] = wz.v[i] * wx.v[j] - wx.v[i] * wz.v[j];
+      nz.v[i] = wx.v[i] * wy.v[j] - wy.v[i] * wx.v[j];
+      nl.v[i] = sqrt(nx.v[i] * nx.v[i] + ny.v[i] * ny.v[i] + nz.v[i] * nz.v[i]);
+    }
+    int outAll = b.n == 4 ? 0xF : 0xB;
+    for (int k = 0; k < 3 && outAll; ++k) {
+      const V r = ti.r[k];
+      const double lr = len(r) * kEps;
+      int m = 0;
+      for (int i = 0; i < 4; ++i) {
+        const double s = nx.v[i] * r.x + ny.v[i] * r.y + nz.v[i] * r.z;
+        if (s < -(nl.v[i] * lr)) m |= 1 << i;
+      }
+      outAll &= m;
+    }
+    return outAll != 0;
+  }
+
+  // testLine (beam_core.h:356-362).
+  __device__ bool testLine(const Beam& B, const Line& line, Line* L, int& nl, bool& allIn) const {
+    const L4 dv = affine(B.qx, line.a, B.qy, line.b, line.c);
+    bool anyIn = false, anyOut = false;
+    for (int i = 0; i < 4; ++i) {
+      anyIn = anyIn || dv.v[i] > eps;
+      anyOut = anyOut || dv.v[i] < -eps;
+    }
+    if (!anyIn) return false;
+    if (anyOut) allIn = false;
+    L[nl++] = line;
+    return true;
+  }
+
+  // classifyTri (beam_core.h:374-409) of an any-hit query: 0 Keep, 1 AllIn, 2 Clip.
+  __device__ int classify(const Beam& B, const Tri& ti, Line* L, int& nl) const {
+    nl = 0;
+    bool allIn = true;
+    for (int k = 0; k < 3; ++k)
+      if (!testLine(B, ti.edge[k], L, nl, allIn)) return 0;
+    if (!testLine(B, farLine(ti), L, nl, allIn)) return 0;
+    if (allIn) return 1;
+    if (separated(B, ti)) return 0;
+    return 2;
+  }
+
+  __device__ void listPush(int l, const Poly& p) {
+    if (nList[l] >= S) {
+      overflow = true;
+      return;
+    }
+    lists[l][nList[l]++] = p;
+  }
+
+  // clipTri (beam_core.h:419-479) into lists[0] (outside) and lists[1] (inside).
+  __device__ bool clip(const Beam& B, Line* L, int nl) {
+    nList[0] = nList[1] = 0;
+    double score[3];
+    for (int k = 0; k < 3; ++k) {
+      const L4 dv = affine(B.qx, L[k].a, B.qy, L[k].b, L[k].c);
+      double q[4];
+      for (int i = 0; i < 4; ++i) q[i] = dv.v[i] > 0.0 ? dv.v[i] : 0.0;
+      score[k] = q[0] + q[1] + q[2] + (B.n == 4 ? q[3] : 0.0);
+    }
+    for (int i = 1; i < 3; ++i)
+      for (int j = i; j > 0 && score[j] < score[j - 1]; --j) {
+        const double ts = score[j];
+        score[j] = score[j - 1];
+        score[j - 1] = ts;
+        const Line tl = L[j];
+        L[j] = L[j - 1];
+        L[j - 1] = tl;
+      }
+    Poly stack[8];
+    int stackLine[8];
+    int sp = 1;
+    stack[0].n = B.n;
+    for (int i = 0; i < B.n; ++i) {
+      stack[0].x[i] = B.qx.v[i];
+      stack[0].y[i] = B.qy.v[i];
+    }
+    stackLine[0] = 0;
+    while (sp > 0) {
+      --sp;
+      Poly p = stack[sp];
+      int li = stackLine[sp];
+      bool inside = true;
+      for (; li < nl; ++li) {
+        const int c = sides(p, L[li], eps);
+        if (!(c & 2)) continue;
+        if (!(c & 1)) {
+          listPush(0, p);
+          inside = false;
+          break;
+        }
+        Poly in, outp;
+        if (!splitPoly(p, L[li], eps, in, outp)) return false;
+        cleanPoly(outp, eps);
+        if (outp.n >= 3) listPush(0, outp);
+        cleanPoly(in, eps);
+        if (in.n < 3) {
+          inside = false;
+          break;
+        }
+        if (in.n == 5) {
+          Poly a, b;
+          splitFive(in, a, b);
+          cnt->v[sFive]++;
+          if (sp >= 8) return false;
+          stack[sp] = b;
+          stackLine[sp] = li + 1;
+          ++sp;
+          p = a;
+        } else {
+          p = in;
+        }
+      }
+      if (inside) listPush(1, p);
+    }
+    return true;
+  }
+
+  // output (beam_core.h:783-793): a finished piece; the miss pieces add their
+  // area and their exact Lambert integral (polygonFormFactorG, soft_core.h:18-33).
+  __device__ void output(const Beam& b, int tri) {
+    Poly p;
+    p.n = b.n;
+    for (int i = 0; i < b.n; ++i) {
+      p.x[i] = b.qx.v[i];
+      p.y[i] = b.qy.v[i];
+    }
+    const double area = p.area();
+    if (tri >= 0) cnt->v[sHits]++;
+    cnt->v[sBeams]++;
+    if (tri >= 0) return;
+    missArea += area;
+    if (!exact) return;
+    D3 P4[4];
+    for (int i = 0; i < 4; ++i) {
+      const int k = i < p.n ? i : p.n - 1;
+      const V q = Q.point(p.x[k], p.y[k]);
+      P4[i] = D3{q.x, q.y, q.z};
+    }
+    G += formFactorD(D3{x.x, x.y, x.z}, D3{ns.x, ns.y, ns.z}, P4, p.n);
+  }
+
+  __device__ void finalize(int bi) {
+    output(pool[bi], -1);
+    release(bi);
+  }
+
+  // emit (beam_core.h:801-826) of an any-hit query.
+  __device__ void emit(const Poly& p0, const Beam& parent, bool hitNow, int tri) {
+    Poly pieces[2];
+    int np = 1;
+    if (p0.n == 5) {
+      splitFive(p0, pieces[0], pieces[1]);
+      cnt->v[sFive]++;
+      np = 2;
+    } else {
+      pieces[0] = p0;
+    }
+    for (int i = 0; i < np; ++i) {
+      Beam nb;
+      if (!geom(pieces[i], parent.sgn, nb)) continue;
+      if (hitNow) {
+        output(nb, tri);
+        continue;
+      }
+      const int idx = alloc();
+      if (idx < 0) return;
+      pool[idx] = nb;
+      pushNext(idx);
+    }
+  }
+
+  // intersect (beam_core.h:832-882).
+  __device__ void intersect(int bi, int t, const Tri& ti) {
+    cnt->v[sTests]++;
+    Line L[5];
+    int nl = 0;
+    const Beam B = pool[bi];
+    const int r = classify(B, ti, L, nl);
+    if (r == 0) {
+      pushNext(bi);
+      return;
+    }
+    if (r == 1) {
+      output(B, t);
+      release(bi);
+      return;
+    }
+    if (!clip(B, L, nl)) {
+      overflow = true;
+      return;
+    }
+    double newArea = 0;
+    for (int k = 0; k < nList[1]; ++k) newArea += lists[1][k].area();
+    if (!(newArea >= minArea)) {
+      pushNext(bi);
+      return;
+    }
+    cnt->v[sSplits]++;
+    release(bi);
+    // lists are reused by the next clip: copy out before emitting
+    const int n0 = nList[0], n1 = nList[1];
+    for (int k = 0; k < n0; ++k) emit(lists[0][k], B, false, t);
+    for (int k = 0; k < n1; ++k) emit(lists[1][k], B, true, t);
+  }
+
+  // processLeaf without mailboxes (beam_core.h:884-912).
+  __device__ void processLeaf(int bi, const KdNodeD& leaf) {
+    cnt->v[sLeaf]++;
+    nCur = 1;
+    cur[0] = bi;
+    const uint32_t n = leaf.data >> 2;
+    for (uint32_t i = 0; i < n && !overflow; ++i) {
+      const LeafTri ref = sv->refs[leaf.offset + i];
+      if (ref.tri == exclude) continue;
+      const Tri ti = setupTri(ref);
+      if (!ti.ok) continue;
+      nNxt = 0;
+      for (int k = 0; k < nCur && !overflow; ++k) intersect(cur[k], ref.tri, ti);
+      for (int k = 0; k < nNxt; ++k) cur[k] = nxt[k];
+      nCur = nNxt;
+      if (nCur == 0) break;
+    }
+  }
+
+  // advance with the restart trail (beam_core.h:924-969).
+  __device__ void advance(int bi, uint32_t node, int depth, uint64_t trail) {
+    const Beam b = pool[bi];
+    const Aw w = awOf(b);
+    while (trail != 0) {
+      const int d = 63 - __builtin_clzll(trail);
+      for (; depth > d; --depth) {
+        node = sv->parent[node];
+        cnt->v[sClimbs]++;
+      }
+      trail &= (uint64_t(1) << d) - 1;
+      const KdNodeD kn = sv->nodes[node];
+      const uint32_t cs = b.sgn[kn.data & 3u] > 0 ? (kn.data >> 2) : node + 1;
+      const Box bx = sv->nodeBox[cs];
+      if (overlaps(b, w, bx)) {
+        pushWork(Work{bi, cs, d + 1, trail, bx});
+        return;
+      }
+    }
+    finalize(bi);
+  }
+
+  __device__ void rootPiece(const Poly& p, const int sgn[3], uint32_t node0, const Box& rootBox) {
+    Beam b;
+    if (!geom(p, sgn, b)) return;
+    const int idx = alloc();
+    if (idx < 0) return;
+    pool[idx] = b;
+    cnt->v[sRoots]++;
+    if (overlaps(b, awOf(b), rootBox))
+      pushWork(Work{idx, node0, 0, 0, rootBox});
+    else
+      finalize(idx);
+  }
+
+  // presplit (beam_core.h:487-546) by the ray-direction sign lines per axis.
+  __device__ void presplit(const Poly& root, uint32_t node0, const Box& rootBox) {
+    struct Item {
+      Poly p;
+      int axis;
+      int sgn[3];
+    };
+    Item stack[8];
+    int sp = 0;
+    stack[sp++] = Item{root, 0, {1, 1, 1}};
+    while (sp > 0 && !overflow) {
+      Item it = stack[--sp];
+      const Poly& p = it.p;
+      if (p.n < 3) continue;
+      if (it.axis == 3) {
+        for (int k = 1; k + 1 < p.n; k += 2) {
+          Poly qd;
+          qd.n = 0;
+          qd.push(p.x[0], p.y[0]);
+          qd.push(p.x[k], p.y[k]);
+          qd.push(p.x[k + 1], p.y[k + 1]);
+          if (k + 2 < p.n) qd.push(p.x[k + 2], p.y[k + 2]);
+          rootPiece(qd, it.sgn, node0, rootBox);
+        }
+        continue;
+      }
+      const int axis = it.axis;
+      const Line L = makeLine(at(Q.u, axis), at(Q.v, axis), at(Q.d, axis), ext);
+      const int c = sides(p, L, eps);
+      if (c == 3) {
+        Poly in, outp;
+        splitPoly(p, L, eps, in, outp);
+        cleanPoly(in, eps);
+        cleanPoly(outp, eps);
+        if (sp + 2 > 8) {
+          overflow = true;
+          return;
+        }
+        Item a = it, b = it;
+        a.p = in;
+        a.axis = axis + 1;
+        a.sgn[axis] = 1;
+        b.p = outp;
+        b.axis = axis + 1;
+        b.sgn[axis] = -1;
+        stack[sp++] = b;
+        stack[sp++] = a;
+        cnt->v[sPresplit]++;
+      } else if (c != 0) {
+        it.axis = axis + 1;
+        it.sgn[axis] = (c & 1) ? 1 : -1;
+        if (sp >= 8) {
+          overflow = true;
+          return;
+        }
+        stack[sp++] = it;
+      }
+    }
+  }
+
+  // BeamCore::trace with the restart trail (beam_core.h:569-661).
+  __device__ void trace(const Poly& rootIn, uint32_t node0, const Box& rootBox) {
+    const double e0 = rootIn.extent();
+    setup(0.0 > e0 ? 0.0 : e0);
+    Poly p = rootIn;
+    if (p.n >= 3) {
+      if (p.area() < 0) p.reverse();
+      presplit(p, node0, rootBox);
+    }
+    while (nWork > 0 && !overflow) {
+      const Work wk = work[--nWork];
+      uint32_t node = wk.node;
+      Box box = wk.box;
+      int depth = wk.depth;
+      uint64_t trail = wk.trail;
+      const Beam B = pool[wk.beam];
+      const Aw w = awOf(B);
+      for (;;) {
+        const KdNodeD kn = sv->nodes[node];
+        if ((kn.data & 3u) == 3u) break;
+        cnt->v[sKd]++;
+        const int d = depth++;
+        const int a = int(kn.data & 3u);
+        const float s = kn.split;
+        const double Ns = double(B.sgn[a]) * Q.rel(double(s), a);
+        const uint32_t left = node + 1, right = kn.data >> 2;
+        Box lb = box, rb = box;
+        setComp(lb.hi, a, s);
+        setComp(rb.lo, a, s);
+        const bool toRight = B.sgn[a] > 0;
+        const uint32_t cs = toRight ? right : left, co = toRight ? left : right;
+        if (Ns <= 0) {
+          node = cs;
+          box = toRight ? rb : lb;
+          continue;
+        }
+        const int dec = decide(B, w, a, Ns, box);
+        if (dec == 0) {
+          node = co;
+          box = toRight ? lb : rb;
+        } else if (dec == 1) {
+          node = cs;
+          box = toRight ? rb : lb;
+        } else {
+          trail |= uint64_t(1) << d;
+          node = co;
+          box = toRight ? lb : rb;
+        }
+      }
+      const KdNodeD leaf = sv->nodes[node];
+      if ((leaf.data >> 2) == 0) {
+        cnt->v[sLeaf]++;
+        advance(wk.beam, node, depth, trail);
+        continue;
+      }
+      processLeaf(wk.beam, leaf);
+      const int n = nCur;
+      for (int k = 0; k < n && !overflow; ++k) advance(cur[k], node, depth, trail);
+    }
+  }
+
+  // The unit normal of a triangle, in double as Scene::finalize computes it.
+  __device__ static V normalOf(V a, V b, V c) {
+    const V e1 = b - a, e2 = c - a;
+    const V nn = crs(e1, e2);
+    const double l = sqrt(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+    return l > 0 ? V{nn.x / l, nn.y / l, nn.z / l} : nn;
+  }
+};
+
+}  // namespace fp64
