@@ -374,7 +374,7 @@ def test_fp64_soft_shadows_match_the_double_build(require_gpu, name, scale, side
     d = refd.soft(sp["view"], W, H, light, exact=True, mailbox=False, trail=True, threads=16)
     refd.free()
     m = (d["vis"] >= 0) & (g["vis"] >= 0) & (g["tri"] == d["tri"])
-    assert m.mean() > 0.9
+    assert m.sum() >= 0.9 * (d["vis"] >= 0).sum()  # the same primary hit on nearly every lit pixel
     err = np.abs(g["vis"].astype(np.float64) - d["vis"])[m]
     e_err = np.abs(g["E"].astype(np.float64) - d["E"])[m] / max(float(np.abs(d["E"][m]).max()), 1e-30)
     print(f"fp64 {name}: vis max {err.max():.2e} mean {err.mean():.2e}; E max {e_err.max():.2e}", flush=True)
