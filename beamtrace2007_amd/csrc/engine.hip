@@ -907,35 +907,7 @@ __device__ float formFactor(F3 x, F3 n, const F3 (&poly)[4], int count) {
   return float((sum < 0 ? -sum : sum) * 0.5);
 }
 
-// polygonFormFactorG as the double build computes it (Real = double): the
-// guarded mode's Lambert integral, every step in double from the double
-// shading point and the piece's corners formed in double on the light plane.
-struct D3 {
-  double x, y, z;
-};
-__device__ __forceinline__ D3 d3sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ double d3dot(D3 a, D3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ D3 d3cross(D3 a, D3 b) {
-  return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-__device__ double formFactorD(D3 x, D3 n, const D3 (&poly)[4], int count) {
-  double sum = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    if (i >= count) break;
-    const D3 a = d3sub(poly[i], x), b = d3sub(i + 1 < count ? poly[(i + 1) & 3] : poly[0], x);
-    const double la = sqrt(d3dot(a, a)), lb = sqrt(d3dot(b, b));
-    if (la == 0 || lb == 0) continue;
-    double cc = d3dot(a, b) / (la * lb);
-    cc = cc < -1 ? -1 : (cc > 1 ? 1 : cc);
-    const double theta = acos(cc);
-    const D3 g = d3cross(a, b);
-    const double lg = sqrt(d3dot(g, g));
-    if (lg == 0) continue;
-    sum += theta * d3dot(g, n) / lg;
-  }
-  return (sum < 0 ? -sum : sum) * 0.5;
-}
+#include "trace_fp64.h"
 
 // Signed area of a beam's 3- or 4-gon, as PolyN::area (beam_geom.h:27-33)
 // sums it. A triangle's fourth corner repeats its third; the extra term
@@ -2406,24 +2378,22 @@ __device__ bool rayNearest(const DevScene& sv, uint32_t node0, const Box& bounds
 
 // The hit point in double on the plane of the triangle's float corners, as
 // the float pair X + Xlo: o + d t with n = (v1 - v0) x (v2 - v0) and
-// t = n.(v0 - o) / n.d, every step in double (the guarded mode's receiver
+// t = n.(v0 - o) / n.d, every step in double (the guarded modes' receiver
 // point; the float pipeline's o + d t can sit ~1e-7 of the scene off that
-// plane, far more than the double build's apex offset).
-__device__ void hitPointD(const LeafTri& t, F3 o, F3 d, F3& X, F3& Xlo) {
+// plane, far more than the double build's apex offset). d: the ray direction
+// in double (the float ray's, or the FP64 mode's double sum).
+__device__ void hitPointD(const LeafTri& t, F3 o, double dx, double dy, double dz, F3& X, F3& Xlo) {
   const double v0x = t.v[0], v0y = t.v[1], v0z = t.v[2];
   const double ax = double(t.v[3]) - v0x, ay = double(t.v[4]) - v0y, az = double(t.v[5]) - v0z;
   const double bx = double(t.v[6]) - v0x, by = double(t.v[7]) - v0y, bz = double(t.v[8]) - v0z;
   const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-  const double den = (nx * double(d.x) + ny * double(d.y)) + nz * double(d.z);
+  const double den = (nx * dx + ny * dy) + nz * dz;
   const double num = (nx * (v0x - double(o.x)) + ny * (v0y - double(o.y))) + nz * (v0z - double(o.z));
   const double th = den != 0 ? num / den : 0.0;
-  const double px = double(o.x) + double(d.x) * th, py = double(o.y) + double(d.y) * th,
-               pz = double(o.z) + double(d.z) * th;
+  const double px = double(o.x) + dx * th, py = double(o.y) + dy * th, pz = double(o.z) + dz * th;
   X = f3(float(px), float(py), float(pz));
   Xlo = f3(float(px - double(X.x)), float(py - double(X.y)), float(pz - double(X.z)));
 }
-
-#include "trace_fp64.h"
 
 // The job of a batch pixel: the last job whose first pixel is <= k.
 __device__ int jobOfPixel(const Params& p, int k) {
@@ -2438,9 +2408,8 @@ __device__ int jobOfPixel(const Params& p, int k) {
   return lo;
 }
 
-// One pixel of FP64 soft shadows: beamLightVisibility (render.cpp:279-303)
-// with lightBeamSetup and lightCenterG (soft_core.h:38-84) in double; false
-// when the tracer's pools overflowed (nothing written).
+// One pixel of FP64 soft shadows (fp64::shadePoint at the pixel's double hit
+// point); false when the tracer's pools overflowed (nothing written).
 template <class TR>
 __device__ bool fp64Pixel(const Params& p, const SoftJob& J, int pix, TR& tr, Cnt& cnt) {
   using fp64::V;
@@ -2455,78 +2424,33 @@ __device__ bool fp64Pixel(const Params& p, const SoftJob& J, int pix, TR& tr, Cn
                         double(p.pixX[3 * pix + 1]) + double(p.pixXlo[3 * pix + 1]),
                         double(p.pixX[3 * pix + 2]) + double(p.pixXlo[3 * pix + 2]));
   const LeafTri ref = p.sv.refs[uint32_t(p.pixRef[pix])];
-  const V nd = TR::normalOf(fp64::vmk(ref.v[0], ref.v[1], ref.v[2]), fp64::vmk(ref.v[3], ref.v[4], ref.v[5]),
-                            fp64::vmk(ref.v[6], ref.v[7], ref.v[8]));
-  const V nf = fp64::vmk(p.pixN[3 * pix], p.pixN[3 * pix + 1], p.pixN[3 * pix + 2]);  // faces the eye
-  const V ns = fp64::dot(nd, nf) < 0 ? fp64::neg(nd) : nd;
-  const V c = fp64::vf(J.lc), U = fp64::vf(J.lU), Vv = fp64::vf(J.lV);
-  const V uv = fp64::crs(U, Vv);
-  const double A = fp64::len(uv);
-  const V nL = fp64::nrmz(uv);
-  const bool front = fp64::dot(x - c, nL) > 0;
-  double G0 = 0;
-  if (front) {
-    const V w = c - x;
-    const double r2 = fp64::dot(w, w), rl = sqrt(r2);
-    const V wn = fp64::vmk(w.x / rl, w.y / rl, w.z / rl);
-    const double ci = fp64::dot(ns, wn), cl = -fp64::dot(nL, wn);
-    G0 = (ci > 0 ? ci : 0) * (cl > 0 ? cl : 0) / r2;
-  }
-  p.Eu[pix] = float(A * G0);
-  p.vis[pix] = 0;
-  p.E[pix] = 0;
-  if (!front) return true;
-  tr.Q = fp64::Plane::make(x + ns * double(J.offset), c, U, Vv);
-  const double hzc = fp64::dot(ns, c - x);
-  const V hu = U * 0.5, hv = Vv * 0.5;
-  const V corners[4] = {(c - hu) - hv, (c + hu) - hv, (c + hu) + hv, (c - hu) + hv};
-  fp64::Poly quad;
-  quad.n = 0;
-  for (int i = 0; i < 4; ++i) {
-    double qx, qy;
-    tr.Q.coords(corners[i], qx, qy);
-    quad.push(qx, qy);
-  }
-  if (quad.area() < 0) quad.reverse();
-  const double lightArea = quad.area();
-  const fp64::Line hz{fp64::dot(ns, tr.Q.u), fp64::dot(ns, tr.Q.v), hzc};
-  int above = 0;
-  for (int i = 0; i < quad.n; ++i) above += hz.at(quad.x[i], quad.y[i]) > 0;
-  if (above == 0) return true;
-  fp64::Poly root = quad;
-  if (above != quad.n) {
-    fp64::Poly rest;
-    fp64::splitPoly(quad, hz, 0, root, rest);
-  }
-  if (!(root.n >= 3 && root.area() > 0)) return true;
-  tr.sv = &p.sv;
-  tr.exclude = T;
-  tr.x = x;
-  tr.ns = ns;
-  tr.exact = p.exact != 0;
-  tr.cnt = &cnt;
-  tr.trace(root, J.node0, J.bounds);
-  if (tr.overflow) return false;
-  double vis = tr.missArea / lightArea;
-  vis = vis < 0 ? 0.0 : (vis > 1 ? 1.0 : vis);
+  const V nf = fp64::vmk(p.pixN[3 * pix], p.pixN[3 * pix + 1], p.pixN[3 * pix + 2]);
+  double vis, E, Eu;
+  if (!fp64::shadePoint(tr, p.sv, x, ref, nf, fp64::vf(J.lc), fp64::vf(J.lU), fp64::vf(J.lV), double(J.offset),
+                        p.exact != 0, J.node0, J.bounds, cnt, vis, E, Eu))
+    return false;
   p.vis[pix] = float(vis);
-  p.E[pix] = p.exact ? float(tr.G) : float(A * G0 * vis);
+  p.E[pix] = float(E);
+  p.Eu[pix] = float(Eu);
   return true;
 }
 
-// FP64 soft shadows over `count` pixels (list[i], or i without a list), one
-// pixel per thread with pools of Pn beams and Kn work items in private
-// memory. A pixel whose pools overflow is appended to `retry` (traced again
-// with larger pools); without a retry list it is counted in *failed.
+// FP64 soft shadows over `count` pixels (list[i], or i without a list): each
+// thread takes pixels i, i + threads, ... with pools of Pn beams and Kn work
+// items in its region of `mem` (Tracer::kWords words per thread, interleaved
+// across the block's 64 threads; the grid is no larger than `mem` holds). A
+// pixel whose pools overflow is appended to `retry` (traced again with larger
+// pools); without a retry list it is counted in *failed.
 template <int Pn, int Kn>
-__global__ void __launch_bounds__(64) softFp64Kernel(Params p, const int* list, int count, int* retry,
+__global__ void __launch_bounds__(64) softFp64Kernel(Params p, const int* list, int count, uint64_t* mem, int* retry,
                                                      int* retryCount, int* failed) {
+  using TR = fp64::Tracer<Pn, Kn>;
   Cnt cnt;
   cnt.zero();
-  const int i = int(blockIdx.x) * 64 + int(threadIdx.x);
-  if (i < count) {
+  TR tr;
+  tr.m = fp64::WordMem{mem + size_t(blockIdx.x) * size_t(TR::kWords) * 64, 64, int(threadIdx.x)};
+  for (int i = int(blockIdx.x) * 64 + int(threadIdx.x); i < count; i += int(gridDim.x) * 64) {
     const int pix = list ? list[i] : i;
-    fp64::Tracer<Pn, Kn> tr;
     if (!fp64Pixel(p, p.jobs[jobOfPixel(p, pix)], pix, tr, cnt)) {
       if (retry)
         retry[atomicAdd(retryCount, 1)] = pix;
@@ -2572,17 +2496,31 @@ __global__ void __launch_bounds__(kRayBlock) primaryRayKernel(Params p) {
       triOut[k] = th_tri;
       if (p.pixRef) p.pixRef[k] = int(th_ref);
       F3 hp = J.eye + d * th;
+      const float4 nn = p.sv.triNV[2 * size_t(th_tri)];
+      F3 nrm = f3(nn.x, nn.y, nn.z);
+      bool away = dot(nrm, d) > 0;
       if (p.pixXlo) {
+        // FP64 mode (pixRef set): the ray the double build shoots, the pixel
+        // centre and direction summed in double from the camera's floats (the
+        // float sum moves the receiver point by ~1e-7 of the distance, which
+        // shows as up to ~2e-5 of visible fraction at sharp penumbrae)
+        double dx = d.x, dy = d.y, dz = d.z;
+        if (p.pixRef) {
+          const double qxd = (2 * (double(x) + 0.5) / double(J.W) - 1) * double(J.halfW);
+          const double qyd = (2 * (double(y) + 0.5) / double(J.H) - 1) * double(J.halfH);
+          dx = (double(J.fwd.x) + double(J.right.x) * qxd) + double(J.down.x) * qyd;
+          dy = (double(J.fwd.y) + double(J.right.y) * qxd) + double(J.down.y) * qyd;
+          dz = (double(J.fwd.z) + double(J.right.z) * qxd) + double(J.down.z) * qyd;
+          away = (double(nn.x) * dx + double(nn.y) * dy) + double(nn.z) * dz > 0;
+        }
         F3 lo;
-        hitPointD(p.sv.refs[th_ref], J.eye, d, hp, lo);
+        hitPointD(p.sv.refs[th_ref], J.eye, dx, dy, dz, hp, lo);
         float* XL = const_cast<float*>(p.pixXlo);
         XL[3 * k] = lo.x;
         XL[3 * k + 1] = lo.y;
         XL[3 * k + 2] = lo.z;
       }
-      const float4 nn = p.sv.triNV[2 * size_t(th_tri)];
-      F3 nrm = f3(nn.x, nn.y, nn.z);
-      if (dot(nrm, d) > 0) nrm = -nrm;
+      if (away) nrm = -nrm;
       X[3 * k] = hp.x;
       X[3 * k + 1] = hp.y;
       X[3 * k + 2] = hp.z;
@@ -2648,7 +2586,8 @@ class Engine {
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
-  DevBuf pixRef, fpList;  // FP64 soft shadows: hit leaf reference per pixel; pixels to trace again, counters
+  DevBuf pixRef, fpList, fpMem;  // FP64 soft shadows: hit leaf reference per pixel; pixels to trace again,
+                                 // counters; the tracers' arrays
   DevBuf rasterImg, rasterRows;  // device rasterization: image; row counts, span offsets and tile sums
   void* noAllocScratch = nullptr;  // this device's static rasterization scratch (null: unavailable)
   WState* hostState = nullptr;  // pinned
@@ -2700,7 +2639,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs, &pixRef, &fpList};
+                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs, &pixRef, &fpList, &fpMem};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -3202,24 +3141,35 @@ Options& engineOptions(Engine* e) { return e->opt; }
 namespace {
 
 // FP64 soft shadows of every pixel of a batch (after its primary pass): pools
-// of 64 beams first, then 384 for the pixels that need more.
+// of 64 beams first, then 384 for the pixels that need more. The tracers'
+// arrays take fpMem: the first pass runs at most 4 blocks of 64 threads per
+// CU (one wave per SIMD; 1 MB per block), the second as many blocks as that
+// memory holds (3.8 MB per block), each thread looping over its pixels.
 int softFp64(Engine& e, Params& p, size_t np, std::string& err) {
+  using Small = fp64::Tracer<64, 64>;
+  using Large = fp64::Tracer<384, 256>;
+  constexpr size_t kSmallBlock = size_t(Small::kWords) * 64 * 8, kLargeBlock = size_t(Large::kWords) * 64 * 8;
+  const size_t smallBlocks = std::min((np + 63) / 64, size_t(std::max(e.cus, 1)) * 4);
+  int rc = e.ensure(e.fpMem, std::max(smallBlocks * kSmallBlock, kLargeBlock), err);
+  if (rc) return rc;
+  uint64_t* mem = static_cast<uint64_t*>(e.fpMem.ptr);
   int* retry = static_cast<int*>(e.fpList.ptr);
   int* counters = retry + np;  // [0] pixels to trace again, [1] pixels that overflowed twice
   clearInt<<<1, 64, 0, e.stream>>>(counters);
   clearInt<<<1, 64, 0, e.stream>>>(counters + 1);
   int lt = e.beginLaunch(1);
-  softFp64Kernel<64, 64><<<unsigned((np + 63) / 64), 64, 0, e.stream>>>(p, nullptr, int(np), retry, counters,
-                                                                        counters + 1);
+  softFp64Kernel<64, 64><<<unsigned(smallBlocks), 64, 0, e.stream>>>(p, nullptr, int(np), mem, retry, counters,
+                                                                    counters + 1);
   e.endLaunch(lt);
   BTG_TRY(hipGetLastError());
   int again = 0;
   BTG_TRY(hipMemcpyAsync(&again, counters, 4, hipMemcpyDeviceToHost, e.stream));
   BTG_TRY(hipStreamSynchronize(e.stream));
   if (again > 0) {
+    const size_t largeBlocks = std::min((size_t(again) + 63) / 64, e.fpMem.bytes / kLargeBlock);
     lt = e.beginLaunch(1);
-    softFp64Kernel<384, 256><<<unsigned((again + 63) / 64), 64, 0, e.stream>>>(p, retry, again, nullptr, nullptr,
-                                                                              counters + 1);
+    softFp64Kernel<384, 256><<<unsigned(largeBlocks), 64, 0, e.stream>>>(p, retry, again, mem, nullptr, nullptr,
+                                                                        counters + 1);
     e.endLaunch(lt);
     BTG_TRY(hipGetLastError());
     int failed = 0;

@@ -10,13 +10,50 @@
 // computed in double as Scene::finalize does (src/scene/scene.cpp:18-33).
 // The tree is the float build's (the double build makes its own): kd
 // decisions are conservative, so the visible fraction is the same up to the
-// double rounding of the piece areas, about 1e-12 of the light.
+// slivers the other split order drops and the double rounding of the piece
+// areas, below 1e-10 of the light.
 //
-// Every beam of a pixel lives in the thread's private memory (bounded pools:
-// a pixel that needs more is traced again with larger pools; results do not
-// depend on the pool sizes). For an any-hit query with the light plane as the
+// Every beam of a pixel lives in the thread's region of a device buffer
+// (WordMem; bounded pools: a pixel that needs more is traced again with
+// larger pools; results do not depend on the pool sizes). For an any-hit query with the light plane as the
 // far plane, a beam carries only its corners and ray-direction signs: the far
 // plane is the same for all of them and |w| is recomputed from the corners.
+// FP64_HD marks every function: device code in the library; a host build of
+// the same code (a test harness) defines it as __host__ __device__.
+#ifndef FP64_HD
+#define FP64_HD __device__
+#endif
+
+// polygonFormFactorG as the double build computes it (Real = double): the
+// FP64 modes' Lambert integral, every step in double from the double shading
+// point and the piece's corners formed in double on the light plane.
+struct D3 {
+  double x, y, z;
+};
+FP64_HD inline D3 d3sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+FP64_HD inline double d3dot(D3 a, D3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+FP64_HD inline D3 d3cross(D3 a, D3 b) {
+  return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+FP64_HD inline double formFactorD(D3 x, D3 n, const D3 (&poly)[4], int count) {
+  double sum = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= count) break;
+    const D3 a = d3sub(poly[i], x), b = d3sub(i + 1 < count ? poly[(i + 1) & 3] : poly[0], x);
+    const double la = sqrt(d3dot(a, a)), lb = sqrt(d3dot(b, b));
+    if (la == 0 || lb == 0) continue;
+    double cc = d3dot(a, b) / (la * lb);
+    cc = cc < -1 ? -1 : (cc > 1 ? 1 : cc);
+    const double theta = acos(cc);
+    const D3 g = d3cross(a, b);
+    const double lg = sqrt(d3dot(g, g));
+    if (lg == 0) continue;
+    sum += theta * d3dot(g, n) / lg;
+  }
+  return (sum < 0 ? -sum : sum) * 0.5;
+}
+
 namespace fp64 {
 
 constexpr double kEps = 1e-9;      // fuzzy split epsilon, relative to the root's extent
@@ -28,48 +65,48 @@ constexpr double kCopRel = 1e-8;   // coplanar: offset
 struct V {
   double x, y, z;
 };
-__device__ __forceinline__ V vmk(double x, double y, double z) { return V{x, y, z}; }
-__device__ __forceinline__ V vf(F3 a) { return V{double(a.x), double(a.y), double(a.z)}; }
-__device__ __forceinline__ V operator+(V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ V operator-(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ V operator*(V a, double s) { return V{a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ V neg(V a) { return V{-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V crs(V a, V b) {
+FP64_HD inline V vmk(double x, double y, double z) { return V{x, y, z}; }
+FP64_HD inline V vf(F3 a) { return V{double(a.x), double(a.y), double(a.z)}; }
+FP64_HD inline V operator+(V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; }
+FP64_HD inline V operator-(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
+FP64_HD inline V operator*(V a, double s) { return V{a.x * s, a.y * s, a.z * s}; }
+FP64_HD inline V neg(V a) { return V{-a.x, -a.y, -a.z}; }
+FP64_HD inline double dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+FP64_HD inline V crs(V a, V b) {
   return V{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ double len(V a) { return sqrt(dot(a, a)); }
-__device__ __forceinline__ V nrmz(V a) {
+FP64_HD inline double len(V a) { return sqrt(dot(a, a)); }
+FP64_HD inline V nrmz(V a) {
   const double l = len(a);
   return l > 0 ? V{a.x / l, a.y / l, a.z / l} : a;
 }
-__device__ __forceinline__ double at(V a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-__device__ __forceinline__ double absS(double v) { return v < 0 ? -v : v; }
+FP64_HD inline double at(V a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+FP64_HD inline double absS(double v) { return v < 0 ? -v : v; }
 
 // Four corner lanes (R4, src/core/simd4.h).
 struct L4 {
   double v[4];
 };
-__device__ __forceinline__ L4 affine(const L4& x, double a, const L4& y, double b, double c) {
+FP64_HD inline L4 affine(const L4& x, double a, const L4& y, double b, double c) {
   L4 r;
 #pragma unroll
   for (int i = 0; i < 4; ++i) r.v[i] = x.v[i] * a + y.v[i] * b + c;
   return r;
 }
-__device__ __forceinline__ L4 mul(const L4& a, double s) {
+FP64_HD inline L4 mul(const L4& a, double s) {
   L4 r;
 #pragma unroll
   for (int i = 0; i < 4; ++i) r.v[i] = a.v[i] * s;
   return r;
 }
 // lhs < rhs - (|lhs| + |rhs|) k on every lane; the same with > and +.
-__device__ __forceinline__ bool allLess(const L4& a, const L4& b, double k) {
+FP64_HD inline bool allLess(const L4& a, const L4& b, double k) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     if (!(a.v[i] < b.v[i] - (fabs(a.v[i]) + fabs(b.v[i])) * k)) return false;
   return true;
 }
-__device__ __forceinline__ bool allGreater(const L4& a, const L4& b, double k) {
+FP64_HD inline bool allGreater(const L4& a, const L4& b, double k) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
     if (!(a.v[i] > b.v[i] + (fabs(a.v[i]) + fabs(b.v[i])) * k)) return false;
@@ -80,14 +117,14 @@ __device__ __forceinline__ bool allGreater(const L4& a, const L4& b, double k) {
 struct Poly {
   double x[8], y[8];
   int n;
-  __device__ void push(double px, double py) {
+  FP64_HD void push(double px, double py) {
     if (n < 8) {
       x[n] = px;
       y[n] = py;
       ++n;
     }
   }
-  __device__ double area() const {
+  FP64_HD double area() const {
     double a = 0;
     for (int i = 0; i < n; ++i) {
       const int j = i + 1 == n ? 0 : i + 1;
@@ -95,7 +132,7 @@ struct Poly {
     }
     return a * 0.5;
   }
-  __device__ void reverse() {
+  FP64_HD void reverse() {
     for (int i = 0, j = n - 1; i < j; ++i, --j) {
       const double tx = x[i], ty = y[i];
       x[i] = x[j];
@@ -104,7 +141,7 @@ struct Poly {
       y[j] = ty;
     }
   }
-  __device__ double extent() const {
+  FP64_HD double extent() const {
     double lx = INFINITY, hx = -INFINITY, ly = INFINITY, hy = -INFINITY;
     for (int i = 0; i < n; ++i) {
       lx = x[i] < lx ? x[i] : lx;
@@ -119,11 +156,11 @@ struct Poly {
 
 struct Line {
   double a, b, c;
-  __device__ double at(double px, double py) const { return a * px + b * py + c; }
+  FP64_HD double at(double px, double py) const { return a * px + b * py + c; }
 };
 
 // makeLine (beam_geom.h:65-76).
-__device__ Line makeLine(double a, double b, double c, double ext) {
+FP64_HD inline Line makeLine(double a, double b, double c, double ext) {
   Line L{0, 0, 0};
   const double nrm = sqrt(a * a + b * b);
   if (!(nrm > 0) || nrm * ext * 1e9 < fabs(c)) {
@@ -137,7 +174,7 @@ __device__ Line makeLine(double a, double b, double c, double ext) {
 }
 
 // splitPoly (beam_geom.h:82-114), capacity 8.
-__device__ bool splitPoly(const Poly& p, const Line& L, double eps, Poly& in, Poly& out) {
+FP64_HD inline bool splitPoly(const Poly& p, const Line& L, double eps, Poly& in, Poly& out) {
   in.n = out.n = 0;
   double d[8];
   int cls[8];
@@ -170,7 +207,7 @@ __device__ bool splitPoly(const Poly& p, const Line& L, double eps, Poly& in, Po
 }
 
 // cleanPoly (beam_geom.h:121-150).
-__device__ void cleanPoly(Poly& p, double epsSplit) {
+FP64_HD inline void cleanPoly(Poly& p, double epsSplit) {
   const double e = epsSplit * 1e-3;
   bool changed = true;
   while (changed && p.n >= 3) {
@@ -198,7 +235,7 @@ __device__ void cleanPoly(Poly& p, double epsSplit) {
 }
 
 // classifyPoly (beam_core.h:68-78): bit 0 some vertex inside, bit 1 outside.
-__device__ int sides(const Poly& p, const Line& L, double eps) {
+FP64_HD inline int sides(const Poly& p, const Line& L, double eps) {
   int c = 0;
   for (int i = 0; i < p.n; ++i) {
     const double d = L.at(p.x[i], p.y[i]);
@@ -211,7 +248,7 @@ __device__ int sides(const Poly& p, const Line& L, double eps) {
 }
 
 // splitFive (beam_core.h:80-89): pentagon -> quad (0..3) + triangle (0, 3, 4).
-__device__ void splitFive(const Poly& p, Poly& a, Poly& b) {
+FP64_HD inline void splitFive(const Poly& p, Poly& a, Poly& b) {
   a.n = b.n = 0;
   for (int i = 0; i < 4; ++i) a.push(p.x[i], p.y[i]);
   b.push(p.x[0], p.y[0]);
@@ -222,15 +259,15 @@ __device__ void splitFive(const Poly& p, Poly& a, Poly& b) {
 // BeamPlane (beam_types.h:14-52).
 struct Plane {
   V O, p0, u, v, n, d;
-  __device__ V rel(V p) const { return p - O; }
-  __device__ double rel(double s, int a) const { return s - at(O, a); }
-  __device__ V point(double qx, double qy) const { return (p0 + u * qx) + v * qy; }
-  __device__ void coords(V p, double& qx, double& qy) const {
+  FP64_HD V rel(V p) const { return p - O; }
+  FP64_HD double rel(double s, int a) const { return s - at(O, a); }
+  FP64_HD V point(double qx, double qy) const { return (p0 + u * qx) + v * qy; }
+  FP64_HD void coords(V p, double& qx, double& qy) const {
     const V r = p - p0;
     qx = dot(r, u);
     qy = dot(r, v);
   }
-  __device__ static Plane make(V O, V p0, V U, V Vv) {
+  FP64_HD static Plane make(V O, V p0, V U, V Vv) {
     Plane P;
     P.O = O;
     P.p0 = p0;
@@ -274,11 +311,69 @@ struct Work {
   Box box;
 };
 
+// A piece of the presplit's stack.
+struct PreItem {
+  Poly p;
+  int axis;
+  int sgn[3];
+};
+
+// The tracer's arrays live in device memory that the library owns, not in
+// private memory: the runtime reserves a queue's private memory for every
+// wave the device can hold, which at tens of KB per thread exhausts it once
+// a few contexts have run. Records are interleaved by 8-byte word across the
+// threads that share a region (a wave: 64; a host build: 1): word w of thread
+// `lane` is base[w * threads + lane], so a wave's same-word accesses are one
+// contiguous 512-byte run, the layout the hardware gives private memory.
+struct WordMem {
+  uint64_t* base;
+  int threads, lane;
+  template <class T>
+  FP64_HD T load(uint32_t w) const {
+    static_assert(sizeof(T) % 8 == 0, "records are whole words");
+    uint64_t u[sizeof(T) / 8];
+#pragma unroll
+    for (int i = 0; i < int(sizeof(T) / 8); ++i) u[i] = base[size_t(w + uint32_t(i)) * size_t(threads) + size_t(lane)];
+    T v;
+    __builtin_memcpy(&v, u, sizeof(T));
+    return v;
+  }
+  template <class T>
+  FP64_HD void store(uint32_t w, const T& v) const {
+    static_assert(sizeof(T) % 8 == 0, "records are whole words");
+    uint64_t u[sizeof(T) / 8];
+    __builtin_memcpy(u, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < int(sizeof(T) / 8); ++i) base[size_t(w + uint32_t(i)) * size_t(threads) + size_t(lane)] = u[i];
+  }
+  FP64_HD int loadInt(uint32_t w) const {
+    return int32_t(uint32_t(base[size_t(w) * size_t(threads) + size_t(lane)]));
+  }
+  FP64_HD void storeInt(uint32_t w, int v) const {
+    base[size_t(w) * size_t(threads) + size_t(lane)] = uint64_t(uint32_t(v));
+  }
+};
+
 // One pixel's query, depth-first. P: beams per pixel, K: pending work items,
 // S: pieces per clip list.
 template <int P, int K>
 struct Tracer {
   static constexpr int S = 16;
+  // the thread's arrays: word offsets in its WordMem region
+  static constexpr uint32_t wBeam = sizeof(Beam) / 8, wWork = sizeof(Work) / 8, wPoly = sizeof(Poly) / 8,
+                            wItem = sizeof(PreItem) / 8;
+  static constexpr uint32_t oPool = 0;                      // P beams
+  static constexpr uint32_t oFree = oPool + P * wBeam;      // P free beam slots
+  static constexpr uint32_t oCurA = oFree + P;              // P + P: the beams left against the current
+  static constexpr uint32_t oCurB = oCurA + P;              //   triangle of a leaf and those for the next
+  static constexpr uint32_t oWork = oCurB + P;              // K pending work items
+  static constexpr uint32_t oList = oWork + K * wWork;      // 2 x S clip pieces (outside, inside)
+  static constexpr uint32_t oClip = oList + 2 * S * wPoly;  // clip's stack: 8 polygons, 8 line indices
+  static constexpr uint32_t oPre = oClip + 8 * wPoly + 8;   // presplit's stack: 8 items
+  static constexpr uint32_t kWords = oPre + 8 * wItem;      // words per thread
+  static_assert(sizeof(Beam) % 8 == 0 && sizeof(Work) % 8 == 0 && sizeof(Poly) % 8 == 0 && sizeof(PreItem) % 8 == 0,
+                "records are whole words");
+  WordMem m;
   const DevScene* sv;
   Plane Q;
   int exclude;
@@ -288,15 +383,17 @@ struct Tracer {
   double missArea, G;
   bool overflow;
   Cnt* cnt;
-  Beam pool[P];
-  int freeL[P], nFree, nPool;
-  int cur[P], nCur, nxt[P], nNxt;
-  Work work[K];
+  int nFree, nPool;
+  uint32_t oCur, oNxt;  // oCurA / oCurB, swapped after each triangle of a leaf
+  int nCur, nNxt;
   int nWork;
-  Poly lists[2][S];
   int nList[2];
 
-  __device__ void setup(double extent) {
+  FP64_HD Beam beam(int b) const { return m.load<Beam>(oPool + uint32_t(b) * wBeam); }
+  FP64_HD void putBeam(int b, const Beam& v) const { m.store(oPool + uint32_t(b) * wBeam, v); }
+  FP64_HD Poly piece(int l, int k) const { return m.load<Poly>(oList + uint32_t(l * S + k) * wPoly); }
+
+  FP64_HD void setup(double extent) {
     ext = extent > 0 ? extent : 1.0;
     eps = kEps * ext;
     minArea = eps * eps;
@@ -304,35 +401,37 @@ struct Tracer {
     missArea = G = 0;
     overflow = false;
     nFree = nPool = nCur = nNxt = nWork = 0;
+    oCur = oCurA;
+    oNxt = oCurB;
   }
 
-  __device__ int alloc() {
-    if (nFree > 0) return freeL[--nFree];
+  FP64_HD int alloc() {
+    if (nFree > 0) return m.loadInt(oFree + uint32_t(--nFree));
     if (nPool >= P) {
       overflow = true;
       return -1;
     }
     return nPool++;
   }
-  __device__ void release(int b) {
-    if (nFree < P) freeL[nFree++] = b;
+  FP64_HD void release(int b) {
+    if (nFree < P) m.storeInt(oFree + uint32_t(nFree++), b);
   }
-  __device__ void pushWork(const Work& w) {
+  FP64_HD void pushWork(const Work& w) {
     if (nWork >= K) {
       overflow = true;
       return;
     }
-    work[nWork++] = w;
+    m.store(oWork + uint32_t(nWork++) * wWork, w);
   }
-  __device__ void pushNext(int b) {
+  FP64_HD void pushNext(int b) {
     if (nNxt >= P) {
       overflow = true;
       return;
     }
-    nxt[nNxt++] = b;
+    m.storeInt(oNxt + uint32_t(nNxt++), b);
   }
 
-  __device__ Aw awOf(const Beam& b) const {
+  FP64_HD Aw awOf(const Beam& b) const {
     Aw r;
     for (int a = 0; a < 3; ++a) {
       const L4 w = affine(b.qx, at(Q.u, a), b.qy, at(Q.v, a), at(Q.d, a));
@@ -344,10 +443,10 @@ struct Tracer {
     return r;
   }
   // The light plane as every beam's far plane (setStatus, beam_core.h:225-240).
-  __device__ double farC() const { return absS(farCq); }
+  FP64_HD double farC() const { return absS(farCq); }
 
   // decide (beam_core.h:139-164).
-  __device__ int decide(const Beam& b, const Aw& w, int a, double Ns, const Box& bx) const {
+  FP64_HD int decide(const Beam& b, const Aw& w, int a, double Ns, const Box& bx) const {
     const L4& awa = w.aw[a];
     for (int k = 1; k <= 2; ++k) {
       const int c = a + k >= 3 ? a + k - 3 : a + k;
@@ -365,7 +464,7 @@ struct Tracer {
   }
 
   // overlaps (beam_core.h:166-195).
-  __device__ bool overlaps(const Beam& b, const Aw& w, const Box& bx) const {
+  FP64_HD bool overlaps(const Beam& b, const Aw& w, const Box& bx) const {
     double nmin[3], nmax[3];
     for (int c = 0; c < 3; ++c) {
       const double sg = double(b.sgn[c]);
@@ -385,7 +484,7 @@ struct Tracer {
   }
 
   // beamGeom (beam_core.h:200-221): false (and the area dropped) for a sliver.
-  __device__ bool geom(const Poly& p, const int sgn[3], Beam& b) {
+  FP64_HD bool geom(const Poly& p, const int sgn[3], Beam& b) {
     Poly c = p;
     cleanPoly(c, eps);
     const double A = c.n >= 3 ? c.area() : 0.0;
@@ -399,7 +498,7 @@ struct Tracer {
   }
 
   // computeTriInfo + triSetup (beam_core.h:242-286) of a leaf triangle.
-  __device__ Tri setupTri(const LeafTri& t) const {
+  FP64_HD Tri setupTri(const LeafTri& t) const {
     Tri ti;
     ti.ok = false;
     V p[3];
@@ -428,7 +527,7 @@ struct Tracer {
   }
 
   // closerLine (beam_core.h:302-317) against the light plane.
-  __device__ Line farLine(const Tri& T) const {
+  FP64_HD Line farLine(const Tri& T) const {
     const double AF[3] = {0, 0, farCq}, CF = farCq;
     const double cosang = dot(T.n, Q.n);
     const double aC = absS(T.C), aF = absS(CF);
@@ -442,7 +541,7 @@ struct Tracer {
   }
 
   // separated (beam_core.h:321-342).
-  __device__ bool separated(const Beam& b, const Tri& ti) const {
+  FP64_HD bool separated(const Beam& b, const Tri& ti) const {
     const L4 wx = affine(b.qx, Q.u.x, b.qy, Q.v.x, Q.d.x);
     const L4 wy = affine(b.qx, Q.u.y, b.qy, Q.v.y, Q.d.y);
     const L4 wz = affine(b.qx, Q.u.z, b.qy, Q.v.z, Q.d.z);
@@ -469,7 +568,7 @@ struct Tracer {
   }
 
   // testLine (beam_core.h:356-362).
-  __device__ bool testLine(const Beam& B, const Line& line, Line* L, int& nl, bool& allIn) const {
+  FP64_HD bool testLine(const Beam& B, const Line& line, Line* L, int& nl, bool& allIn) const {
     const L4 dv = affine(B.qx, line.a, B.qy, line.b, line.c);
     bool anyIn = false, anyOut = false;
     for (int i = 0; i < 4; ++i) {
@@ -483,7 +582,7 @@ struct Tracer {
   }
 
   // classifyTri (beam_core.h:374-409) of an any-hit query: 0 Keep, 1 AllIn, 2 Clip.
-  __device__ int classify(const Beam& B, const Tri& ti, Line* L, int& nl) const {
+  FP64_HD int classify(const Beam& B, const Tri& ti, Line* L, int& nl) const {
     nl = 0;
     bool allIn = true;
     for (int k = 0; k < 3; ++k)
@@ -494,16 +593,16 @@ struct Tracer {
     return 2;
   }
 
-  __device__ void listPush(int l, const Poly& p) {
+  FP64_HD void listPush(int l, const Poly& p) {
     if (nList[l] >= S) {
       overflow = true;
       return;
     }
-    lists[l][nList[l]++] = p;
+    m.store(oList + uint32_t(l * S + nList[l]++) * wPoly, p);
   }
 
-  // clipTri (beam_core.h:419-479) into lists[0] (outside) and lists[1] (inside).
-  __device__ bool clip(const Beam& B, Line* L, int nl) {
+  // clipTri (beam_core.h:419-479) into the piece lists 0 (outside) and 1 (inside).
+  FP64_HD bool clip(const Beam& B, Line* L, int nl) {
     nList[0] = nList[1] = 0;
     double score[3];
     for (int k = 0; k < 3; ++k) {
@@ -521,19 +620,22 @@ struct Tracer {
         L[j] = L[j - 1];
         L[j - 1] = tl;
       }
-    Poly stack[8];
-    int stackLine[8];
+    // the pieces still to clip: polygons at oClip, their next line at oClip + 8 wPoly
     int sp = 1;
-    stack[0].n = B.n;
-    for (int i = 0; i < B.n; ++i) {
-      stack[0].x[i] = B.qx.v[i];
-      stack[0].y[i] = B.qy.v[i];
+    {
+      Poly p0;
+      p0.n = B.n;
+      for (int i = 0; i < B.n; ++i) {
+        p0.x[i] = B.qx.v[i];
+        p0.y[i] = B.qy.v[i];
+      }
+      m.store(oClip, p0);
+      m.storeInt(oClip + 8 * wPoly, 0);
     }
-    stackLine[0] = 0;
     while (sp > 0) {
       --sp;
-      Poly p = stack[sp];
-      int li = stackLine[sp];
+      Poly p = m.load<Poly>(oClip + uint32_t(sp) * wPoly);
+      int li = m.loadInt(oClip + 8 * wPoly + uint32_t(sp));
       bool inside = true;
       for (; li < nl; ++li) {
         const int c = sides(p, L[li], eps);
@@ -557,8 +659,8 @@ struct Tracer {
           splitFive(in, a, b);
           cnt->v[sFive]++;
           if (sp >= 8) return false;
-          stack[sp] = b;
-          stackLine[sp] = li + 1;
+          m.store(oClip + uint32_t(sp) * wPoly, b);
+          m.storeInt(oClip + 8 * wPoly + uint32_t(sp), li + 1);
           ++sp;
           p = a;
         } else {
@@ -572,7 +674,7 @@ struct Tracer {
 
   // output (beam_core.h:783-793): a finished piece; the miss pieces add their
   // area and their exact Lambert integral (polygonFormFactorG, soft_core.h:18-33).
-  __device__ void output(const Beam& b, int tri) {
+  FP64_HD void output(const Beam& b, int tri) {
     Poly p;
     p.n = b.n;
     for (int i = 0; i < b.n; ++i) {
@@ -594,13 +696,13 @@ struct Tracer {
     G += formFactorD(D3{x.x, x.y, x.z}, D3{ns.x, ns.y, ns.z}, P4, p.n);
   }
 
-  __device__ void finalize(int bi) {
-    output(pool[bi], -1);
+  FP64_HD void finalize(int bi) {
+    output(beam(bi), -1);
     release(bi);
   }
 
   // emit (beam_core.h:801-826) of an any-hit query.
-  __device__ void emit(const Poly& p0, const Beam& parent, bool hitNow, int tri) {
+  FP64_HD void emit(const Poly& p0, const Beam& parent, bool hitNow, int tri) {
     Poly pieces[2];
     int np = 1;
     if (p0.n == 5) {
@@ -619,17 +721,17 @@ struct Tracer {
       }
       const int idx = alloc();
       if (idx < 0) return;
-      pool[idx] = nb;
+      putBeam(idx, nb);
       pushNext(idx);
     }
   }
 
   // intersect (beam_core.h:832-882).
-  __device__ void intersect(int bi, int t, const Tri& ti) {
+  FP64_HD void intersect(int bi, int t, const Tri& ti) {
     cnt->v[sTests]++;
     Line L[5];
     int nl = 0;
-    const Beam B = pool[bi];
+    const Beam B = beam(bi);
     const int r = classify(B, ti, L, nl);
     if (r == 0) {
       pushNext(bi);
@@ -645,7 +747,7 @@ struct Tracer {
       return;
     }
     double newArea = 0;
-    for (int k = 0; k < nList[1]; ++k) newArea += lists[1][k].area();
+    for (int k = 0; k < nList[1]; ++k) newArea += piece(1, k).area();
     if (!(newArea >= minArea)) {
       pushNext(bi);
       return;
@@ -654,15 +756,15 @@ struct Tracer {
     release(bi);
     // lists are reused by the next clip: copy out before emitting
     const int n0 = nList[0], n1 = nList[1];
-    for (int k = 0; k < n0; ++k) emit(lists[0][k], B, false, t);
-    for (int k = 0; k < n1; ++k) emit(lists[1][k], B, true, t);
+    for (int k = 0; k < n0; ++k) emit(piece(0, k), B, false, t);
+    for (int k = 0; k < n1; ++k) emit(piece(1, k), B, true, t);
   }
 
   // processLeaf without mailboxes (beam_core.h:884-912).
-  __device__ void processLeaf(int bi, const KdNodeD& leaf) {
+  FP64_HD void processLeaf(int bi, const KdNodeD& leaf) {
     cnt->v[sLeaf]++;
     nCur = 1;
-    cur[0] = bi;
+    m.storeInt(oCur, bi);
     const uint32_t n = leaf.data >> 2;
     for (uint32_t i = 0; i < n && !overflow; ++i) {
       const LeafTri ref = sv->refs[leaf.offset + i];
@@ -670,16 +772,18 @@ struct Tracer {
       const Tri ti = setupTri(ref);
       if (!ti.ok) continue;
       nNxt = 0;
-      for (int k = 0; k < nCur && !overflow; ++k) intersect(cur[k], ref.tri, ti);
-      for (int k = 0; k < nNxt; ++k) cur[k] = nxt[k];
+      for (int k = 0; k < nCur && !overflow; ++k) intersect(m.loadInt(oCur + uint32_t(k)), ref.tri, ti);
+      const uint32_t t = oCur;
+      oCur = oNxt;
+      oNxt = t;
       nCur = nNxt;
       if (nCur == 0) break;
     }
   }
 
   // advance with the restart trail (beam_core.h:924-969).
-  __device__ void advance(int bi, uint32_t node, int depth, uint64_t trail) {
-    const Beam b = pool[bi];
+  FP64_HD void advance(int bi, uint32_t node, int depth, uint64_t trail) {
+    const Beam b = beam(bi);
     const Aw w = awOf(b);
     while (trail != 0) {
       const int d = 63 - __builtin_clzll(trail);
@@ -699,12 +803,12 @@ struct Tracer {
     finalize(bi);
   }
 
-  __device__ void rootPiece(const Poly& p, const int sgn[3], uint32_t node0, const Box& rootBox) {
+  FP64_HD void rootPiece(const Poly& p, const int sgn[3], uint32_t node0, const Box& rootBox) {
     Beam b;
     if (!geom(p, sgn, b)) return;
     const int idx = alloc();
     if (idx < 0) return;
-    pool[idx] = b;
+    putBeam(idx, b);
     cnt->v[sRoots]++;
     if (overlaps(b, awOf(b), rootBox))
       pushWork(Work{idx, node0, 0, 0, rootBox});
@@ -713,17 +817,12 @@ struct Tracer {
   }
 
   // presplit (beam_core.h:487-546) by the ray-direction sign lines per axis.
-  __device__ void presplit(const Poly& root, uint32_t node0, const Box& rootBox) {
-    struct Item {
-      Poly p;
-      int axis;
-      int sgn[3];
-    };
-    Item stack[8];
+  FP64_HD void presplit(const Poly& root, uint32_t node0, const Box& rootBox) {
     int sp = 0;
-    stack[sp++] = Item{root, 0, {1, 1, 1}};
+    m.store(oPre, PreItem{root, 0, {1, 1, 1}});
+    ++sp;
     while (sp > 0 && !overflow) {
-      Item it = stack[--sp];
+      PreItem it = m.load<PreItem>(oPre + uint32_t(--sp) * wItem);
       const Poly& p = it.p;
       if (p.n < 3) continue;
       if (it.axis == 3) {
@@ -750,15 +849,15 @@ struct Tracer {
           overflow = true;
           return;
         }
-        Item a = it, b = it;
+        PreItem a = it, b = it;
         a.p = in;
         a.axis = axis + 1;
         a.sgn[axis] = 1;
         b.p = outp;
         b.axis = axis + 1;
         b.sgn[axis] = -1;
-        stack[sp++] = b;
-        stack[sp++] = a;
+        m.store(oPre + uint32_t(sp++) * wItem, b);
+        m.store(oPre + uint32_t(sp++) * wItem, a);
         cnt->v[sPresplit]++;
       } else if (c != 0) {
         it.axis = axis + 1;
@@ -767,13 +866,13 @@ struct Tracer {
           overflow = true;
           return;
         }
-        stack[sp++] = it;
+        m.store(oPre + uint32_t(sp++) * wItem, it);
       }
     }
   }
 
   // BeamCore::trace with the restart trail (beam_core.h:569-661).
-  __device__ void trace(const Poly& rootIn, uint32_t node0, const Box& rootBox) {
+  FP64_HD void trace(const Poly& rootIn, uint32_t node0, const Box& rootBox) {
     const double e0 = rootIn.extent();
     setup(0.0 > e0 ? 0.0 : e0);
     Poly p = rootIn;
@@ -782,12 +881,12 @@ struct Tracer {
       presplit(p, node0, rootBox);
     }
     while (nWork > 0 && !overflow) {
-      const Work wk = work[--nWork];
+      const Work wk = m.load<Work>(oWork + uint32_t(--nWork) * wWork);
       uint32_t node = wk.node;
       Box box = wk.box;
       int depth = wk.depth;
       uint64_t trail = wk.trail;
-      const Beam B = pool[wk.beam];
+      const Beam B = beam(wk.beam);
       const Aw w = awOf(B);
       for (;;) {
         const KdNodeD kn = sv->nodes[node];
@@ -829,17 +928,86 @@ struct Tracer {
       }
       processLeaf(wk.beam, leaf);
       const int n = nCur;
-      for (int k = 0; k < n && !overflow; ++k) advance(cur[k], node, depth, trail);
+      for (int k = 0; k < n && !overflow; ++k) advance(m.loadInt(oCur + uint32_t(k)), node, depth, trail);
     }
   }
 
   // The unit normal of a triangle, in double as Scene::finalize computes it.
-  __device__ static V normalOf(V a, V b, V c) {
+  FP64_HD static V normalOf(V a, V b, V c) {
     const V e1 = b - a, e2 = c - a;
     const V nn = crs(e1, e2);
     const double l = sqrt(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
     return l > 0 ? V{nn.x / l, nn.y / l, nn.z / l} : nn;
   }
 };
+
+// The visibility of one receiver point: beamLightVisibility (render.cpp:279-303)
+// with lightBeamSetup and lightCenterG (soft_core.h:38-84) in double. x: the
+// hit point; ref: the hit triangle's leaf reference (its normal is taken in
+// double from the corners); nf: the triangle normal facing the eye; the light
+// (c, U, V); the apex offset along the normal; the kd root node0 with its box.
+// Writes the visible fraction, E (the exact Lambert integral of the miss
+// pieces when `exact`, else the centre estimate times vis) and Eu (the
+// unshadowed centre estimate). False when the tracer's pools overflowed (a
+// pool of more beams gives the same result).
+template <class TR>
+FP64_HD bool shadePoint(TR& tr, const DevScene& sv, V x, const LeafTri& ref, V nf, V c, V U, V Vv, double offset,
+                        bool exact, uint32_t node0, const Box& bounds, Cnt& cnt, double& vis, double& E,
+                        double& Eu) {
+  const V nd = TR::normalOf(vmk(ref.v[0], ref.v[1], ref.v[2]), vmk(ref.v[3], ref.v[4], ref.v[5]),
+                            vmk(ref.v[6], ref.v[7], ref.v[8]));
+  const V ns = dot(nd, nf) < 0 ? neg(nd) : nd;
+  const V uv = crs(U, Vv);
+  const double A = len(uv);
+  const V nL = nrmz(uv);
+  const bool front = dot(x - c, nL) > 0;
+  double G0 = 0;
+  if (front) {
+    const V w = c - x;
+    const double r2 = dot(w, w), rl = sqrt(r2);
+    const V wn = vmk(w.x / rl, w.y / rl, w.z / rl);
+    const double ci = dot(ns, wn), cl = -dot(nL, wn);
+    G0 = (ci > 0 ? ci : 0) * (cl > 0 ? cl : 0) / r2;
+  }
+  Eu = A * G0;
+  vis = 0;
+  E = 0;
+  if (!front) return true;
+  tr.Q = Plane::make(x + ns * offset, c, U, Vv);
+  const double hzc = dot(ns, c - x);
+  const V hu = U * 0.5, hv = Vv * 0.5;
+  const V corners[4] = {(c - hu) - hv, (c + hu) - hv, (c + hu) + hv, (c - hu) + hv};
+  Poly quad;
+  quad.n = 0;
+  for (int i = 0; i < 4; ++i) {
+    double qx, qy;
+    tr.Q.coords(corners[i], qx, qy);
+    quad.push(qx, qy);
+  }
+  if (quad.area() < 0) quad.reverse();
+  const double lightArea = quad.area();
+  const Line hz{dot(ns, tr.Q.u), dot(ns, tr.Q.v), hzc};
+  int above = 0;
+  for (int i = 0; i < quad.n; ++i) above += hz.at(quad.x[i], quad.y[i]) > 0;
+  if (above == 0) return true;
+  Poly root = quad;
+  if (above != quad.n) {
+    Poly rest;
+    splitPoly(quad, hz, 0, root, rest);
+  }
+  if (!(root.n >= 3 && root.area() > 0)) return true;
+  tr.sv = &sv;
+  tr.exclude = ref.tri;
+  tr.x = x;
+  tr.ns = ns;
+  tr.exact = exact;
+  tr.cnt = &cnt;
+  tr.trace(root, node0, bounds);
+  if (tr.overflow) return false;
+  const double v = tr.missArea / lightArea;
+  vis = v < 0 ? 0.0 : (v > 1 ? 1.0 : v);
+  E = exact ? tr.G : A * G0 * vis;
+  return true;
+}
 
 }  // namespace fp64

@@ -349,10 +349,11 @@ def test_fp64_soft_shadows_match_the_double_build(require_gpu, name, scale, side
     """precision = 2 (the whole beam trace in double with the double build's
     tolerances) against bt_render_d's algorithm (the reference compiled with
     Real = double, oracle/_ref): on every pixel with the same primary hit the
-    visible fraction is within 1e-5 (tolerance of the precision target; the
-    measured difference is the float camera's ray directions against the
-    double camera's, ~1e-7) and the exact irradiance within 1e-5 of the
-    image's largest."""
+    visible fraction is within 1e-5 (tolerance of the precision target; what
+    differs is the camera: its vectors are floats here and doubles in the
+    double build, which moves receiver points by ~1e-8 of the distance and
+    visible fractions by up to ~6e-6 at sharp penumbrae) and the exact
+    irradiance within 1e-5 of the image's largest."""
     from beamtrace2007_amd.host import camera
     from beamtrace2007_amd.workloads import scaled_light
     from refpy import Ref, ref_available
