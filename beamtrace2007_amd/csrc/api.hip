@@ -402,7 +402,8 @@ int btg_soft_batch(btg_ctx* ctx, const btg_soft_job* jobs, int numJobs, int exac
     for (int j = 0; j < numJobs; ++j) {
       if (!validCam(&jobs[j].cam)) return failWith(ctx, BTG_ERR_ARG, "job " + std::to_string(j) + ": invalid camera");
       h[size_t(j)] = btg::SoftJobH{jobs[j].scene, camOf(jobs[j].cam), lightOf(jobs[j].light), jobs[j].tri,
-                                   jobs[j].vis, jobs[j].E, jobs[j].Eu};
+                                   jobs[j].vis, jobs[j].E, jobs[j].Eu,
+                                   jobs[j].camD ? jobs[j].camD->eye : nullptr};
     }
     const int rc = btg::engineSoftBatch(ctx->engine, h.data(), numJobs, exact != 0, ri, ctx->err);
     if (rc) return rc;
@@ -906,6 +907,10 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
   const int jobs = c.jobsPerPass > 0 ? c.jobsPerPass : 512;
   const int rounds = c.roundsPerStep > 0 ? c.roundsPerStep : 8;
   const btg::Options& opt = btg::engineOptions(ctx->engine);
+  if (opt.precision == 2)
+    return failWith(ctx, BTG_ERR_ARG,
+                    "a server's passes run precision 0 or 1; FP64 soft shadows (precision 2) trace each pixel "
+                    "on its own and run through btg_soft / btg_soft_batch");
   // per pass: three sub-beam queues (64 B records) and the shade queue (40 B)
   // in at most a quarter of hbmBytes (the largest power-of-two capacity that
   // fits, and no more than the context's), then per pixel its images and per

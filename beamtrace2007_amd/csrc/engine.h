@@ -139,6 +139,7 @@ struct SoftJobH {
   float* vis;
   float* E;
   float* Eu;
+  const double* camD = nullptr;  // FP64 mode: eye, fwd, right, down, halfW, halfH in double (null: cam widened)
 };
 // Many jobs in one persistent wavefront pass over all their roots.
 int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo& info, std::string& err);

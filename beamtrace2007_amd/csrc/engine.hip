@@ -182,6 +182,7 @@ struct Params {
   const float* pixX;
   const float* pixXlo;  // guarded soft: low parts of the double hit points (null otherwise)
   int* pixRef;          // FP64 soft: the hit's leaf reference per pixel (null otherwise)
+  const double* jobCamD;  // FP64 soft: per job eye, fwd, right, down, halfW, halfH in double
   const float* pixN;
   float* vis;
   float* E;
@@ -2380,17 +2381,18 @@ __device__ bool rayNearest(const DevScene& sv, uint32_t node0, const Box& bounds
 // the float pair X + Xlo: o + d t with n = (v1 - v0) x (v2 - v0) and
 // t = n.(v0 - o) / n.d, every step in double (the guarded modes' receiver
 // point; the float pipeline's o + d t can sit ~1e-7 of the scene off that
-// plane, far more than the double build's apex offset). d: the ray direction
-// in double (the float ray's, or the FP64 mode's double sum).
-__device__ void hitPointD(const LeafTri& t, F3 o, double dx, double dy, double dz, F3& X, F3& Xlo) {
+// plane, far more than the double build's apex offset). o, d: the ray's
+// origin and direction in double (the float ray's, or the FP64 mode's ray).
+__device__ void hitPointD(const LeafTri& t, double ox, double oy, double oz, double dx, double dy, double dz, F3& X,
+                          F3& Xlo) {
   const double v0x = t.v[0], v0y = t.v[1], v0z = t.v[2];
   const double ax = double(t.v[3]) - v0x, ay = double(t.v[4]) - v0y, az = double(t.v[5]) - v0z;
   const double bx = double(t.v[6]) - v0x, by = double(t.v[7]) - v0y, bz = double(t.v[8]) - v0z;
   const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
   const double den = (nx * dx + ny * dy) + nz * dz;
-  const double num = (nx * (v0x - double(o.x)) + ny * (v0y - double(o.y))) + nz * (v0z - double(o.z));
+  const double num = (nx * (v0x - ox) + ny * (v0y - oy)) + nz * (v0z - oz);
   const double th = den != 0 ? num / den : 0.0;
-  const double px = double(o.x) + dx * th, py = double(o.y) + dy * th, pz = double(o.z) + dz * th;
+  const double px = ox + dx * th, py = oy + dy * th, pz = oz + dz * th;
   X = f3(float(px), float(py), float(pz));
   Xlo = f3(float(px - double(X.x)), float(py - double(X.y)), float(pz - double(X.z)));
 }
@@ -2408,57 +2410,123 @@ __device__ int jobOfPixel(const Params& p, int k) {
   return lo;
 }
 
-// One pixel of FP64 soft shadows (fp64::shadePoint at the pixel's double hit
-// point); false when the tracer's pools overflowed (nothing written).
-template <class TR>
-__device__ bool fp64Pixel(const Params& p, const SoftJob& J, int pix, TR& tr, Cnt& cnt) {
+// The FP64 query of a pixel with a primary hit: its double hit point (the
+// float pair), the hit's leaf reference and the normal facing the eye.
+__device__ void fp64RootOf(const Params& p, const SoftJob& J, int pix, fp64::PixelRoot& R) {
   using fp64::V;
-  const int T = p.pixTri[pix];
-  if (T < 0) {
-    p.vis[pix] = -1;
-    p.E[pix] = 0;
-    p.Eu[pix] = 0;
-    return true;
-  }
   const V x = fp64::vmk(double(p.pixX[3 * pix]) + double(p.pixXlo[3 * pix]),
                         double(p.pixX[3 * pix + 1]) + double(p.pixXlo[3 * pix + 1]),
                         double(p.pixX[3 * pix + 2]) + double(p.pixXlo[3 * pix + 2]));
   const LeafTri ref = p.sv.refs[uint32_t(p.pixRef[pix])];
   const V nf = fp64::vmk(p.pixN[3 * pix], p.pixN[3 * pix + 1], p.pixN[3 * pix + 2]);
-  double vis, E, Eu;
-  if (!fp64::shadePoint(tr, p.sv, x, ref, nf, fp64::vf(J.lc), fp64::vf(J.lU), fp64::vf(J.lV), double(J.offset),
-                        p.exact != 0, J.node0, J.bounds, cnt, vis, E, Eu))
-    return false;
-  p.vis[pix] = float(vis);
-  p.E[pix] = float(E);
-  p.Eu[pix] = float(Eu);
-  return true;
+  fp64::pixelRoot(ref, x, nf, fp64::vf(J.lc), fp64::vf(J.lU), fp64::vf(J.lV), double(J.offset), R);
 }
 
-// FP64 soft shadows over `count` pixels (list[i], or i without a list): each
-// thread takes pixels i, i + threads, ... with pools of Pn beams and Kn work
-// items in its region of `mem` (Tracer::kWords words per thread, interleaved
-// across the block's 64 threads; the grid is no larger than `mem` holds). A
-// pixel whose pools overflow is appended to `retry` (traced again with larger
-// pools); without a retry list it is counted in *failed.
-template <int Pn, int Kn>
-__global__ void __launch_bounds__(64) softFp64Kernel(Params p, const int* list, int count, uint64_t* mem, int* retry,
-                                                     int* retryCount, int* failed) {
+// Waves per SIMD the FP64 kernels are built for (build knob BTG_FP64_WAVES;
+// 2 and 4 measured slower than 1: the register spills cost more than the
+// extra waves hide).
+#ifndef BTG_FP64_WAVES
+#define BTG_FP64_WAVES 1
+#endif
+constexpr int kFp64Waves = BTG_FP64_WAVES;
+
+// Triangle tests and kd steps a level-0 or level-1 cell may take before it is
+// split into finer cells (build knob BTG_FP64_WORK_CAP): bounds the time one
+// lane spends on one task, which sets the pass's tail.
+#ifndef BTG_FP64_WORK_CAP
+#define BTG_FP64_WORK_CAP 65536
+#endif
+constexpr uint32_t kFp64WorkCap = BTG_FP64_WORK_CAP;
+
+__global__ void zeroWords(unsigned long long* w, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) w[i] = 0;
+}
+
+// One level of FP64 soft shadows. Tasks: level 0, the pixels 0..numPix-1;
+// level 1, 16 per entry of `in` (a pixel: the cells of its 4 x 4 grid);
+// level 2, 16 per entry (pixel << 4 | a level-1 cell: the 16 x 16 grid's
+// cells inside it). Each lane takes tasks from `next` until none is left,
+// tracing with pools of Pn beams and Kn work items in its region of `mem`
+// (Tracer::kWords words per thread, interleaved across the block's 64
+// threads). A finished cell adds its results to the pixel's fixed-point sums;
+// a cell over the work cap or the pools goes to `out` for the next level (on
+// level 2, which has no cap, it counts in *failed, as does a full `out`).
+template <int Pn, int Kn, int Level>
+__global__ void __launch_bounds__(64, kFp64Waves)
+    fp64CellKernel(Params p, int numPix, const long long* in, const int* inCount, int inCap, int* next, uint64_t* mem,
+                   long long* out, int* outCount, int outCap, unsigned long long* visSum, unsigned long long* gSum,
+                   int* failed) {
   using TR = fp64::Tracer<Pn, Kn>;
   Cnt cnt;
   cnt.zero();
   TR tr;
   tr.m = fp64::WordMem{mem + size_t(blockIdx.x) * size_t(TR::kWords) * 64, 64, int(threadIdx.x)};
-  for (int i = int(blockIdx.x) * 64 + int(threadIdx.x); i < count; i += int(gridDim.x) * 64) {
-    const int pix = list ? list[i] : i;
-    if (!fp64Pixel(p, p.jobs[jobOfPixel(p, pix)], pix, tr, cnt)) {
-      if (retry)
-        retry[atomicAdd(retryCount, 1)] = pix;
+  const int tasks = Level == 0 ? numPix : 16 * min(*inCount, inCap);
+  for (;;) {
+    const int t = atomicAdd(next, 1);
+    if (t >= tasks) break;
+    int pix = t, cx = 0, cy = 0;
+    if (Level > 0) {
+      const long long e = in[t >> 4];
+      const int s = t & 15;
+      if (Level == 1) {
+        pix = int(e);
+        cx = s & 3;
+        cy = s >> 2;
+      } else {
+        pix = int(e >> 4);
+        const int c1 = int(e & 15);
+        cx = (c1 & 3) * 4 + (s & 3);
+        cy = (c1 >> 2) * 4 + (s >> 2);
+      }
+    }
+    const int T = p.pixTri[pix];
+    if (T < 0) continue;
+    const SoftJob& J = p.jobs[jobOfPixel(p, pix)];
+    fp64::PixelRoot R;
+    fp64RootOf(p, J, pix, R);
+    if (R.root.n < 3) continue;
+    double miss, G;
+    if (fp64::traceCell(tr, p.sv, R, T, p.exact != 0, Level, cx, cy, Level == 2 ? 0xffffffffu : kFp64WorkCap,
+                        J.node0, J.bounds, cnt, miss, G)) {
+      atomicAdd(&visSum[pix], fp64::visFx(miss / R.lightArea));
+      if (p.exact) atomicAdd(&gSum[pix], fp64::gFx(G));
+    } else if (Level < 2) {
+      const int k = atomicAdd(outCount, 1);
+      if (k < outCap)
+        out[k] = Level == 0 ? (long long)pix : (((long long)pix << 4) | (long long)(cy * 4 + cx));
       else
         atomicAdd(failed, 1);
+    } else {
+      atomicAdd(failed, 1);
     }
   }
   flushCnt(p, cnt);
+}
+
+// The FP64 images from the pixels' sums: vis = visible fraction (-1 without
+// a primary hit), E = the exact Lambert integral (or A * G0 * vis), Eu = A * G0.
+__global__ void __launch_bounds__(256) fp64FinishKernel(Params p, int numPix, const unsigned long long* visSum,
+                                                        const unsigned long long* gSum) {
+  const int pix = int(blockIdx.x) * 256 + int(threadIdx.x);
+  if (pix >= numPix) return;
+  if (p.pixTri[pix] < 0) {
+    p.vis[pix] = -1;
+    p.E[pix] = 0;
+    p.Eu[pix] = 0;
+    return;
+  }
+  fp64::PixelRoot R;
+  fp64RootOf(p, p.jobs[jobOfPixel(p, pix)], pix, R);
+  p.Eu[pix] = float(R.A * R.G0);
+  if (R.root.n < 3) {
+    p.vis[pix] = 0;
+    p.E[pix] = 0;
+    return;
+  }
+  const double vis = fp64::visOfFx(visSum[pix]);
+  p.vis[pix] = float(vis);
+  p.E[pix] = p.exact ? float(fp64::gOfFx(gSum[pix])) : float(R.A * R.G0 * vis);
 }
 
 // The soft-shadow primary pass (cuda/gpu_render.cu:222-244): one ray through
@@ -2500,25 +2568,29 @@ __global__ void __launch_bounds__(kRayBlock) primaryRayKernel(Params p) {
       F3 nrm = f3(nn.x, nn.y, nn.z);
       bool away = dot(nrm, d) > 0;
       if (p.pixXlo) {
-        // FP64 mode (pixRef set): the ray the double build shoots, the pixel
-        // centre and direction summed in double from the camera's floats (the
-        // float sum moves the receiver point by ~1e-7 of the distance, which
-        // shows as up to ~2e-5 of visible fraction at sharp penumbrae)
-        double dx = d.x, dy = d.y, dz = d.z;
+        // FP64 mode (pixRef set): the ray the double build shoots from its
+        // double camera, the pixel centre and direction in double (the float
+        // ray moves the receiver point by ~1e-7 of the distance, which shows
+        // as up to ~2e-5 of visible fraction at sharp penumbrae)
+        double ox = J.eye.x, oy = J.eye.y, oz = J.eye.z, dx = d.x, dy = d.y, dz = d.z;
         if (p.pixRef) {
-          const double qxd = (2 * (double(x) + 0.5) / double(J.W) - 1) * double(J.halfW);
-          const double qyd = (2 * (double(y) + 0.5) / double(J.H) - 1) * double(J.halfH);
-          dx = (double(J.fwd.x) + double(J.right.x) * qxd) + double(J.down.x) * qyd;
-          dy = (double(J.fwd.y) + double(J.right.y) * qxd) + double(J.down.y) * qyd;
-          dz = (double(J.fwd.z) + double(J.right.z) * qxd) + double(J.down.z) * qyd;
+          const double* c = p.jobCamD + size_t(lo) * 14;
+          const double qxd = (2 * (double(x) + 0.5) / double(J.W) - 1) * c[12];
+          const double qyd = (2 * (double(y) + 0.5) / double(J.H) - 1) * c[13];
+          ox = c[0];
+          oy = c[1];
+          oz = c[2];
+          dx = (c[3] + c[6] * qxd) + c[9] * qyd;
+          dy = (c[4] + c[7] * qxd) + c[10] * qyd;
+          dz = (c[5] + c[8] * qxd) + c[11] * qyd;
           away = (double(nn.x) * dx + double(nn.y) * dy) + double(nn.z) * dz > 0;
         }
-        F3 lo;
-        hitPointD(p.sv.refs[th_ref], J.eye, dx, dy, dz, hp, lo);
+        F3 xlo;
+        hitPointD(p.sv.refs[th_ref], ox, oy, oz, dx, dy, dz, hp, xlo);
         float* XL = const_cast<float*>(p.pixXlo);
-        XL[3 * k] = lo.x;
-        XL[3 * k + 1] = lo.y;
-        XL[3 * k + 2] = lo.z;
+        XL[3 * k] = xlo.x;
+        XL[3 * k + 1] = xlo.y;
+        XL[3 * k + 2] = xlo.z;
       }
       if (away) nrm = -nrm;
       X[3 * k] = hp.x;
@@ -2586,8 +2658,9 @@ class Engine {
   // engine buffers (grown on demand)
   DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
-  DevBuf pixRef, fpList, fpMem;  // FP64 soft shadows: hit leaf reference per pixel; pixels to trace again,
-                                 // counters; the tracers' arrays
+  // FP64 soft shadows: hit leaf reference per pixel; counters and cell lists;
+  // the tracers' arrays; the jobs' double cameras; per-pixel fixed-point sums
+  DevBuf pixRef, fpList, fpMem, camD, fpSum;
   DevBuf rasterImg, rasterRows;  // device rasterization: image; row counts, span offsets and tile sums
   void* noAllocScratch = nullptr;  // this device's static rasterization scratch (null: unavailable)
   WState* hostState = nullptr;  // pinned
@@ -2639,7 +2712,7 @@ class Engine {
   }
   ~Engine() {
     DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
-                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs, &pixRef, &fpList, &fpMem};
+                     &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs, &pixRef, &fpList, &fpMem, &camD, &fpSum};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
     for (void* m : sceneMem)
@@ -3140,45 +3213,58 @@ Options& engineOptions(Engine* e) { return e->opt; }
 
 namespace {
 
-// FP64 soft shadows of every pixel of a batch (after its primary pass): pools
-// of 64 beams first, then 384 for the pixels that need more. The tracers'
-// arrays take fpMem: the first pass runs at most 4 blocks of 64 threads per
-// CU (one wave per SIMD; 1 MB per block), the second as many blocks as that
-// memory holds (3.8 MB per block), each thread looping over its pixels.
+// FP64 soft shadows of every pixel of a batch (after its primary pass), in
+// three levels of cells (fp64CellKernel): the whole root with pools of 64
+// beams and the work cap; the 4 x 4 cells of the roots that overflowed, with
+// 128 beams and the cap; the 16 x 16 cells inside those that overflowed
+// again, with 384 beams and no cap. The levels run back to back on the stream
+// (each reads its task count from the device); one synchronisation at the end
+// reads the cells that overflowed their last level.
 int softFp64(Engine& e, Params& p, size_t np, std::string& err) {
-  using Small = fp64::Tracer<64, 64>;
-  using Large = fp64::Tracer<384, 256>;
-  constexpr size_t kSmallBlock = size_t(Small::kWords) * 64 * 8, kLargeBlock = size_t(Large::kWords) * 64 * 8;
-  const size_t smallBlocks = std::min((np + 63) / 64, size_t(std::max(e.cus, 1)) * 4);
-  int rc = e.ensure(e.fpMem, std::max(smallBlocks * kSmallBlock, kLargeBlock), err);
-  if (rc) return rc;
+  using T0 = fp64::Tracer<64, 64>;
+  using T1 = fp64::Tracer<128, 128>;
+  using T2 = fp64::Tracer<384, 256>;
+  constexpr size_t kBlock0 = size_t(T0::kWords) * 64 * 8, kBlock1 = size_t(T1::kWords) * 64 * 8,
+                   kBlock2 = size_t(T2::kWords) * 64 * 8;
+  const size_t cus = size_t(std::max(e.cus, 1)), maxBlocks = cus * 4 * kFp64Waves;
+  const size_t blocks0 = std::min((np + 63) / 64, maxBlocks);
+  const size_t blocks1 = std::min((16 * np + 63) / 64, maxBlocks);
+  const size_t blocks2 = std::min((256 * np + 63) / 64, cus);
+  const size_t cap2 = std::min<size_t>(16 * np, size_t(1) << 24);  // level-2 entries
+  int rc;
+  if ((rc = e.ensure(e.fpMem, std::max({blocks0 * kBlock0, blocks1 * kBlock1, blocks2 * kBlock2}), err)) ||
+      (rc = e.ensure(e.fpList, 64 + (np + cap2) * sizeof(long long), err)) ||
+      (rc = e.ensure(e.fpSum, 2 * np * sizeof(unsigned long long), err)))
+    return rc;
   uint64_t* mem = static_cast<uint64_t*>(e.fpMem.ptr);
-  int* retry = static_cast<int*>(e.fpList.ptr);
-  int* counters = retry + np;  // [0] pixels to trace again, [1] pixels that overflowed twice
-  clearInt<<<1, 64, 0, e.stream>>>(counters);
-  clearInt<<<1, 64, 0, e.stream>>>(counters + 1);
+  int* ctr = static_cast<int*>(e.fpList.ptr);  // next[3], outCount[2], failed
+  long long* list1 = reinterpret_cast<long long*>(static_cast<char*>(e.fpList.ptr) + 64);
+  long long* list2 = list1 + np;
+  unsigned long long* visSum = static_cast<unsigned long long*>(e.fpSum.ptr);
+  unsigned long long* gSum = visSum + np;
+  zeroWords<<<1, 64, 0, e.stream>>>(reinterpret_cast<unsigned long long*>(ctr), 8);
+  zeroWords<<<unsigned(std::min<size_t>((2 * np + 255) / 256, 4096)), 256, 0, e.stream>>>(visSum, 2 * np);
+  const int n = int(np);
   int lt = e.beginLaunch(1);
-  softFp64Kernel<64, 64><<<unsigned(smallBlocks), 64, 0, e.stream>>>(p, nullptr, int(np), mem, retry, counters,
-                                                                    counters + 1);
+  fp64CellKernel<64, 64, 0><<<unsigned(blocks0), 64, 0, e.stream>>>(p, n, nullptr, nullptr, 0, ctr, mem, list1,
+                                                                    ctr + 3, n, visSum, gSum, ctr + 5);
   e.endLaunch(lt);
+  lt = e.beginLaunch(1);
+  fp64CellKernel<128, 128, 1><<<unsigned(blocks1), 64, 0, e.stream>>>(p, n, list1, ctr + 3, n, ctr + 1, mem, list2,
+                                                                      ctr + 4, int(cap2), visSum, gSum, ctr + 5);
+  e.endLaunch(lt);
+  lt = e.beginLaunch(1);
+  fp64CellKernel<384, 256, 2><<<unsigned(blocks2), 64, 0, e.stream>>>(p, n, list2, ctr + 4, int(cap2), ctr + 2, mem,
+                                                                      nullptr, nullptr, 0, visSum, gSum, ctr + 5);
+  e.endLaunch(lt);
+  fp64FinishKernel<<<unsigned((np + 255) / 256), 256, 0, e.stream>>>(p, n, visSum, gSum);
   BTG_TRY(hipGetLastError());
-  int again = 0;
-  BTG_TRY(hipMemcpyAsync(&again, counters, 4, hipMemcpyDeviceToHost, e.stream));
+  int failed = 0;
+  BTG_TRY(hipMemcpyAsync(&failed, ctr + 5, 4, hipMemcpyDeviceToHost, e.stream));
   BTG_TRY(hipStreamSynchronize(e.stream));
-  if (again > 0) {
-    const size_t largeBlocks = std::min((size_t(again) + 63) / 64, e.fpMem.bytes / kLargeBlock);
-    lt = e.beginLaunch(1);
-    softFp64Kernel<384, 256><<<unsigned(largeBlocks), 64, 0, e.stream>>>(p, retry, again, mem, nullptr, nullptr,
-                                                                        counters + 1);
-    e.endLaunch(lt);
-    BTG_TRY(hipGetLastError());
-    int failed = 0;
-    BTG_TRY(hipMemcpyAsync(&failed, counters + 1, 4, hipMemcpyDeviceToHost, e.stream));
-    BTG_TRY(hipStreamSynchronize(e.stream));
-    if (failed > 0) {
-      err = std::to_string(failed) + " pixel(s) need more than 384 live beams in FP64 mode";
-      return BTG_ERR_LIMIT_CODE;
-    }
+  if (failed > 0) {
+    err = std::to_string(failed) + " cell(s) of 1/256 of a pixel's light need more than 384 live beams in FP64 mode";
+    return BTG_ERR_LIMIT_CODE;
   }
   return 0;
 }
@@ -3233,7 +3319,8 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
   }
   Params p;
   const bool guard = e->opt.precision != 0;
-  int rc = prepare(*e, guard ? kSoftG : kSoft, int(R), e->opt.budget, p, err);
+  const bool fp64Mode = e->opt.precision == 2;  // no roots: each pixel is traced on its own
+  int rc = prepare(*e, guard ? kSoftG : kSoft, fp64Mode ? 0 : int(R), e->opt.budget, p, err);
   if (rc) return rc;
   const size_t np = size_t(npix);
   if ((rc = e->ensure(e->pixTri, np * 4, err)) || (rc = e->ensure(e->pixX, np * 12, err)) ||
@@ -3243,10 +3330,32 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
       (guard && (rc = e->ensure(e->pixXlo, np * 12, err))))
     return rc;
   if (guard) p.pixXlo = static_cast<const float*>(e->pixXlo.ptr);
-  const bool fp64Mode = e->opt.precision == 2;
+  // FP64 mode: each job's camera in double (the caller's, or its floats widened)
+  std::vector<double> camD;
   if (fp64Mode) {
-    if ((rc = e->ensure(e->pixRef, np * 4, err)) || (rc = e->ensure(e->fpList, (np + 4) * 4, err))) return rc;
+    camD.resize(size_t(n) * 14);
+    for (int j = 0; j < n; ++j) {
+      double* c = camD.data() + size_t(j) * 14;
+      const HostCam& hc = jobs[j].cam;
+      if (jobs[j].camD) {
+        std::memcpy(c, jobs[j].camD, 14 * sizeof(double));
+      } else {
+        const F3 v[4] = {hc.eye, hc.fwd, hc.right, hc.down};
+        for (int k = 0; k < 4; ++k) {
+          c[3 * k] = v[k].x;
+          c[3 * k + 1] = v[k].y;
+          c[3 * k + 2] = v[k].z;
+        }
+        c[12] = hc.halfW;
+        c[13] = hc.halfH;
+      }
+    }
+    if ((rc = e->ensure(e->pixRef, np * 4, err)) ||
+        (rc = e->ensure(e->camD, camD.size() * sizeof(double), err)))
+      return rc;
+    BTG_TRY(hipMemcpyAsync(e->camD.ptr, camD.data(), camD.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
     p.pixRef = static_cast<int*>(e->pixRef.ptr);
+    p.jobCamD = static_cast<const double*>(e->camD.ptr);
   }
   BTG_TRY(hipMemcpyAsync(e->jobs.ptr, jt.data(), jt.size() * sizeof(SoftJob), hipMemcpyHostToDevice, e->stream));
   p.jobs = static_cast<const SoftJob*>(e->jobs.ptr);

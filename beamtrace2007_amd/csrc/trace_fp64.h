@@ -282,6 +282,15 @@ struct Plane {
   }
 };
 
+// The unit normal of a triangle, in double as Scene::finalize computes it
+// (src/scene/scene.cpp:18-33).
+FP64_HD inline V unitNormal(V a, V b, V c) {
+  const V e1 = b - a, e2 = c - a;
+  const V nn = crs(e1, e2);
+  const double l = sqrt(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+  return l > 0 ? V{nn.x / l, nn.y / l, nn.z / l} : nn;
+}
+
 // A sub-beam: corners on the query plane (lanes past n repeat the last) and
 // the ray-direction sign per axis.
 struct Beam {
@@ -381,7 +390,9 @@ struct Tracer {
   V x, ns;     // shading point and normal (exact Lambert integral of the miss pieces)
   bool exact;  // integrate the miss pieces
   double missArea, G;
-  bool overflow;
+  bool overflow;        // a pool, the work stack or the work cap ran out: the trace is void
+  uint32_t work;        // triangle tests and kd steps of this trace
+  uint32_t workCap = 0xffffffffu;
   Cnt* cnt;
   int nFree, nPool;
   uint32_t oCur, oNxt;  // oCurA / oCurB, swapped after each triangle of a leaf
@@ -400,6 +411,7 @@ struct Tracer {
     farCq = dot(Q.n, Q.d);
     missArea = G = 0;
     overflow = false;
+    work = 0;
     nFree = nPool = nCur = nNxt = nWork = 0;
     oCur = oCurA;
     oNxt = oCurB;
@@ -506,7 +518,7 @@ struct Tracer {
       p[k] = vmk(double(t.v[3 * k]), double(t.v[3 * k + 1]), double(t.v[3 * k + 2]));
       ti.r[k] = Q.rel(p[k]);
     }
-    ti.n = normalOf(p[0], p[1], p[2]);
+    ti.n = unitNormal(p[0], p[1], p[2]);
     V m[3];
     const int E[3][2] = {{0, 1}, {1, 2}, {2, 0}};
     for (int k = 0; k < 3; ++k) {
@@ -728,6 +740,10 @@ struct Tracer {
 
   // intersect (beam_core.h:832-882).
   FP64_HD void intersect(int bi, int t, const Tri& ti) {
+    if (++work > workCap) {
+      overflow = true;
+      return;
+    }
     cnt->v[sTests]++;
     Line L[5];
     int nl = 0;
@@ -871,10 +887,11 @@ struct Tracer {
     }
   }
 
-  // BeamCore::trace with the restart trail (beam_core.h:569-661).
-  FP64_HD void trace(const Poly& rootIn, uint32_t node0, const Box& rootBox) {
-    const double e0 = rootIn.extent();
-    setup(0.0 > e0 ? 0.0 : e0);
+  // BeamCore::trace with the restart trail (beam_core.h:569-661) of the root
+  // polygon rootIn; `extent` sets the tolerances (the extent of the whole
+  // light's root polygon, also when rootIn is a cell of it).
+  FP64_HD void trace(const Poly& rootIn, uint32_t node0, const Box& rootBox, double extent) {
+    setup(0.0 > extent ? 0.0 : extent);
     Poly p = rootIn;
     if (p.n >= 3) {
       if (p.area() < 0) p.reverse();
@@ -891,6 +908,10 @@ struct Tracer {
       for (;;) {
         const KdNodeD kn = sv->nodes[node];
         if ((kn.data & 3u) == 3u) break;
+        if (++work > workCap) {
+          overflow = true;
+          break;
+        }
         cnt->v[sKd]++;
         const int d = depth++;
         const int a = int(kn.data & 3u);
@@ -920,6 +941,7 @@ struct Tracer {
           box = toRight ? lb : rb;
         }
       }
+      if (overflow) break;
       const KdNodeD leaf = sv->nodes[node];
       if ((leaf.data >> 2) == 0) {
         cnt->v[sLeaf]++;
@@ -931,83 +953,139 @@ struct Tracer {
       for (int k = 0; k < n && !overflow; ++k) advance(m.loadInt(oCur + uint32_t(k)), node, depth, trail);
     }
   }
-
-  // The unit normal of a triangle, in double as Scene::finalize computes it.
-  FP64_HD static V normalOf(V a, V b, V c) {
-    const V e1 = b - a, e2 = c - a;
-    const V nn = crs(e1, e2);
-    const double l = sqrt(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
-    return l > 0 ? V{nn.x / l, nn.y / l, nn.z / l} : nn;
-  }
 };
 
-// The visibility of one receiver point: beamLightVisibility (render.cpp:279-303)
-// with lightBeamSetup and lightCenterG (soft_core.h:38-84) in double. x: the
-// hit point; ref: the hit triangle's leaf reference (its normal is taken in
-// double from the corners); nf: the triangle normal facing the eye; the light
-// (c, U, V); the apex offset along the normal; the kd root node0 with its box.
-// Writes the visible fraction, E (the exact Lambert integral of the miss
-// pieces when `exact`, else the centre estimate times vis) and Eu (the
-// unshadowed centre estimate). False when the tracer's pools overflowed (a
-// pool of more beams gives the same result).
-template <class TR>
-FP64_HD bool shadePoint(TR& tr, const DevScene& sv, V x, const LeafTri& ref, V nf, V c, V U, V Vv, double offset,
-                        bool exact, uint32_t node0, const Box& bounds, Cnt& cnt, double& vis, double& E,
-                        double& Eu) {
-  const V nd = TR::normalOf(vmk(ref.v[0], ref.v[1], ref.v[2]), vmk(ref.v[3], ref.v[4], ref.v[5]),
-                            vmk(ref.v[6], ref.v[7], ref.v[8]));
-  const V ns = dot(nd, nf) < 0 ? neg(nd) : nd;
+// One receiver point's query: beamLightVisibility (render.cpp:279-303) with
+// lightBeamSetup and lightCenterG (soft_core.h:38-84) in double, up to the
+// trace. x: the hit point; ref: the hit triangle's leaf reference (its normal
+// is taken in double from the corners); nf: the triangle normal facing the
+// eye; the light (c, U, V); the apex offset along the normal.
+struct PixelRoot {
+  V x, ns;
+  Plane Q;
+  Poly root;         // the light polygon on Q, above the receiver's horizon (n = 0: nothing to trace)
+  double lightArea;  // of the whole light polygon
+  double ext;        // the root's extent (the trace's tolerances)
+  double A, G0;      // the light's area and the centre estimate of G
+  bool front;        // the receiver is in front of the light
+};
+
+FP64_HD inline void pixelRoot(const LeafTri& ref, V x, V nf, V c, V U, V Vv, double offset, PixelRoot& R) {
+  const V nd = unitNormal(vmk(ref.v[0], ref.v[1], ref.v[2]), vmk(ref.v[3], ref.v[4], ref.v[5]),
+                          vmk(ref.v[6], ref.v[7], ref.v[8]));
+  R.x = x;
+  R.ns = dot(nd, nf) < 0 ? neg(nd) : nd;
   const V uv = crs(U, Vv);
-  const double A = len(uv);
+  R.A = len(uv);
   const V nL = nrmz(uv);
-  const bool front = dot(x - c, nL) > 0;
-  double G0 = 0;
-  if (front) {
+  R.front = dot(x - c, nL) > 0;
+  R.G0 = 0;
+  R.root.n = 0;
+  R.lightArea = 0;
+  R.ext = 0;
+  if (!R.front) return;
+  {
     const V w = c - x;
     const double r2 = dot(w, w), rl = sqrt(r2);
     const V wn = vmk(w.x / rl, w.y / rl, w.z / rl);
-    const double ci = dot(ns, wn), cl = -dot(nL, wn);
-    G0 = (ci > 0 ? ci : 0) * (cl > 0 ? cl : 0) / r2;
+    const double ci = dot(R.ns, wn), cl = -dot(nL, wn);
+    R.G0 = (ci > 0 ? ci : 0) * (cl > 0 ? cl : 0) / r2;
   }
-  Eu = A * G0;
-  vis = 0;
-  E = 0;
-  if (!front) return true;
-  tr.Q = Plane::make(x + ns * offset, c, U, Vv);
-  const double hzc = dot(ns, c - x);
+  R.Q = Plane::make(x + R.ns * offset, c, U, Vv);
+  const double hzc = dot(R.ns, c - x);
   const V hu = U * 0.5, hv = Vv * 0.5;
   const V corners[4] = {(c - hu) - hv, (c + hu) - hv, (c + hu) + hv, (c - hu) + hv};
   Poly quad;
   quad.n = 0;
   for (int i = 0; i < 4; ++i) {
     double qx, qy;
-    tr.Q.coords(corners[i], qx, qy);
+    R.Q.coords(corners[i], qx, qy);
     quad.push(qx, qy);
   }
   if (quad.area() < 0) quad.reverse();
-  const double lightArea = quad.area();
-  const Line hz{dot(ns, tr.Q.u), dot(ns, tr.Q.v), hzc};
+  R.lightArea = quad.area();
+  const Line hz{dot(R.ns, R.Q.u), dot(R.ns, R.Q.v), hzc};
   int above = 0;
   for (int i = 0; i < quad.n; ++i) above += hz.at(quad.x[i], quad.y[i]) > 0;
-  if (above == 0) return true;
+  if (above == 0) return;
   Poly root = quad;
   if (above != quad.n) {
     Poly rest;
     splitPoly(quad, hz, 0, root, rest);
   }
-  if (!(root.n >= 3 && root.area() > 0)) return true;
+  if (!(root.n >= 3 && root.area() > 0)) return;
+  R.root = root;
+  const double e0 = root.extent();
+  R.ext = e0;
+}
+
+// The cells a traced root splits into when it needs too much work: level 0
+// is the whole root; level 1 the cells of a 4 x 4 grid over its bounding box
+// on the query plane, level 2 of a 16 x 16 grid (each level-1 cell split
+// 4 x 4; the two grids share their lines exactly). Every cell is traced with
+// the whole root's tolerances and the cells' results add up to the root's.
+FP64_HD inline int cellGrid(int level) { return level == 0 ? 1 : (level == 1 ? 4 : 16); }
+
+FP64_HD inline Poly cellOf(const Poly& root, int level, int cx, int cy) {
+  if (level == 0) return root;
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (int i = 0; i < root.n; ++i) {
+    x0 = root.x[i] < x0 ? root.x[i] : x0;
+    x1 = root.x[i] > x1 ? root.x[i] : x1;
+    y0 = root.y[i] < y0 ? root.y[i] : y0;
+    y1 = root.y[i] > y1 ? root.y[i] : y1;
+  }
+  const int g = cellGrid(level);
+  const double xa = x0 + (x1 - x0) * (double(cx) / g), xb = cx + 1 == g ? x1 : x0 + (x1 - x0) * (double(cx + 1) / g);
+  const double ya = y0 + (y1 - y0) * (double(cy) / g), yb = cy + 1 == g ? y1 : y0 + (y1 - y0) * (double(cy + 1) / g);
+  const Line L[4] = {{1, 0, -xa}, {-1, 0, xb}, {0, 1, -ya}, {0, -1, yb}};
+  Poly p = root;
+  for (int k = 0; k < 4 && p.n >= 3; ++k) {
+    Poly in, out;
+    splitPoly(p, L[k], 0, in, out);
+    p = in;
+  }
+  if (p.n < 3) p.n = 0;
+  return p;
+}
+
+// Traces one cell of a pixel's root (cellOf) with at most `workCap` triangle
+// tests and kd steps. False when the tracer overflowed (pools, work stack or
+// work cap): the cell must be traced again, split or with larger pools.
+// Otherwise its missed area and (exact) Lambert integral are in miss and G.
+template <class TR>
+FP64_HD bool traceCell(TR& tr, const DevScene& sv, const PixelRoot& R, int exclude, bool exact, int level, int cx,
+                       int cy, uint32_t workCap, uint32_t node0, const Box& bounds, Cnt& cnt, double& miss,
+                       double& G) {
+  miss = G = 0;
+  const Poly cell = cellOf(R.root, level, cx, cy);
+  if (cell.n < 3) return true;
+  tr.Q = R.Q;
   tr.sv = &sv;
-  tr.exclude = ref.tri;
-  tr.x = x;
-  tr.ns = ns;
+  tr.exclude = exclude;
+  tr.x = R.x;
+  tr.ns = R.ns;
   tr.exact = exact;
   tr.cnt = &cnt;
-  tr.trace(root, node0, bounds);
+  tr.workCap = workCap;
+  tr.trace(cell, node0, bounds, R.ext);
   if (tr.overflow) return false;
-  const double v = tr.missArea / lightArea;
-  vis = v < 0 ? 0.0 : (v > 1 ? 1.0 : v);
-  E = exact ? tr.G : A * G0 * vis;
+  miss = tr.missArea;
+  G = tr.G;
   return true;
 }
+
+// The per-pixel sums of the cells' results, in fixed point so that their
+// order does not matter: the visible fraction in units of 2^-62 (a cell adds
+// its missed area / the light's), G in units of 2^-60 (G <= pi).
+FP64_HD inline unsigned long long visFx(double missOverLight) {
+  return (unsigned long long)((missOverLight > 0 ? missOverLight : 0.0) * 0x1p62);
+}
+FP64_HD inline unsigned long long gFx(double G) { return (unsigned long long)((G > 0 ? G : 0.0) * 0x1p60); }
+FP64_HD inline double visOfFx(unsigned long long v) {
+  const double r = double(v) * 0x1p-62;
+  return r > 1 ? 1.0 : r;
+}
+FP64_HD inline double gOfFx(unsigned long long v) { return double(v) * 0x1p-60; }
 
 }  // namespace fp64

@@ -44,6 +44,11 @@ class btg_camera(C.Structure):
                 ("halfW", C.c_float), ("halfH", C.c_float), ("W", C.c_int), ("H", C.c_int)]
 
 
+class btg_camera_d(C.Structure):
+    _fields_ = [("eye", C.c_double * 3), ("fwd", C.c_double * 3), ("right", C.c_double * 3),
+                ("down", C.c_double * 3), ("halfW", C.c_double), ("halfH", C.c_double)]
+
+
 class btg_area_light(C.Structure):
     _fields_ = [("c", C.c_float * 3), ("U", C.c_float * 3), ("V", C.c_float * 3)]
 
@@ -70,7 +75,7 @@ class btg_request_status(C.Structure):
 
 class btg_soft_job(C.Structure):
     _fields_ = [("scene", C.c_int), ("cam", btg_camera), ("light", btg_area_light), ("tri", C.c_void_p),
-                ("vis", C.c_void_p), ("E", C.c_void_p), ("Eu", C.c_void_p)]
+                ("vis", C.c_void_p), ("E", C.c_void_p), ("Eu", C.c_void_p), ("camD", C.POINTER(btg_camera_d))]
 
 
 STAT_FIELDS = ["kdSteps", "leafVisits", "triTests", "hits", "splits", "beams", "rootBeams", "presplitBeams",
@@ -261,13 +266,23 @@ class GpuRenderer:
         return int(lib().btg_scene_bytes(self._h, int(scene_id)))
 
     def _jobs(self, jobs, outs=None):
+        """The jobs as btg_soft_job records (a job's optional "cam_d": its camera in 14 doubles, for
+        precision 2); the double cameras are kept alive on the array."""
         arr = (btg_soft_job * len(jobs))()
+        arr._camd = []
         for i, j in enumerate(jobs):
             arr[i].scene = int(j.get("scene", 0))
             arr[i].cam = make_camera(j["cam"], int(j["W"]), int(j["H"]))
             arr[i].light = _light(j["light"])
             for f in ("tri", "vis", "E", "Eu"):
                 setattr(arr[i], f, outs[i][f].ctypes.data if outs else None)
+            if j.get("cam_d") is not None:
+                c = np.ascontiguousarray(j["cam_d"], np.float64)
+                if c.shape != (14,):
+                    raise ValueError("cam_d must hold 14 doubles (eye, fwd, right, down, halfW, halfH)")
+                cd = btg_camera_d.from_buffer_copy(c.tobytes())
+                arr._camd.append(cd)
+                arr[i].camD = C.pointer(cd)
         return arr
 
     def autotune(self, jobs, exact: bool = True, max_seconds: float = 20.0):
@@ -335,9 +350,15 @@ class GpuRenderer:
         if rc:
             self._err("btg_set_options", rc)
 
-    def soft_shadows(self, cam14, W: int, H: int, light9, exact: bool = True, download: bool = True) -> dict:
+    def soft_shadows(self, cam14, W: int, H: int, light9, exact: bool = True, download: bool = True,
+                     cam_d=None) -> dict:
         """softShadows (cuda/gpu_render.h:36): tri, vis, E, Eu images (H, W) and run info.
-        download=False leaves the images in device memory (no PCIe transfer)."""
+        download=False leaves the images in device memory (no PCIe transfer). cam_d: the camera in 14
+        doubles (host.camera_d), which precision 2 shoots its primary rays from (a one-job batch)."""
+        if cam_d is not None:
+            outs, info = self.soft_batch([{"scene": 0, "cam": cam14, "W": W, "H": H, "light": light9,
+                                           "cam_d": cam_d}], exact=exact, download=download)
+            return dict(outs[0], info=info)
         n = W * H if download else 0
         tri, vis = np.empty(n, np.int32), np.empty(n, np.float32)
         E, Eu = np.empty(n, np.float32), np.empty(n, np.float32)

@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
         L.bth_kd_sizes.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(C.c_int)]
         L.bth_kd_arrays.argtypes = [vp, C.POINTER(vp), C.POINTER(u32p), C.POINTER(u32p), C.POINTER(f32p), f32p]
         L.bth_camera.argtypes = [f32p, C.c_int, C.c_int, f32p]
+        L.bth_camera_d.argtypes = [f32p, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.bth_scene_save.argtypes = [vp, C.c_char_p]
         L.bth_scene_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.bth_rasterize.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, i32p, i32p, i64, C.c_int, C.c_int, i32p,
@@ -171,6 +172,14 @@ def camera(view10, W: int, H: int) -> np.ndarray:
     v = np.ascontiguousarray(view10, dtype=np.float32)
     out = np.zeros(14, np.float32)
     lib().bth_camera(fptr(v), int(W), int(H), fptr(out))
+    return out
+
+
+def camera_d(view10, W: int, H: int) -> np.ndarray:
+    """The same camera formed in double, as the reference's double build makes it -> 14 float64."""
+    v = np.ascontiguousarray(view10, dtype=np.float32)
+    out = np.zeros(14, np.float64)
+    lib().bth_camera_d(fptr(v), int(W), int(H), out.ctypes.data_as(C.POINTER(C.c_double)))
     return out
 
 

@@ -2,6 +2,7 @@
 // what callers without a C++ scene library (Python, tests, the benchmark) use
 // to produce the arrays btg_create() uploads. See include/bth.h.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -245,6 +246,29 @@ void bth_camera(const float* view10, int W, int H, float* cam14) {
   const Camera c = Camera::make(viewFrom(view10), W, H);
   const float v[14] = {c.eye.x,   c.eye.y,   c.eye.z,  c.fwd.x,  c.fwd.y,  c.fwd.z,  c.right.x,
                        c.right.y, c.right.z, c.down.x, c.down.y, c.down.z, c.halfW, c.halfH};
+  std::memcpy(cam14, v, sizeof v);
+}
+
+// render.cpp:14-25 with Real = double: the view's floats widened, then every
+// step in double (vec.h's dot, cross, normalize; tan of fovY in double).
+void bth_camera_d(const float* view10, int W, int H, double* cam14) {
+  struct D {
+    double x, y, z;
+  };
+  const auto cross = [](D a, D b) { return D{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; };
+  const auto unit = [](D a) {
+    const double l = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return l > 0 ? D{a.x / l, a.y / l, a.z / l} : a;
+  };
+  const D eye{view10[0], view10[1], view10[2]}, target{view10[3], view10[4], view10[5]};
+  const D up{view10[6], view10[7], view10[8]};
+  const D fwd = unit(D{target.x - eye.x, target.y - eye.y, target.z - eye.z});
+  const D right = unit(cross(fwd, up));
+  const D down = cross(fwd, right);
+  const double halfH = std::tan(double(view10[9]) * M_PI / 360.0);
+  const double halfW = halfH * double(W) / double(H);
+  const double v[14] = {eye.x, eye.y, eye.z, fwd.x, fwd.y, fwd.z, right.x, right.y, right.z,
+                        down.x, down.y, down.z, halfW, halfH};
   std::memcpy(cam14, v, sizeof v);
 }
 

@@ -91,7 +91,11 @@ typedef struct {
                          the double build's offset (1e-9 x scene extent) from it, formed in double and carried as
                          a float pair in every apex-relative difference; the Lambert integral runs in double throughout.
                          Results approach bt_render_d's (mean visibility error 100-1000x below the float build's
-                         on the test scenes; bench.py reports it). Same throughput as 0. */
+                         on the test scenes; bench.py reports it). Same throughput as 0.
+                         2 = FP64: the double build's algorithm (bt_render_d: depth-first beam trace, its
+                         tolerances, its geometry in double) per pixel, from the primary hit in double; with the
+                         job's double camera (btg_soft_job.camD) every visible fraction is bt_render_d's to
+                         ~1e-10. Tens of times slower than 0; not served (btg_server_create refuses it). */
   int useGraphs;      /* 1 (default): each group of roundsPerCheck rounds is captured once per pass into a HIP graph
                          and replayed (no counterpart in the reference, which launches every kernel from the host,
                          cuda/wavefront.cu:641-676); 0: plain launches. Results are the same either way.
@@ -116,6 +120,17 @@ typedef struct {
   float halfW, halfH;
   int W, H;
 } btg_camera;
+
+/* The camera in double, as the reference's double build forms it
+ * (src/render/render_types.h:15-39 with Real = double; include/bth.h's
+ * bth_camera_d makes it from a view). Only FP64 soft shadows
+ * (btg_options.precision = 2) read it: their primary rays then are the
+ * double build's, which the float camera's rounding would move by ~1e-8 of
+ * the distance (up to ~4e-5 of visible fraction at sharp penumbrae). */
+typedef struct {
+  double eye[3], fwd[3], right[3], down[3];
+  double halfW, halfH;
+} btg_camera_d;
 
 /* AreaLight (src/render/soft_core.h:10-14): centre and full edge vectors. */
 typedef struct {
@@ -185,6 +200,8 @@ typedef struct {
   float* vis;
   float* E;
   float* Eu;
+  const btg_camera_d* camD; /* NULL, or the same camera in double: with precision = 2 the primary rays are its
+                               (W and H stay cam's); other precisions ignore it */
 } btg_soft_job;
 
 /* GpuRenderer::softShadows (cuda/gpu_render.h:36) for many jobs at once:

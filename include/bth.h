@@ -65,6 +65,12 @@ int bth_scene_load(const char* path, bth_scene** out);
  * fwd, right, down (3 floats each), halfW, halfH. */
 void bth_camera(const float* view10, int W, int H, float* cam14);
 
+/* The same camera formed in double, as the reference's double build makes it
+ * (Real = double) from the same view: eye, fwd, right, down, halfW, halfH in
+ * 14 doubles. FP64 soft shadows (btg_options.precision = 2) shoot their
+ * primary rays from it when a job carries it (btg_soft_job.camD). */
+void bth_camera_d(const float* view10, int W, int H, double* cam14);
+
 /* rasterize (src/render/render.cpp:49-95), one sample at each pixel centre.
  * Polygons: x4/y4 4 floats each, n corners, tri id. canonical != 0 sorts the
  * polygons into a canonical order first (deterministic ties on shared edges).

@@ -133,3 +133,34 @@ def test_create_rejects_parent_links_that_do_not_match_the_children():
     nodes[2]["data"] = (1 << 2) | 0  # node 2's right child points back at node 1 (not after its left child)
     rc, msg = _create_with(arrays, nodes, parent)
     assert rc < 0 and b"child index" in msg
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every public struct the Python binding mirrors has the header's size and
+    field offsets (compiled with the host C compiler)."""
+    import shutil
+    import subprocess
+    from beamtrace2007_amd import gpu
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = [v for k, v in vars(gpu).items()
+               if k.startswith("btg_") and isinstance(v, type) and issubclass(v, ctypes.Structure)]
+    assert len(structs) >= 10
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "btg.h"', "int main(void) {"]
+    for st in structs:
+        lines.append(f'  printf("{st.__name__} size %zu\\n", sizeof({st.__name__}));')
+        for f in st._fields_:
+            lines.append(f'  printf("{st.__name__} {f[0]} %zu\\n", offsetof({st.__name__}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True,
+                   capture_output=True, timeout=120)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for st in structs:
+        assert got[(st.__name__, "size")] == ctypes.sizeof(st), st.__name__
+        for f in st._fields_:
+            assert got[(st.__name__, f[0])] == getattr(st, f[0]).offset, (st.__name__, f[0])

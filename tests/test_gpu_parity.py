@@ -348,13 +348,14 @@ FP64_CASES = [("plant", 1.0, 48), ("atrium", 4.0, 48), ("conference", 1.0, 48), 
 def test_fp64_soft_shadows_match_the_double_build(require_gpu, name, scale, side):
     """precision = 2 (the whole beam trace in double with the double build's
     tolerances) against bt_render_d's algorithm (the reference compiled with
-    Real = double, oracle/_ref): on every pixel with the same primary hit the
-    visible fraction is within 1e-5 (tolerance of the precision target; what
-    differs is the camera: its vectors are floats here and doubles in the
-    double build, which moves receiver points by ~1e-8 of the distance and
-    visible fractions by up to ~6e-6 at sharp penumbrae) and the exact
-    irradiance within 1e-5 of the image's largest."""
-    from beamtrace2007_amd.host import camera
+    Real = double, oracle/_ref), on every pixel with the same primary hit:
+    * with the double build's camera (camera_d): the visible fraction and the
+      exact irradiance agree to 1e-6 (the float32 rounding of the outputs;
+      the precision target is 1e-5);
+    * with the float camera alone: within 1e-4 (its vectors' rounding moves
+      receiver points by ~1e-8 of the distance: up to ~4e-5 of visible
+      fraction at sharp penumbrae)."""
+    from beamtrace2007_amd.host import camera, camera_d
     from beamtrace2007_amd.workloads import scaled_light
     from refpy import Ref, ref_available
     if not ref_available("d"):
@@ -368,17 +369,36 @@ def test_fp64_soft_shadows_match_the_double_build(require_gpu, name, scale, side
     r.set_options(precision=2)
     try:
         print(f"fp64 {name} {W}x{H} light x{scale}", flush=True)
-        g = r.soft_shadows(cam, W, H, light, exact=True)
+        g = r.soft_shadows(cam, W, H, light, exact=True, cam_d=camera_d(sp["view"], W, H))
+        gf = r.soft_shadows(cam, W, H, light, exact=True)
     finally:
         r.set_options(**saved)
     refd = Ref("d").load(a["pos"], a["tris"])
     d = refd.soft(sp["view"], W, H, light, exact=True, mailbox=False, trail=True, threads=16)
     refd.free()
-    m = (d["vis"] >= 0) & (g["vis"] >= 0) & (g["tri"] == d["tri"])
-    assert m.sum() >= 0.9 * (d["vis"] >= 0).sum()  # the same primary hit on nearly every lit pixel
-    err = np.abs(g["vis"].astype(np.float64) - d["vis"])[m]
-    e_err = np.abs(g["E"].astype(np.float64) - d["E"])[m] / max(float(np.abs(d["E"][m]).max()), 1e-30)
-    print(f"fp64 {name}: vis max {err.max():.2e} mean {err.mean():.2e}; E max {e_err.max():.2e}", flush=True)
-    assert err.max() <= 1e-5
-    assert e_err.max() <= 1e-5
-    assert np.array_equal(g["vis"] < 0, g["tri"] < 0)
+    for img, tol, what in ((g, 1e-6, "double camera"), (gf, 1e-4, "float camera")):
+        m = (d["vis"] >= 0) & (img["vis"] >= 0) & (img["tri"] == d["tri"])
+        assert m.sum() >= 0.9 * (d["vis"] >= 0).sum()  # the same primary hit on nearly every lit pixel
+        err = np.abs(img["vis"].astype(np.float64) - d["vis"])[m]
+        e_err = np.abs(img["E"].astype(np.float64) - d["E"])[m] / max(float(np.abs(d["E"][m]).max()), 1e-30)
+        print(f"fp64 {name} ({what}): vis max {err.max():.2e} mean {err.mean():.2e}; E max {e_err.max():.2e}",
+              flush=True)
+        assert err.max() <= tol
+        assert e_err.max() <= max(tol, 1e-6)
+        assert np.array_equal(img["vis"] < 0, img["tri"] < 0)
+    assert np.array_equal(g["Eu"] >= 0, gf["Eu"] >= 0)
+
+
+def test_fp64_mode_is_not_served(require_gpu):
+    """A server runs its passes in the float modes: on a context set to
+    precision 2 btg_server_create refuses with a message that names the
+    entry points that run it."""
+    from beamtrace2007_amd.gpu import Server
+    r = renderer("plant")
+    saved = r.options
+    r.set_options(precision=2)
+    try:
+        with pytest.raises(RuntimeError, match="precision 2"):
+            Server(r)
+    finally:
+        r.set_options(**saved)
