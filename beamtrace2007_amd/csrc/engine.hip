@@ -2434,9 +2434,14 @@ constexpr int kFp64Waves = BTG_FP64_WAVES;
 // split into finer cells (build knob BTG_FP64_WORK_CAP): bounds the time one
 // lane spends on one task, which sets the pass's tail.
 #ifndef BTG_FP64_WORK_CAP
-#define BTG_FP64_WORK_CAP 65536
+#define BTG_FP64_WORK_CAP 2048
 #endif
 constexpr uint32_t kFp64WorkCap = BTG_FP64_WORK_CAP;
+// Beams and work items of a level-1 cell's pools (build knob BTG_FP64_POOL1).
+#ifndef BTG_FP64_POOL1
+#define BTG_FP64_POOL1 128
+#endif
+constexpr int kFp64Pool1 = BTG_FP64_POOL1;
 
 __global__ void zeroWords(unsigned long long* w, size_t n) {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) w[i] = 0;
@@ -2457,9 +2462,8 @@ __global__ void __launch_bounds__(64, kFp64Waves)
                    long long* out, int* outCount, int outCap, unsigned long long* visSum, unsigned long long* gSum,
                    int* failed) {
   using TR = fp64::Tracer<Pn, Kn>;
-  Cnt cnt;
-  cnt.zero();
   TR tr;
+  tr.cnt.zero();
   tr.m = fp64::WordMem{mem + size_t(blockIdx.x) * size_t(TR::kWords) * 64, 64, int(threadIdx.x)};
   const int tasks = Level == 0 ? numPix : 16 * min(*inCount, inCap);
   for (;;) {
@@ -2488,7 +2492,7 @@ __global__ void __launch_bounds__(64, kFp64Waves)
     if (R.root.n < 3) continue;
     double miss, G;
     if (fp64::traceCell(tr, p.sv, R, T, p.exact != 0, Level, cx, cy, Level == 2 ? 0xffffffffu : kFp64WorkCap,
-                        J.node0, J.bounds, cnt, miss, G)) {
+                        J.node0, J.bounds, miss, G)) {
       atomicAdd(&visSum[pix], fp64::visFx(miss / R.lightArea));
       if (p.exact) atomicAdd(&gSum[pix], fp64::gFx(G));
     } else if (Level < 2) {
@@ -2501,7 +2505,7 @@ __global__ void __launch_bounds__(64, kFp64Waves)
       atomicAdd(failed, 1);
     }
   }
-  flushCnt(p, cnt);
+  flushCnt(p, tr.cnt);
 }
 
 // The FP64 images from the pixels' sums: vis = visible fraction (-1 without
@@ -3222,7 +3226,7 @@ namespace {
 // reads the cells that overflowed their last level.
 int softFp64(Engine& e, Params& p, size_t np, std::string& err) {
   using T0 = fp64::Tracer<64, 64>;
-  using T1 = fp64::Tracer<128, 128>;
+  using T1 = fp64::Tracer<kFp64Pool1, kFp64Pool1>;
   using T2 = fp64::Tracer<384, 256>;
   constexpr size_t kBlock0 = size_t(T0::kWords) * 64 * 8, kBlock1 = size_t(T1::kWords) * 64 * 8,
                    kBlock2 = size_t(T2::kWords) * 64 * 8;
@@ -3250,7 +3254,7 @@ int softFp64(Engine& e, Params& p, size_t np, std::string& err) {
                                                                     ctr + 3, n, visSum, gSum, ctr + 5);
   e.endLaunch(lt);
   lt = e.beginLaunch(1);
-  fp64CellKernel<128, 128, 1><<<unsigned(blocks1), 64, 0, e.stream>>>(p, n, list1, ctr + 3, n, ctr + 1, mem, list2,
+  fp64CellKernel<kFp64Pool1, kFp64Pool1, 1><<<unsigned(blocks1), 64, 0, e.stream>>>(p, n, list1, ctr + 3, n, ctr + 1, mem, list2,
                                                                       ctr + 4, int(cap2), visSum, gSum, ctr + 5);
   e.endLaunch(lt);
   lt = e.beginLaunch(1);

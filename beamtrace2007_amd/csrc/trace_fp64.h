@@ -393,7 +393,7 @@ struct Tracer {
   bool overflow;        // a pool, the work stack or the work cap ran out: the trace is void
   uint32_t work;        // triangle tests and kd steps of this trace
   uint32_t workCap = 0xffffffffu;
-  Cnt* cnt;
+  Cnt cnt;  // counters of every trace this tracer ran (registers: the owner reads them at the end)
   int nFree, nPool;
   uint32_t oCur, oNxt;  // oCurA / oCurB, swapped after each triangle of a leaf
   int nCur, nNxt;
@@ -669,7 +669,7 @@ struct Tracer {
         if (in.n == 5) {
           Poly a, b;
           splitFive(in, a, b);
-          cnt->v[sFive]++;
+          cnt.v[sFive]++;
           if (sp >= 8) return false;
           m.store(oClip + uint32_t(sp) * wPoly, b);
           m.storeInt(oClip + 8 * wPoly + uint32_t(sp), li + 1);
@@ -694,8 +694,8 @@ struct Tracer {
       p.y[i] = b.qy.v[i];
     }
     const double area = p.area();
-    if (tri >= 0) cnt->v[sHits]++;
-    cnt->v[sBeams]++;
+    if (tri >= 0) cnt.v[sHits]++;
+    cnt.v[sBeams]++;
     if (tri >= 0) return;
     missArea += area;
     if (!exact) return;
@@ -719,7 +719,7 @@ struct Tracer {
     int np = 1;
     if (p0.n == 5) {
       splitFive(p0, pieces[0], pieces[1]);
-      cnt->v[sFive]++;
+      cnt.v[sFive]++;
       np = 2;
     } else {
       pieces[0] = p0;
@@ -744,7 +744,7 @@ struct Tracer {
       overflow = true;
       return;
     }
-    cnt->v[sTests]++;
+    cnt.v[sTests]++;
     Line L[5];
     int nl = 0;
     const Beam B = beam(bi);
@@ -768,33 +768,12 @@ struct Tracer {
       pushNext(bi);
       return;
     }
-    cnt->v[sSplits]++;
+    cnt.v[sSplits]++;
     release(bi);
     // lists are reused by the next clip: copy out before emitting
     const int n0 = nList[0], n1 = nList[1];
     for (int k = 0; k < n0; ++k) emit(piece(0, k), B, false, t);
     for (int k = 0; k < n1; ++k) emit(piece(1, k), B, true, t);
-  }
-
-  // processLeaf without mailboxes (beam_core.h:884-912).
-  FP64_HD void processLeaf(int bi, const KdNodeD& leaf) {
-    cnt->v[sLeaf]++;
-    nCur = 1;
-    m.storeInt(oCur, bi);
-    const uint32_t n = leaf.data >> 2;
-    for (uint32_t i = 0; i < n && !overflow; ++i) {
-      const LeafTri ref = sv->refs[leaf.offset + i];
-      if (ref.tri == exclude) continue;
-      const Tri ti = setupTri(ref);
-      if (!ti.ok) continue;
-      nNxt = 0;
-      for (int k = 0; k < nCur && !overflow; ++k) intersect(m.loadInt(oCur + uint32_t(k)), ref.tri, ti);
-      const uint32_t t = oCur;
-      oCur = oNxt;
-      oNxt = t;
-      nCur = nNxt;
-      if (nCur == 0) break;
-    }
   }
 
   // advance with the restart trail (beam_core.h:924-969).
@@ -805,7 +784,7 @@ struct Tracer {
       const int d = 63 - __builtin_clzll(trail);
       for (; depth > d; --depth) {
         node = sv->parent[node];
-        cnt->v[sClimbs]++;
+        cnt.v[sClimbs]++;
       }
       trail &= (uint64_t(1) << d) - 1;
       const KdNodeD kn = sv->nodes[node];
@@ -825,7 +804,7 @@ struct Tracer {
     const int idx = alloc();
     if (idx < 0) return;
     putBeam(idx, b);
-    cnt->v[sRoots]++;
+    cnt.v[sRoots]++;
     if (overlaps(b, awOf(b), rootBox))
       pushWork(Work{idx, node0, 0, 0, rootBox});
     else
@@ -874,7 +853,7 @@ struct Tracer {
         b.sgn[axis] = -1;
         m.store(oPre + uint32_t(sp++) * wItem, b);
         m.store(oPre + uint32_t(sp++) * wItem, a);
-        cnt->v[sPresplit]++;
+        cnt.v[sPresplit]++;
       } else if (c != 0) {
         it.axis = axis + 1;
         it.sgn[axis] = (c & 1) ? 1 : -1;
@@ -889,7 +868,13 @@ struct Tracer {
 
   // BeamCore::trace with the restart trail (beam_core.h:569-661) of the root
   // polygon rootIn; `extent` sets the tolerances (the extent of the whole
-  // light's root polygon, also when rootIn is a cell of it).
+  // light's root polygon, also when rootIn is a cell of it). The depth-first
+  // order of the reference, written as one loop of small steps (take a work
+  // item; one kd step; one beam against one triangle of the leaf
+  // (processLeaf without mailboxes, beam_core.h:884-912); advance one of the
+  // leaf's surviving beams) so that the lanes of a wave in the same kind of
+  // step run together, instead of each waiting for the slowest lane's whole
+  // descent or leaf.
   FP64_HD void trace(const Poly& rootIn, uint32_t node0, const Box& rootBox, double extent) {
     setup(0.0 > extent ? 0.0 : extent);
     Poly p = rootIn;
@@ -897,60 +882,117 @@ struct Tracer {
       if (p.area() < 0) p.reverse();
       presplit(p, node0, rootBox);
     }
-    while (nWork > 0 && !overflow) {
-      const Work wk = m.load<Work>(oWork + uint32_t(--nWork) * wWork);
-      uint32_t node = wk.node;
-      Box box = wk.box;
-      int depth = wk.depth;
-      uint64_t trail = wk.trail;
-      const Beam B = beam(wk.beam);
-      const Aw w = awOf(B);
-      for (;;) {
+    enum { kTake, kDescend, kTest, kAdvance };
+    int state = kTake;
+    int bi = 0, depth = 0;  // the work item: its beam, node, box, depth and restart trail
+    uint32_t node = 0;
+    Box box = rootBox;
+    uint64_t trail = 0;
+    Beam B;
+    Aw w;
+    uint32_t leafOff = 0, leafN = 0, li = 0;  // the leaf's references and the next one to take
+    bool haveTri = false;                     // ti / triId: the triangle the survivors meet now
+    Tri ti;
+    int triId = 0, k = 0;                     // the next survivor to test or advance
+    while (!overflow) {
+      if (state == kTake) {
+        if (nWork == 0) break;
+        const Work wk = m.load<Work>(oWork + uint32_t(--nWork) * wWork);
+        bi = wk.beam;
+        node = wk.node;
+        box = wk.box;
+        depth = wk.depth;
+        trail = wk.trail;
+        B = beam(bi);
+        w = awOf(B);
+        state = kDescend;
+      }
+      if (state == kDescend) {
         const KdNodeD kn = sv->nodes[node];
-        if ((kn.data & 3u) == 3u) break;
-        if (++work > workCap) {
-          overflow = true;
-          break;
-        }
-        cnt->v[sKd]++;
-        const int d = depth++;
-        const int a = int(kn.data & 3u);
-        const float s = kn.split;
-        const double Ns = double(B.sgn[a]) * Q.rel(double(s), a);
-        const uint32_t left = node + 1, right = kn.data >> 2;
-        Box lb = box, rb = box;
-        setComp(lb.hi, a, s);
-        setComp(rb.lo, a, s);
-        const bool toRight = B.sgn[a] > 0;
-        const uint32_t cs = toRight ? right : left, co = toRight ? left : right;
-        if (Ns <= 0) {
-          node = cs;
-          box = toRight ? rb : lb;
-          continue;
-        }
-        const int dec = decide(B, w, a, Ns, box);
-        if (dec == 0) {
-          node = co;
-          box = toRight ? lb : rb;
-        } else if (dec == 1) {
-          node = cs;
-          box = toRight ? rb : lb;
+        if ((kn.data & 3u) == 3u) {
+          cnt.v[sLeaf]++;
+          if ((kn.data >> 2) == 0) {
+            advance(bi, node, depth, trail);
+            state = kTake;
+          } else {
+            nCur = 1;
+            m.storeInt(oCur, bi);
+            leafOff = kn.offset;
+            leafN = kn.data >> 2;
+            li = 0;
+            haveTri = false;
+            state = kTest;
+          }
         } else {
-          trail |= uint64_t(1) << d;
-          node = co;
-          box = toRight ? lb : rb;
+          if (++work > workCap) {
+            overflow = true;
+            break;
+          }
+          cnt.v[sKd]++;
+          const int d = depth++;
+          const int a = int(kn.data & 3u);
+          const float s = kn.split;
+          const double Ns = double(B.sgn[a]) * Q.rel(double(s), a);
+          const uint32_t left = node + 1, right = kn.data >> 2;
+          Box lb = box, rb = box;
+          setComp(lb.hi, a, s);
+          setComp(rb.lo, a, s);
+          const bool toRight = B.sgn[a] > 0;
+          const uint32_t cs = toRight ? right : left, co = toRight ? left : right;
+          if (Ns <= 0) {
+            node = cs;
+            box = toRight ? rb : lb;
+          } else {
+            const int dec = decide(B, w, a, Ns, box);
+            if (dec == 0) {
+              node = co;
+              box = toRight ? lb : rb;
+            } else if (dec == 1) {
+              node = cs;
+              box = toRight ? rb : lb;
+            } else {
+              trail |= uint64_t(1) << d;
+              node = co;
+              box = toRight ? lb : rb;
+            }
+          }
         }
       }
-      if (overflow) break;
-      const KdNodeD leaf = sv->nodes[node];
-      if ((leaf.data >> 2) == 0) {
-        cnt->v[sLeaf]++;
-        advance(wk.beam, node, depth, trail);
-        continue;
+      if (state == kTest) {
+        while (!haveTri && li < leafN) {  // the leaf's next triangle the survivors can meet
+          const LeafTri ref = sv->refs[leafOff + li];
+          if (ref.tri != exclude) {
+            ti = setupTri(ref);
+            if (ti.ok) {
+              haveTri = true;
+              triId = ref.tri;
+              nNxt = 0;
+              k = 0;
+              break;
+            }
+          }
+          ++li;
+        }
+        if (!haveTri) {
+          k = 0;
+          state = kAdvance;
+        } else {
+          intersect(m.loadInt(oCur + uint32_t(k)), triId, ti);
+          if (++k == nCur) {
+            const uint32_t t = oCur;
+            oCur = oNxt;
+            oNxt = t;
+            nCur = nNxt;
+            haveTri = false;
+            ++li;
+            if (nCur == 0) li = leafN;
+          }
+        }
       }
-      processLeaf(wk.beam, leaf);
-      const int n = nCur;
-      for (int k = 0; k < n && !overflow; ++k) advance(m.loadInt(oCur + uint32_t(k)), node, depth, trail);
+      if (state == kAdvance) {
+        if (k < nCur) advance(m.loadInt(oCur + uint32_t(k++)), node, depth, trail);
+        if (k >= nCur) state = kTake;
+      }
     }
   }
 };
@@ -1055,7 +1097,7 @@ FP64_HD inline Poly cellOf(const Poly& root, int level, int cx, int cy) {
 // Otherwise its missed area and (exact) Lambert integral are in miss and G.
 template <class TR>
 FP64_HD bool traceCell(TR& tr, const DevScene& sv, const PixelRoot& R, int exclude, bool exact, int level, int cx,
-                       int cy, uint32_t workCap, uint32_t node0, const Box& bounds, Cnt& cnt, double& miss,
+                       int cy, uint32_t workCap, uint32_t node0, const Box& bounds, double& miss,
                        double& G) {
   miss = G = 0;
   const Poly cell = cellOf(R.root, level, cx, cy);
@@ -1066,7 +1108,6 @@ FP64_HD bool traceCell(TR& tr, const DevScene& sv, const PixelRoot& R, int exclu
   tr.x = R.x;
   tr.ns = R.ns;
   tr.exact = exact;
-  tr.cnt = &cnt;
   tr.workCap = workCap;
   tr.trace(cell, node0, bounds, R.ext);
   if (tr.overflow) return false;

@@ -55,8 +55,9 @@ int shadeAll(const void* refs, const void* nodes, const float* nodeBox, const ui
   t0->m = fp64::WordMem{mem0.data(), 1, 0};
   t1->m = fp64::WordMem{mem1.data(), 1, 0};
   t2->m = fp64::WordMem{mem2.data(), 1, 0};
-  Cnt cnt;
-  std::memset(&cnt, 0, sizeof cnt);
+  std::memset(&t0->cnt, 0, sizeof(Cnt));
+  std::memset(&t1->cnt, 0, sizeof(Cnt));
+  std::memset(&t2->cnt, 0, sizeof(Cnt));
   levels[0] = levels[1] = 0;
   for (int i = 0; i < n; ++i) {
     const fp64::V xi = fp64::vmk(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
@@ -73,20 +74,20 @@ int shadeAll(const void* refs, const void* nodes, const float* nodeBox, const ui
       vs += fp64::visFx(miss / R.lightArea);
       if (exact) gs += fp64::gFx(G);
     };
-    if (fp64::traceCell(*t0, sv, R, r.tri, exact != 0, 0, 0, 0, workCap, 0u, root, cnt, miss, G)) {
+    if (fp64::traceCell(*t0, sv, R, r.tri, exact != 0, 0, 0, 0, workCap, 0u, root, miss, G)) {
       add();
     } else {
       ++levels[0];
       for (int c1 = 0; c1 < 16; ++c1) {
         const int cx1 = c1 & 3, cy1 = c1 >> 2;
-        if (fp64::traceCell(*t1, sv, R, r.tri, exact != 0, 1, cx1, cy1, workCap, 0u, root, cnt, miss, G)) {
+        if (fp64::traceCell(*t1, sv, R, r.tri, exact != 0, 1, cx1, cy1, workCap, 0u, root, miss, G)) {
           add();
           continue;
         }
         ++levels[1];
         for (int s = 0; s < 16; ++s) {
           if (!fp64::traceCell(*t2, sv, R, r.tri, exact != 0, 2, cx1 * 4 + (s & 3), cy1 * 4 + (s >> 2), 0xffffffffu,
-                               0u, root, cnt, miss, G))
+                               0u, root, miss, G))
             return -1 - i;
           add();
         }
@@ -95,7 +96,7 @@ int shadeAll(const void* refs, const void* nodes, const float* nodeBox, const ui
     vis[i] = fp64::visOfFx(vs);
     E[i] = exact ? fp64::gOfFx(gs) : R.A * R.G0 * vis[i];
   }
-  for (int k = 0; k < sNum; ++k) counters[k] = cnt.v[k];
+  for (int k = 0; k < sNum; ++k) counters[k] = uint64_t(t0->cnt.v[k]) + t1->cnt.v[k] + t2->cnt.v[k];
   return 0;
 }
 
