@@ -150,7 +150,7 @@ def main() -> int:
         print(json.dumps(res), flush=True)
         return 0
     from beamtrace2007_amd import gpu
-    from beamtrace2007_amd.host import Scene, camera
+    from beamtrace2007_amd.host import Scene, camera, camera_d
 
     scene_name, res0, desc = WORKLOADS[args.workload]
     res = args.res or res0
@@ -212,6 +212,11 @@ def main() -> int:
     pcie_s = time.perf_counter() - t1
     r.set_options(precision=1)
     guarded = r.soft_shadows(cam, W, H, light, exact=True, download=True)
+    # FP64 mode from the double build's camera (one frame, images downloaded)
+    r.set_options(precision=2)
+    t1 = time.perf_counter()
+    fp64 = r.soft_shadows(cam, W, H, light, exact=True, download=True, cam_d=camera_d(sp["view"], W, H))
+    fp64_s = time.perf_counter() - t1
     r.close()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
@@ -226,10 +231,12 @@ def main() -> int:
         "kernel_ms_per_step": kernel_ms, "alg_bytes_per_step": int(b_job),
         "pcie_inclusive_px_per_s": round(px / pcie_s, 1),  # one frame with tri/vis/E/Eu copied to pageable host memory
         "roofline": roofline, "host_kd_build_s": round(build_s, 3),
+        "fp64_mode": {"pixels_per_s": round(px / fp64_s, 1), "seconds": round(fp64_s, 3),
+                      "mode": "precision 2 from the double build's camera, one frame, images downloaded"},
     }
     if any(infos[-1]["phase_cycles"].values()):  # instrumented builds only
         out["trace_phase_cycles"] = infos[-1]["phase_cycles"]
-    out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded))
+    out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded, fp64))
     print(json.dumps(out), flush=True)
     return 0
 
@@ -386,12 +393,15 @@ def bench_batch(args) -> int:
     return 0
 
 
-def batch_precision(r, scenes, jobs, px) -> dict | None:
-    """Error against the reference's double build, stated for both modes on a
-    row sample (the middle row of the first two jobs of each scene), and the
-    FP64-guarded mode's batched throughput on the full batch."""
+def batch_precision(r, scenes, jobs, px, side: int = 128) -> dict | None:
+    """The FP64-guarded mode's batched throughput on the full batch, and every
+    mode's error against the reference's double build on a sample: the first
+    two jobs of each scene (their scene, view and light) re-rendered at
+    side x side and compared on whole frames (modes 0 and 1 from the float
+    camera, mode 2 from the double build's camera, all three batched)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from refpy import Ref, ref_available
+    from beamtrace2007_amd.host import camera, camera_d
 
     saved = r.options
     r.set_options(precision=1)
@@ -405,44 +415,42 @@ def batch_precision(r, scenes, jobs, px) -> dict | None:
                                "carried as a float pair, FP64 Lambert integral"}}
     if not ref_available("d"):
         return out
-    sample = [j for n in ("blob", "atrium") for j in [x for x in jobs if x["name"] == n][:2]]
-    res = {}
-    for mode in (0, 1):
+    names = sorted({j["name"] for j in jobs})
+    sample = []
+    for n in names:
+        for j in [x for x in jobs if x["name"] == n][:2]:
+            view = scenes[n].spec()["view"]
+            sample.append(dict(j, W=side, H=side, cam=camera(view, side, side), cam_d=camera_d(view, side, side)))
+    res, secs = {}, {}
+    for mode in (0, 1, 2):
         r.set_options(precision=mode)
+        t0 = time.perf_counter()
         res[mode] = r.soft_batch(sample, exact=True, download=True)[0]
+        secs[mode] = time.perf_counter() - t0
     r.set_options(**saved)
-    errs = {0: [], 1: []}
-    eerrs = {0: [], 1: []}
     threads, _ = host_cores()
-    refs = {}
-    for j, o0, o1 in zip(sample, res[0], res[1]):
+    refs, dimg = {}, []
+    t0 = time.perf_counter()
+    for j in sample:
         name = j["name"]
         if name not in refs:
             a = scenes[name].arrays()
             refs[name] = Ref("d").load(a["pos"], a["tris"])
-        y = j["H"] // 2
-        rr = refs[name].soft_rows(scenes[name].spec()["view"], j["W"], j["H"], j["light"], y, y + 1, exact=True,
-                                  mailbox=False, trail=True, threads=threads)
-        d, dtri, dE = rr["vis"][0], rr["tri"][0], rr["E"][0].astype(np.float64)
-        for mode, o in ((0, o0), (1, o1)):
-            g = o["vis"][y].astype(np.float64)
-            m = (d >= 0) & (g >= 0) & (o["tri"][y] == dtri)  # same primary hit: the shadow computation compared
-            errs[mode].append(np.abs(g - d)[m])
-            if m.any():  # irradiance, relative to the row's largest exact E
-                eerrs[mode].append(np.abs(o["E"][y].astype(np.float64) - dE)[m] / max(np.abs(dE[m]).max(), 1e-30))
+        dimg.append(refs[name].soft(scenes[name].spec()["view"], side, side, j["light"], exact=True, mailbox=False,
+                                    trail=True, threads=threads))
+    d_s = time.perf_counter() - t0
     for rf in refs.values():
         rf.free()
-
-    def summary(e):
-        e = np.concatenate(e)
-        return {"max": float(e.max()), "mean": float(e.mean()), "frac_above_1e-5": float(np.mean(e > 1e-5)),
-                "pixels": int(e.size)}
-
-    out["vis_err_vs_fp64"] = {"float_build": summary(errs[0]), "guarded": summary(errs[1]),
-                              "sample": "middle rows of the first two jobs of each scene, pixels with the same primary hit, against bt_render_d's "
-                                        "algorithm (the reference compiled with Real = double)"}
-    out["E_err_vs_fp64"] = {"float_build": summary(eerrs[0]), "guarded": summary(eerrs[1]),
-                            "unit": "|E - E_fp64| / the row's largest E_fp64", "sample": "the same pixels"}
+    cat = {k: np.concatenate([im[k].ravel() for im in dimg]) for k in ("tri", "vis", "E")}
+    stated = {}
+    for mode, key in ((0, "0_float_build"), (1, "1_guarded"), (2, "2_fp64")):
+        stated[key] = err_vs_fp64({k: np.concatenate([o[k].ravel() for o in res[mode]]) for k in ("tri", "vis", "E")},
+                                  cat)
+        stated[key]["sample_px_per_s"] = round(len(sample) * side * side / secs[mode], 1)
+    stated["against"] = (f"bt_render_d's algorithm (the reference compiled with Real = double) on whole frames: "
+                         f"the first two jobs of each scene ({', '.join(names)}) at {side}x{side}, their lights and "
+                         f"views; {d_s:.1f} s on {threads} threads")
+    out["precision"] = stated
     return out
 
 
@@ -573,6 +581,8 @@ def bench_serve(args) -> dict | None:
     def pct(v, q):
         return round(float(np.percentile(np.array(v), q)) * 1e3, 2) if len(v) else None
 
+    kept = []  # served soft results kept for their error against the double build
+
     def serve(rate):
         srv = gpu.Server(r)
         for t in (srv.submit_soft(0, camera(sp["view"], 64, 64), 64, 64, sp["area_light"]),
@@ -609,6 +619,8 @@ def bench_serve(args) -> dict | None:
                 return None
             lat[d["kind"]].append(res["latency_s"])
             queue.append(res["queue_s"])
+            if d["kind"] == "soft" and len(kept) < 4:
+                kept.append((d["side"], res))
             if time.perf_counter() - last > 10:
                 print(f"bench.py: rate {rate:g}: served {i + 1}/{len(base)}", file=sys.stderr, flush=True)
                 last = time.perf_counter()
@@ -658,15 +670,56 @@ def bench_serve(args) -> dict | None:
                          "primary + pointShadows, polygon lists downloaded)"}
     r.close()
     sat = next((p for p in points if p["qps"] == "saturation"), None)
-    return {"requests": len(base), "arrival_seed": 5, "pixels": total_px,
+    served_err = None if args.no_cpu else served_precision(sc, sp, kept)
+    return {"requests": len(base), "arrival_seed": 5, "pixels": total_px, "precision": served_err,
             "mix": {"soft": sum(d["kind"] == "soft" for d in base), "point": sum(d["kind"] == "point" for d in base)},
             "points": points, "plain": plain,
             "served_over_plain": round(sat["pixels_per_s"] / plain["pixels_per_s"], 3) if sat and plain else None,
             "config": WORKLOADS["serve"][2]}
 
 
-def cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded=None) -> dict:
-    """The CPU baseline and the parity band (test oracles, never the product)."""
+def err_vs_fp64(img: dict, d: dict) -> dict:
+    """Visible fraction and exact irradiance of `img` against the double
+    build's image `d`, on the pixels where both have the same primary hit."""
+    m = (d["vis"] >= 0) & (img["vis"] >= 0) & (img["tri"] == d["tri"])
+    err = np.abs(img["vis"].astype(np.float64) - d["vis"])[m]
+    e_err = np.abs(img["E"].astype(np.float64) - d["E"])[m] / max(float(np.abs(d["E"][m]).max()) if m.any() else 0.0,
+                                                                   1e-30)
+    if not err.size:
+        return {"pixels": 0}
+    return {"max": float(err.max()), "mean": float(err.mean()), "frac_above_1e-5": float(np.mean(err > 1e-5)),
+            "pixels": int(err.size), "same_hit_frac": round(float(m.sum()) / max(int((d["vis"] >= 0).sum()), 1), 6),
+            "E_max_rel": float(e_err.max())}
+
+
+def served_precision(sc, sp, kept, rows: int = 8) -> dict | None:
+    """Served soft results (the server runs the float build's mode) against
+    the reference's double build: `rows` middle rows of each kept result."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from refpy import Ref, ref_available
+    if not ref_available("d") or not kept:
+        return None
+    threads, _ = host_cores()
+    a = sc.arrays()
+    refd = Ref("d").load(a["pos"], a["tris"])
+    imgs, refs = [], []
+    for side, res in kept:
+        y0 = side // 2 - rows // 2
+        rr = refd.soft_rows(sp["view"], side, side, sp["area_light"], y0, y0 + rows, exact=True, mailbox=False,
+                            trail=True, threads=threads)
+        imgs.append({k: res[k][y0:y0 + rows].ravel() for k in ("tri", "vis", "E")})
+        refs.append({k: rr[k].ravel() for k in ("tri", "vis", "E")})
+    refd.free()
+    cat = lambda L: {k: np.concatenate([x[k] for x in L]) for k in ("tri", "vis", "E")}  # noqa: E731
+    out = err_vs_fp64(cat(imgs), cat(refs))
+    out["sample"] = (f"{rows} middle rows of the first {len(kept)} served soft requests (sides "
+                     f"{', '.join(str(k[0]) for k in kept)}), precision 0, against bt_render_d's algorithm")
+    return out
+
+
+def cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded=None, fp64=None) -> dict:
+    """The CPU baseline, the parity band (test oracles, never the product)
+    and every mode's error against the double build on the whole frame."""
     res = {"cpu_baseline": None, "parity": None}
     if args.no_cpu:
         return res
@@ -697,21 +750,22 @@ def cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded=None) -> dict:
     par = {"rows": [py0, py1], "tri_bitexact": bool(np.array_equal(full["tri"][py0:py1], o["tri"][py0:py1])),
            "vis_bitexact_frac": float(np.mean(gv == ov)),
            "max_abs_vis_diff": float(np.max(np.abs(gv.astype(np.float64) - ov)))}
+    res["parity"] = par
     if ref_available("d"):
         refd = Ref("d").load(a["pos"], a["tris"])
-        d = refd.soft_rows(sp["view"], W, H, light, py0, py1, exact=True, mailbox=False, trail=True, threads=threads)
-        m = d["vis"] >= 0
-
-        def err_of(v):
-            err = np.abs(v.astype(np.float64) - d["vis"])[m & (v >= 0)]
-            return {"max": float(err.max()) if err.size else 0.0, "mean": float(err.mean()) if err.size else 0.0,
-                    "frac_above_1e-5": float(np.mean(err > 1e-5)) if err.size else 0.0}
-
-        par["vis_err_vs_fp64"] = err_of(gv)
-        if guarded is not None:
-            par["vis_err_vs_fp64_guarded"] = err_of(guarded["vis"][py0:py1])
+        t0 = time.perf_counter()
+        d = refd.soft(sp["view"], W, H, light, exact=True, mailbox=False, trail=True, threads=threads)
+        d_s = time.perf_counter() - t0
         refd.free()
-    res["parity"] = par
+        modes = {"0_float_build": full}
+        if guarded is not None:
+            modes["1_guarded"] = guarded
+        if fp64 is not None:
+            modes["2_fp64"] = fp64
+        res["precision"] = {k: err_vs_fp64(v, d) for k, v in modes.items()}
+        res["precision"]["against"] = (f"bt_render_d's algorithm (the reference compiled with Real = double) on the "
+                                       f"whole {W}x{H} frame, {d_s:.1f} s on {threads} threads; modes 0 and 1 shoot "
+                                       f"from the float camera, mode 2 from the double build's camera")
     return res
 
 
