@@ -1,7 +1,7 @@
 """Benchmark: exact soft shadows on one MI355X, batched vs plain, with
 served-stream latency.
 
-    python bench.py [--workload batch64|plant|room|serve] [--steps K] [--warmup W]
+    python bench.py [--workload batch64|plant|room|serve|primary|building] [--steps K] [--warmup W]
 
 batch64 (default): one step = one batch of 64 uneven exact soft-shadow jobs
 on two resident scenes (a ~345K-triangle noisy blob and a ~66K-triangle
@@ -19,8 +19,15 @@ primary-ray pass + wavefront shadow beams, as the reference times it:
 pixels / (primary_s + trace_s), scripts/make_report.py:171), with a parity
 band against the oracle and the error against the double build.
 
-Scenes and trees are resident in HBM and images stay in device memory (no
-PCIe in the timed region). The line also reports:
+primary / building: one step = one frame of primary visibility (the room at
+512 x 512 from 8 x 8-pixel tile roots) or of primary visibility plus
+point-light shadows (the 2M-triangle building at 1024 x 1024, 16 x 16 tile
+roots), polygons traced and rasterized into hit ids (and the shadow mask) on
+the device, the images downloaded; the reference CPU build renders the same
+frame beside it and its images are compared bit for bit.
+
+Scenes and trees are resident in HBM and soft-shadow images stay in device
+memory (no PCIe in their timed region). The line also reports:
   roofline      the trace kernel's algorithmic bytes (one 12-B kd node per
                 inner step and leaf visit, one 64-B leaf triangle per test,
                 one 4-B parent link per climb) over its HIP-event time,
@@ -49,6 +56,11 @@ WORKLOADS = {
     "plant": ("plant", 512, "exact soft shadows, seeded procedural plant (curved stem, 292 bent 3x3-grid leaves "
                             "2-6 cm, ground quad; 5378 tris), 0.5x0.5 m area light 1 m above, light-scale 1"),
     "room": ("room", 512, "exact soft shadows, seeded 10x8x3 m room (888 tris), 1x1 m ceiling light"),
+    "primary": ("room", 512, "primary visibility, seeded 10x8x3 m room (888 tris), pinhole camera at the room "
+                             "centre, 8x8-pixel tile roots, hit ids rasterized on the device"),
+    "building": ("building", 1024, "primary visibility + point-light shadows, seeded 7-floor office building (12x8 "
+                                   "rooms per floor, door and window cut-outs, desks and chairs; 2.03M tris), "
+                                   "16x16-pixel tile roots, hit ids and shadow mask rasterized on the device"),
     "serve": (None, 0, "served mixed stream on the seeded conference hall (20x12 m, 8 tables, 120 chairs; 124800 "
                        "tris): Poisson arrivals (seed 5), 70% exact soft shadows / 30% point shadows, square side "
                        "uniform 256..1024"),
@@ -149,6 +161,8 @@ def main() -> int:
             return 1
         print(json.dumps(res), flush=True)
         return 0
+    if args.workload in ("primary", "building"):
+        return bench_visibility(args)
     from beamtrace2007_amd import gpu
     from beamtrace2007_amd.host import Scene, camera, camera_d
 
@@ -237,6 +251,113 @@ def main() -> int:
     if any(infos[-1]["phase_cycles"].values()):  # instrumented builds only
         out["trace_phase_cycles"] = infos[-1]["phase_cycles"]
     out.update(cpu_and_parity(args, s, sp, cam, W, H, light, full, guarded, fp64))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def trace_bytes(st: dict) -> int:
+    """Algorithmic bytes of the trace kernel for a pass's counters: a 12-B kd
+    node per inner step and leaf visit, a 64-B leaf triangle per test, a 4-B
+    parent link per climbed level."""
+    return 12 * (st["kdSteps"] + st["leafVisits"]) + 64 * st["triTests"] + 4 * st["climbs"]
+
+
+def bench_visibility(args) -> int:
+    """Primary visibility (room, 8 x 8 tile roots) or primary visibility plus
+    point-light shadows (the 2M-triangle building at 1024 x 1024, 16 x 16 tile
+    roots): one step = one frame's polygons traced on the device and
+    rasterized there into per-pixel hit ids (and the shadow mask), the images
+    then downloaded (5 B per pixel). The reference CPU build renders the same
+    frame with the same algorithm (restart trail, no mailbox: its primary and
+    point-shadow paths are single-threaded, render.cpp:150-158, 209-232) as
+    the baseline, and its ids and mask are compared with the GPU's."""
+    from beamtrace2007_amd import gpu
+    from beamtrace2007_amd.host import Scene, camera
+
+    scene_name, res0, desc = WORKLOADS[args.workload]
+    W = H = args.res or res0
+    tile = 8 if args.workload == "primary" else 16
+    dev = gpu.device_info()
+    if dev is None:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    t0 = time.perf_counter()
+    s = Scene.generate(scene_name)
+    s.build_kd(0.4)
+    build_s = time.perf_counter() - t0
+    sp = s.spec()
+    cam = camera(sp["view"], W, H)
+    light3 = sp["point_light"] if args.workload == "building" else None
+    r = gpu.GpuRenderer(s)
+    kw = {"profile": not args.no_profile}
+    if args.leaf_cap >= 0:
+        kw["leaf_cap"] = args.leaf_cap
+    if args.mailbox >= 0:
+        kw["mailbox"] = bool(args.mailbox)
+    r.set_options(**kw)
+    for _ in range(args.warmup):
+        r.images(cam, W, H, light3, tile=tile)
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ids, mask, info = r.images(cam, W, H, light3, tile=tile)
+        infos.append(info)
+    wall = time.perf_counter() - t0
+    r.close()
+    px = W * H
+    print(f"bench.py: {args.workload} {W}x{H}: {args.steps} frames in {wall:.3f} s", file=sys.stderr, flush=True)
+    passes = ("primary", "point") if light3 is not None else ("primary",)
+    b_trace = sum(trace_bytes(infos[-1][k]["stats"]) for k in passes)
+    trace_s = sum(i[k]["kernel_s"]["trace"] for i in infos for k in passes)
+    launches = sum(i[k]["launches"]["trace"] for i in infos for k in passes)
+    roofline = None
+    if trace_s > 0:
+        achieved = b_trace * args.steps / trace_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None, "kernel": "traceKernel",
+                    "launches_per_step": launches / args.steps,
+                    "avg_launch_ms": round(trace_s / max(1, launches) * 1e3, 4), "alg_bytes_per_step": int(b_trace)}
+    per_pass = {k: {"seconds": round(sum(i[k]["total_s"] for i in infos) / args.steps, 5),
+                    "rounds": infos[-1][k]["rounds"], "killed_roots": infos[-1][k]["killed_roots"],
+                    "beams": infos[-1][k]["stats"]["beams"],
+                    "kernel_ms": {n: round(sum(i[k]["kernel_s"][n] for i in infos) / args.steps * 1e3, 3)
+                                  for n in gpu.KERNELS}} for k in passes}
+    beams = sum(infos[-1][k]["stats"]["beams"] for k in passes)
+    out = {
+        "metric": "primary visibility" + (" + point-shadow" if light3 is not None else "") + " pixels/s (1 GPU)",
+        "value": round(px * args.steps / wall, 1), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": f"{args.workload} {W}x{H}: {desc}", "tile": tile, "tris": s.num_tris,
+                   "kd_nodes": len(s.kd_arrays()["nodes"])},
+        "device": device_label(dev), "arch": dev["arch"], "beams_per_s": round(beams * args.steps / wall, 1),
+        "passes": per_pass, "raster_ms": round(sum(i["raster_s"] for i in infos) / args.steps * 1e3, 3),
+        "roofline": roofline, "host_kd_build_s": round(build_s, 3), "cpu_baseline": None, "parity": None,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from refpy import Ref, ref_available
+        if ref_available("f"):
+            a = s.arrays()
+            ref = Ref("f").load(a["pos"], a["tris"])
+            t0 = time.perf_counter()
+            prim = ref.primary(sp["view"], W, H, mailbox=False, trail=True, tile=tile)
+            rid = ref.raster(sp["view"], W, H, prim)
+            rmask = None
+            if light3 is not None:
+                sh = ref.point(sp["view"], W, H, prim, light3, mailbox=False, trail=True)
+                rmask = ref.raster(sp["view"], W, H, sh, shadow=True)
+            secs = time.perf_counter() - t0
+            ref.free()
+            _, cores = host_cores()
+            out["cpu_baseline"] = {"value": round(px / secs, 1), "unit": "pixels/s", "cores": 1, "kind": "reference",
+                                   "host": cores,
+                                   "sample": f"the whole {W}x{H} frame on one thread ({secs:.2f} s: primary beams"
+                                             + (" + point-shadow beams" if light3 is not None else "")
+                                             + " + rasterization, restart trail, no mailbox)"}
+            out["parity"] = {"against": "the reference CPU build, same frame",
+                             "ids_bitexact": bool(np.array_equal(ids, rid)),
+                             "mask_bitexact": None if rmask is None else bool(np.array_equal(mask, rmask))}
     print(json.dumps(out), flush=True)
     return 0
 
