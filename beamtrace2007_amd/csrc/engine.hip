@@ -49,6 +49,7 @@ namespace btg {
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, each with its own L2
 // Threads per workgroup of the wavefront kernels (beams per workgroup), 4
 // waves; -DBTG_BLOCK=128 or 512 builds the other sizes (the LDS columns of the
 // trace and split kernels scale with it).
@@ -119,6 +120,11 @@ struct WState {
   int shadeCount;  // soft, exact: unoccluded pieces queued this round for the Lambert integral
   unsigned long long reserved;
   unsigned long long traceItems, splitItems;
+  // Finished workgroups of a split launch, counted per group of workgroups
+  // (workgroup b in group b % kXcds, one counter per 128-byte line) before
+  // blocksDone counts the finished groups: a few hundred workgroups on one
+  // counter serialise at the memory side, as the chunk counters would.
+  int groupDone[kXcds * 32];
 };
 
 // One soft-shadow job of a batch (a single call is a batch of one). Every
@@ -218,6 +224,46 @@ __device__ __forceinline__ int waveAppend(int* counter, int n) {
   if (lane() == kWave - 1 && total > 0) base = atomicAdd(counter, total);
   base = __shfl(base, kWave - 1, kWave);
   return base + x - n;
+}
+
+// How a persistent launch deals its items (queue records, clips, roots) to
+// waves, 64 at a time: each wave's first chunk statically (wave w takes items
+// [64 w, 64 w + 64)), later chunks from the round's atomic counter, numbered
+// after the static ones. Thousands of waves incrementing one counter at the
+// start of a launch serialise at the memory side (~20 ns each: a short
+// round's trace launch took 60 us for that alone), so a round with fewer
+// items than the grid has waves x 64 now takes no atomic at all. Workgroups
+// are dealt to the XCDs round-robin, so waves are numbered XCD-major:
+// consecutive chunks (sibling sub-beams, neighbouring pixels) land on one XCD
+// and share its L2.
+struct Deal {
+  int wave;   // this wave's static chunk
+  int waves;  // waves in the grid: items below 64 * waves are dealt statically
+};
+__device__ __forceinline__ Deal dealOf() {
+  const int G = int(gridDim.x), b = int(blockIdx.x);
+  const int bx = G % kXcds == 0 ? (b % kXcds) * (G / kXcds) + b / kXcds : b;
+  return Deal{bx * (kBlock / kWave) + int(threadIdx.x) / kWave, G * (kBlock / kWave)};
+}
+
+// The next chunk of `n` items for the calling wave (all 64 lanes call it):
+// the static chunk on the first call, then a chunk of `want` from `counter`.
+// Returns the chunk's first item; `exhausted` is set once no item is left.
+__device__ __forceinline__ int nextChunk(const Deal& dl, int* counter, int want, int n, bool& first,
+                                         bool& exhausted) {
+  const int statics = dl.waves * kWave;
+  int base;
+  if (first) {
+    first = false;
+    base = dl.wave * kWave;
+    if (statics >= n) exhausted = true;
+    return base;
+  }
+  base = 0;
+  if (lane() == 0) base = atomicAdd(counter, want);
+  base = __shfl(base, 0, kWave) + statics;
+  if (base + want >= n) exhausted = true;
+  return base;
 }
 
 // One slot of `counter` per calling lane, one atomic per wave; callable from
@@ -1484,7 +1530,8 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
   const int n = int(min((unsigned long long)p.st->count[cur], p.capacity));
   if (n == 0) return;  // an empty round (a pass draining, or an idle open pass)
   const Rec* q = p.tq[cur];
-  bool active = false, exhausted = false;
+  const Deal dl = dealOf();
+  bool active = false, exhausted = false, first = true;
   int leaves = 0;
   Rec r;
   Work w;
@@ -1496,11 +1543,8 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const int nIdle = __popcll(idle);
-    if (!exhausted && nIdle > 0) {
-      int base = 0;
-      if (lane() == 0) base = atomicAdd(&p.st->head[kHeadTrace], nIdle);
-      base = __shfl(base, 0, kWave);
-      if (base + nIdle >= n) exhausted = true;
+    if (!exhausted && nIdle > 0) {  // on the first pass every lane is idle: the static chunk of 64
+      const int base = nextChunk(dl, &p.st->head[kHeadTrace], nIdle, n, first, exhausted);
       if (!active) {
         const int i = base + rankBelow(idle);
         if (i < n) {
@@ -1657,10 +1701,10 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
   float* const stageCol = stageLds + threadIdx.x;
   const int n = int(min((unsigned long long)p.st->splitCount, p.capacity));
   const int nxt = cur ^ 1;
-  for (; n > 0;) {  // an empty round skips to the scheduling below
-    int base = 0;
-    if (lane() == 0) base = atomicAdd(&p.st->head[kHeadSplit], kWave);
-    base = __shfl(base, 0, kWave);
+  const Deal dl = dealOf();
+  bool first = true, exhausted = false;
+  while (n > 0 && !exhausted) {  // an empty round skips to the scheduling below
+    const int base = nextChunk(dl, &p.st->head[kHeadSplit], kWave, n, first, exhausted);
     if (base >= n) break;
     const int i = base + lane();
     int slots = 0;
@@ -1754,14 +1798,20 @@ __global__ void __launch_bounds__(kBlock) SPLIT_OCCUPANCY splitKernel(Params p, 
     }
   }
   flushCnt(p, cnt);
-  // The last workgroup to finish schedules the next round.
+  // The last workgroup to finish schedules the next round: the last of each
+  // group of workgroups counts its group done, the last group schedules.
   __syncthreads();
   if (threadIdx.x == 0) {
+    const int G = int(gridDim.x), g = int(blockIdx.x) % kXcds;
+    const int groups = G < kXcds ? G : kXcds;
+    const int members = G / kXcds + (g < G % kXcds ? 1 : 0);
     __threadfence();
-    const int done = atomicAdd(&p.st->blocksDone, 1);
-    if (done == int(gridDim.x) - 1) {
+    if (atomicAdd(&p.st->groupDone[g * 32], 1) == members - 1) {
       __threadfence();
-      scheduleNext(p, cur);
+      if (atomicAdd(&p.st->blocksDone, 1) == groups - 1) {
+        __threadfence();
+        scheduleNext(p, cur);
+      }
     }
   }
 }
@@ -1789,6 +1839,7 @@ __device__ void scheduleNext(const Params& p, int cur) {
   WState* s = p.st;
   const int nxt = cur ^ 1;
   volatile WState* vs = s;
+  for (int g = 0; g < kXcds; ++g) s->groupDone[g * 32] = 0;
   if (vs->done) {
     s->blocksDone = 0;
     s->head[0] = s->head[1] = s->head[2] = 0;
@@ -1835,6 +1886,7 @@ __global__ void initKernel(Params p) {
     s->overflow = 0;
     if (p.pass == 0) s->outCount = 0;  // retry passes append to the first pass's output
     s->blocksDone = 0;
+    for (int g = 0; g < kXcds; ++g) s->groupDone[g * 32] = 0;
     s->round = 0;
     s->reserved = 0;
     s->traceItems = s->splitItems = 0;
@@ -2172,10 +2224,10 @@ __global__ void __launch_bounds__(kBlock) admitKernel(Params p, int cur) {
   cnt.zero();
   const int begin = p.st->admitBegin, n = p.st->admitCount;
   if (n == 0) return;  // most rounds admit nothing: no queue atomics, no counter flush
-  for (;;) {
-    int base = 0;
-    if (lane() == 0) base = atomicAdd(&p.st->head[kHeadAdmit], kWave);
-    base = __shfl(base, 0, kWave);
+  const Deal dl = dealOf();
+  bool first = true, exhausted = false;
+  while (!exhausted) {
+    const int base = nextChunk(dl, &p.st->head[kHeadAdmit], kWave, n, first, exhausted);
     if (base >= n) break;
     const int i = base + lane();
     if (i < n) {
