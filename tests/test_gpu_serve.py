@@ -166,6 +166,48 @@ def test_guarded_requests_equal_guarded_alone(require_gpu):
     r.close()
 
 
+def test_requests_over_budget_are_traced_again(require_gpu):
+    """Roots that go over their live sub-beam budget inside a served pass are
+    abandoned there, and their request is traced again alone on the server's
+    fallback engine (GPU passes with a larger budget): at a budget of 4 the
+    plant's roots go over, and every served result, soft and point, still
+    equals the request rendered alone at the default budget."""
+    from beamtrace2007_amd.gpu import GpuRenderer, Server
+    from beamtrace2007_amd.host import camera
+    sp = load_scene("plant")[3]
+    plan = [(48, 40, 1.0), (32, 32, 2.0), (40, 56, 0.5)]
+    r = GpuRenderer(load_scene("plant")[0])
+    r.set_options(budget=4, primary_budget=4)
+    cam0 = camera(sp["view"], 48, 40)
+    killed = r.soft_shadows(cam0, 48, 40, sp["area_light"], exact=True)["info"]["killed_roots"]
+    assert killed > 0, "a budget of 4 should abandon some of the plant's roots"
+    srv = Server(r, hbm_bytes=SERVER_HBM)
+    tickets = []
+    for W, H, s in plan:
+        cam = camera(sp["view"], W, H)
+        light = scaled_light(sp["area_light"], s)
+        tickets.append(("soft", cam, W, H, light, srv.submit_soft(0, cam, W, H, light)))
+    pcam = camera(sp["view"], 40, 40)
+    tickets.append(("point", pcam, 40, 40, sp["point_light"], srv.submit_point(pcam, 40, 40, sp["point_light"])))
+    print(f"submitted {len(tickets)} requests at budget 4 ({killed} roots of the first over budget alone)",
+          flush=True)
+    results = []
+    for kind, cam, W, H, light, t in tickets:
+        res = srv.wait(t, timeout=120)
+        assert res is not None and res["status"] == 0, res
+        results.append((kind, cam, W, H, light, res))
+    srv.close()
+    r.close()
+    for kind, cam, W, H, light, res in results:
+        if kind == "soft":
+            a = soft_alone("plant", cam, W, H, light)
+            for f in ("tri", "vis", "E", "Eu"):
+                assert np.array_equal(res[f], a[f]), (W, H, f)
+        else:
+            hits, shadow = point_alone("plant", cam, W, H, light)
+            assert same_polys(res["hits"], hits) and same_polys(res["polys"], shadow), (W, H)
+
+
 def test_bad_requests_fail_alone(require_gpu, server_setup):
     """A request for a scene that is not resident, or one larger than a whole
     pass, finishes with an error status and a message; requests around it
