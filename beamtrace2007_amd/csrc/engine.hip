@@ -104,27 +104,41 @@ struct RootQ {
   F3 xlo;       // guarded soft: low part of the shading point (x + xlo, the double hit point)
 };
 
+// Ints that many waves update with atomics, each in its own 128-byte line:
+// atomics on one line are performed one after another at the memory side, so
+// counters that share a line (the queue heads and tails of a round) would
+// queue behind each other's traffic.
+template <int N>
+struct LineInts {
+  struct alignas(128) Cell {
+    int v;
+  };
+  Cell c[N];
+  __host__ __device__ int& operator[](int i) { return c[i].v; }
+  __host__ __device__ volatile int& operator[](int i) volatile { return c[i].v; }
+};
+
 struct WState {
   int next;
   int admitBegin, admitCount;
-  int count[2];
-  int splitCount;
-  int head[3];
   int done;
-  int killed;
   int maxQueue;
   int overflow;
-  int outCount;
-  int blocksDone;
   int round;
-  int shadeCount;  // soft, exact: unoccluded pieces queued this round for the Lambert integral
-  unsigned long long reserved;
   unsigned long long traceItems, splitItems;
+  LineInts<2> count;  // records in the trace queues (ping-pong by round)
+  LineInts<3> head;   // items taken dynamically by admit, trace, split
+  alignas(128) int splitCount;
+  alignas(128) int shadeCount;  // soft, exact: unoccluded pieces queued this round for the Lambert integral
+  alignas(128) int outCount;
+  alignas(128) int killed;
+  alignas(128) int blocksDone;
+  alignas(128) unsigned long long reserved;
   // Finished workgroups of a split launch, counted per group of workgroups
   // (workgroup b in group b % kXcds, one counter per 128-byte line) before
   // blocksDone counts the finished groups: a few hundred workgroups on one
   // counter serialise at the memory side, as the chunk counters would.
-  int groupDone[kXcds * 32];
+  alignas(128) int groupDone[kXcds * 32];
 };
 
 // One soft-shadow job of a batch (a single call is a batch of one). Every
@@ -966,9 +980,10 @@ __device__ __forceinline__ float area4(const C4& x, const C4& y) {
   return a * 0.5f;
 }
 
-// Appends an output polygon (primary / point shadows).
+// Appends an output polygon (primary / point shadows); the active lanes of a
+// wave share one atomic.
 __device__ void appendPoly(const Params& p, int root, const OutPoly& ob) {
-  const int k = atomicAdd(&p.st->outCount, 1);
+  const int k = activeAppend(&p.st->outCount);
   if (k >= p.outCap) return;  // the host traces again with a larger buffer
   p.out[k] = ob;
   p.outRoot[k] = root;
