@@ -50,6 +50,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kXcds = 8;  // MI355X: 8 XCDs of 32 CUs, each with its own L2
+constexpr int kStatSlots = 128, kStatWords = 16;  // counter slots (flushCnt), one 128-byte line each
 // Threads per workgroup of the wavefront kernels (beams per workgroup), 4
 // waves; -DBTG_BLOCK=128 or 512 builds the other sizes (the LDS columns of the
 // trace and split kernels scale with it).
@@ -188,7 +189,6 @@ struct Params {
   unsigned long long* gFx;
   unsigned long long* stats;
   unsigned long long* cycles;  // trace-loop phase clocks (BTG_PHASE_CYCLES builds)
-  double* dropped;
   WState* st;
   int W, H;
   Plane camPlane;
@@ -301,18 +301,24 @@ struct Cnt {
   }
 };
 
+// A wave's counters are added into the stat slot of its workgroup
+// (blockIdx mod kStatSlots, one 128-byte line each: the counters in words
+// 0..sNum-1, the dropped area as a double in word kStatWords - 1); the host
+// sums the slots. With one shared set of counters, the waves of a launch that
+// finish together queued their atomics on one line.
 __device__ void flushCnt(const Params& p, const Cnt& c) {
+  unsigned long long* const slot = p.stats + (blockIdx.x % kStatSlots) * kStatWords;
 #pragma unroll
   for (int f = 0; f < sNum; ++f) {
     unsigned long long s = c.v[f];
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_down(s, o, kWave);
-    if (lane() == 0 && s) atomicAdd(&p.stats[f], s);
+    if (lane() == 0 && s) atomicAdd(slot + f, s);
   }
   double d = c.dropped;
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) d += __shfl_down(d, o, kWave);
-  if (lane() == 0 && d != 0) atomicAdd(p.dropped, d);
+  if (lane() == 0 && d != 0) atomicAdd(reinterpret_cast<double*>(slot + kStatWords - 1), d);
 }
 
 // ------------------------------------------------------------ beam context
@@ -2727,7 +2733,7 @@ class Engine {
   // scene storage
   void* sceneMem[6] = {};
   // engine buffers (grown on demand)
-  DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, dropped, st, out, outRoot, outTag, list, listCount;
+  DevBuf tq0, tq1, sq, roots, live, state, epoch, missFx, gFx, stats, st, out, outRoot, outTag, list, listCount;
   DevBuf pixTri, pixX, pixXlo, pixN, vis, E, Eu, prim, retry, cycles, shq, jobs;
   // FP64 soft shadows: hit leaf reference per pixel; counters and cell lists;
   // the tracers' arrays; the jobs' double cameras; per-pixel fixed-point sums
@@ -2782,7 +2788,7 @@ class Engine {
     return 0;
   }
   ~Engine() {
-    DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &dropped, &st, &out,
+    DevBuf* all[] = {&tq0, &tq1, &sq, &roots, &live, &state, &epoch, &missFx, &gFx, &stats, &st, &out,
                      &outRoot, &outTag, &list, &listCount, &rasterImg, &rasterRows, &pixTri, &pixX, &pixXlo, &pixN, &vis, &E, &Eu, &prim, &retry, &cycles, &shq, &jobs, &pixRef, &fpList, &fpMem, &camD, &fpSum};
     for (DevBuf* b : all)
       if (b->ptr) (void)hipFree(b->ptr);
@@ -2816,8 +2822,8 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
       (rc = e.ensure(e.sq, cap * sizeof(Rec), err)) || (rc = e.ensure(e.roots, size_t(R) * sizeof(RootQ), err)) ||
       (rc = e.ensure(e.live, size_t(R) * 4, err)) || (rc = e.ensure(e.state, size_t(R) * 4, err)) ||
       (rc = e.ensure(e.epoch, size_t(R) * 4, err)) || (rc = e.ensure(e.missFx, size_t(R) * 8, err)) ||
-      (rc = e.ensure(e.gFx, size_t(R) * 8, err)) || (rc = e.ensure(e.stats, sizeof(uint64_t) * sNum, err)) ||
-      (rc = e.ensure(e.dropped, sizeof(double), err)) || (rc = e.ensure(e.st, sizeof(WState), err)) ||
+      (rc = e.ensure(e.gFx, size_t(R) * 8, err)) || (rc = e.ensure(e.stats, sizeof(uint64_t) * kStatSlots * kStatWords, err)) ||
+      (rc = e.ensure(e.st, sizeof(WState), err)) ||
       (rc = e.ensure(e.cycles, 8 * sizeof(uint64_t), err)) ||
       (rc = e.ensure(e.list, size_t(R) * 4, err)) || (rc = e.ensure(e.listCount, 4, err)))
     return rc;
@@ -2842,7 +2848,6 @@ int prepare(Engine& e, int app, int R, int budget, Params& p, std::string& err) 
   p.gFx = static_cast<unsigned long long*>(e.gFx.ptr);
   p.stats = static_cast<unsigned long long*>(e.stats.ptr);
   p.cycles = static_cast<unsigned long long*>(e.cycles.ptr);
-  p.dropped = static_cast<double*>(e.dropped.ptr);
   p.st = static_cast<WState*>(e.st.ptr);
   // Primary passes are chains of short rounds whose trace launches last as
   // long as their slowest lane: they yield after at most kPrimaryLeafCap
@@ -2957,22 +2962,28 @@ int runPass(Engine& e, Params& p, RunInfo& info, std::string& err) {
 }
 
 int readStats(Engine& e, uint64_t* dst, double* dropped, std::string& err) {
-  uint64_t s[sNum];
-  double d = 0;
+  static_assert(sNum < kStatWords, "a stat slot holds the counters and the dropped area");
+  std::vector<uint64_t> s(size_t(kStatSlots) * kStatWords);
   BTG_TRY(hipMemcpyAsync(e.phaseCycles, e.cycles.ptr, sizeof(e.phaseCycles), hipMemcpyDeviceToHost, e.stream));
-  BTG_TRY(hipMemcpyAsync(s, e.stats.ptr, sizeof(s), hipMemcpyDeviceToHost, e.stream));
-  BTG_TRY(hipMemcpyAsync(&d, e.dropped.ptr, sizeof(d), hipMemcpyDeviceToHost, e.stream));
+  BTG_TRY(hipMemcpyAsync(s.data(), e.stats.ptr, s.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, e.stream));
   BTG_TRY(hipStreamSynchronize(e.stream));
-  for (int i = 0; i < sNum; ++i) dst[i] += s[i];
+  double d = 0;
+  for (int k = 0; k < kStatSlots; ++k) {
+    const uint64_t* slot = s.data() + size_t(k) * kStatWords;
+    for (int i = 0; i < sNum; ++i) dst[i] += slot[i];
+    double x;
+    std::memcpy(&x, slot + kStatWords - 1, sizeof(x));
+    d += x;
+  }
   if (dropped) *dropped += d;
   return 0;
 }
 
 int zeroStats(Engine& e, std::string& err) {
-  const uint64_t z[sNum + 1] = {};
-  BTG_TRY(hipMemcpyAsync(e.stats.ptr, z, sizeof(uint64_t) * sNum, hipMemcpyHostToDevice, e.stream));
-  BTG_TRY(hipMemcpyAsync(e.dropped.ptr, z, sizeof(double), hipMemcpyHostToDevice, e.stream));
-  BTG_TRY(hipMemcpyAsync(e.cycles.ptr, z, 8 * sizeof(uint64_t), hipMemcpyHostToDevice, e.stream));
+  zeroWords<<<kStatSlots * kStatWords / 256, 256, 0, e.stream>>>(static_cast<unsigned long long*>(e.stats.ptr),
+                                                               size_t(kStatSlots) * kStatWords);
+  zeroWords<<<1, 64, 0, e.stream>>>(static_cast<unsigned long long*>(e.cycles.ptr), 8);
+  BTG_TRY(hipGetLastError());
   return 0;
 }
 
