@@ -404,9 +404,27 @@ struct Work {
   F3 hitN;
 };
 
+// The beam's |w| per axis kept in the lane's LDS column as well (word
+// (4 a + i) * kBlock for axis a, corner i): the descent picks an axis per node
+// at run time, and reading that axis's four values from LDS costs no VALU
+// instruction, where selecting them among the registers of the three axes took
+// eight selects per axis.
+__device__ __forceinline__ C4 awAt(const float* awCol, int a) {
+  const float* q = awCol + a * 4 * kBlock;
+  return c4(q[0], q[kBlock], q[2 * kBlock], q[3 * kBlock]);
+}
+__device__ __forceinline__ void awStore(float* awCol, const Beam& b) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    awCol[i * kBlock] = b.aw0.v[i];
+    awCol[(4 + i) * kBlock] = b.aw1.v[i];
+    awCol[(8 + i) * kBlock] = b.aw2.v[i];
+  }
+}
+
 // decide (beam_core.h:139-164): near / far / both children of an inner node.
-__device__ int decide(const Ctx& c, const Beam& b, int a, float Ns, const Box& box) {
-  const C4 awa = awOf(b, a);
+__device__ int decide(const Ctx& c, const Beam& b, int a, float Ns, const Box& box, const float* awCol) {
+  const C4 awa = awAt(awCol, a);
 #pragma unroll
   for (int k = 1; k <= 2; ++k) {
     const int cc = a + k >= 3 ? a + k - 3 : a + k;
@@ -415,7 +433,7 @@ __device__ int decide(const Ctx& c, const Beam& b, int a, float Ns, const Box& b
     const float entry = sg > 0 ? comp(box.lo, cc) : comp(box.hi, cc);
     const float exit = sg > 0 ? comp(box.hi, cc) : comp(box.lo, cc);
     const float nmin = s * relApex(c, entry, cc), nmax = s * relApex(c, exit, cc);
-    const C4 lhs = scaled(awOf(b, cc), Ns);
+    const C4 lhs = scaled(awAt(awCol, cc), Ns);
     if (allBelow(lhs, scaled(awa, nmin), kKdRel)) return kFar;
     if (allAbove(lhs, scaled(awa, nmax), kKdRel)) return kNear;
   }
@@ -1101,7 +1119,7 @@ __device__ __forceinline__ void killRoot(const Params& p, int root) {
 // -DBTG_PATH_LEVELS=4 sends most climbs through the parent-link walk (a test
 // build for that path).
 #ifndef BTG_PATH_LEVELS
-#define BTG_PATH_LEVELS 40
+#define BTG_PATH_LEVELS 32
 #endif
 constexpr int kPathLevels = BTG_PATH_LEVELS;
 
@@ -1195,7 +1213,7 @@ struct PhaseClock {
 // Both children of an inner node are fetched while it is decided.
 template <int App>
 __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& ss, const Path& path,
-                       Cnt& cnt PHASE_ARG) {
+                       const float* awCol, Cnt& cnt PHASE_ARG) {
   const DevScene& sv = p.sv;
   const Beam& B = w.b;
   for (;;) {
@@ -1214,7 +1232,7 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
         const bool toRight = sga > 0;
         if (d < kPathLevels) path.set(d, toRight ? right : left);
         int dec = kFar;
-        if (Ns > 0) dec = decide(c, B, a, Ns, ss.box);
+        if (Ns > 0) dec = decide(c, B, a, Ns, ss.box, awCol);
         // kFar: into the child the rays move into; otherwise the apex side
         const bool goRight = (dec == kFar) == toRight;
         if (dec == kBoth) r.trail |= uint64_t(1) << d;
@@ -1560,7 +1578,9 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
   Stepper ss;
   __shared__ uint32_t pathLds[kPathLevels * kBlock];
   __shared__ int32_t mboxLds[kMbox * kBlock];
+  __shared__ float awLds[12 * kBlock];
   const Path path{pathLds + threadIdx.x};
+  float* const awCol = awLds + threadIdx.x;
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const int nIdle = __popcll(idle);
@@ -1575,6 +1595,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
           } else {
             c = modeCtx<App>(p.roots[r.root]);
             expand<App>(p, c, r, w);
+            awStore(awCol, w.b);
             if (r.cursor < 0) {
               ss.phase = kDescend;
               ss.box = p.sv.nodeBox[r.node];
@@ -1602,7 +1623,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
     }
     PHASE_MARK(kPhRefill);
     int res = kContinue;
-    if (active) res = advance<App>(p, c, r, w, ss, path, cnt PHASE_PASS);
+    if (active) res = advance<App>(p, c, r, w, ss, path, awCol, cnt PHASE_PASS);
     PHASE_MARK(kPhRest);
     if (kCoopTest) {
       const bool want = active && res == kContinue;  // a leaf with triangles pending
