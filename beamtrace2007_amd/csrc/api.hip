@@ -316,8 +316,8 @@ int btg_footprint(const btg_ctx* ctx, const btg_request* req, int64_t* bytes) {
   if (!ctx || !req || !bytes || !validCam(&req->cam)) return BTG_ERR_ARG;
   const int64_t W = req->cam.W, H = req->cam.H;
   if (req->kind == BTG_REQ_SOFT) {
-    // roots: 64 per 8 x 8 pixel tile, partial tiles included (the soft passes' layout)
-    const int64_t roots = ((W + 7) / 8) * ((H + 7) / 8) * 64;
+    // roots: 64 per tile of pixels, partial tiles included (the soft passes' layout)
+    const int64_t roots = btg::softRootCount(W, H);
     *bytes = W * H * btg::softPixelBytes(btg::engineOptions(ctx->engine).precision != 0) +
              roots * btg::rootStateBytes();
     return BTG_OK;
@@ -661,6 +661,7 @@ namespace {
 struct SrvReq {
   uint64_t id = 0;
   btg_request req{};
+  btg_camera_d camD{};  // the request's double camera, copied at submit (FP64 servers; req.camD points here)
   double tSubmit = 0, tAdmit = 0, tDone = 0;
   int status = BTG_OK;
   bool done = false;
@@ -703,7 +704,8 @@ void finishLocked(btg_server* s, SrvReq* r, int status, const std::string& why) 
 }
 
 btg::SoftJobH jobOf(const btg_request& q) {
-  return btg::SoftJobH{q.scene, camOf(q.cam), lightOf(q.light), q.tri, q.vis, q.E, q.Eu};
+  return btg::SoftJobH{q.scene, camOf(q.cam), lightOf(q.light), q.tri, q.vis, q.E, q.Eu,
+                       q.camD ? q.camD->eye : nullptr};
 }
 
 // Fails every request of pass g and starts the pass afresh.
@@ -830,6 +832,55 @@ void softLoop(btg_server* s) {
   }
 }
 
+// Soft requests of an FP64 server (precision 2): each batch takes every
+// request pending when the previous one ended, up to kFp64BatchJobs requests
+// and the server's pixel budget, and runs them through one engineSoftBatch
+// call (FP64 cells of all their pixels, dealt dynamically to the device).
+constexpr int kFp64BatchJobs = 64;
+
+void fp64Loop(btg_server* s) {
+  std::string err;
+  std::vector<SrvReq*> batch;
+  std::vector<btg::SoftJobH> jobs;
+  for (;;) {
+    batch.clear();
+    jobs.clear();
+    {
+      std::unique_lock<std::mutex> lk(s->m);
+      s->cvSoft.wait(lk, [&] { return s->stop || !s->softQ.empty(); });
+      if (s->softQ.empty()) return;
+      int64_t pix = 0;
+      while (!s->softQ.empty() && int(batch.size()) < kFp64BatchJobs) {
+        SrvReq* r = s->softQ.front();
+        const int64_t rp = int64_t(r->req.cam.W) * r->req.cam.H;
+        if (btg::engineSceneBytes(s->fallback, r->req.scene) < 0 || rp > s->pixCap) {  // fails alone
+          s->softQ.pop_front();
+          finishLocked(s, r, rp > s->pixCap ? BTG_ERR_LIMIT : BTG_ERR_ARG,
+                       rp > s->pixCap ? "request larger than the server's pixel budget"
+                                      : "no resident scene " + std::to_string(r->req.scene));
+          continue;
+        }
+        if (!batch.empty() && pix + rp > s->pixCap) break;
+        s->softQ.pop_front();
+        r->tAdmit = now();
+        pix += rp;
+        batch.push_back(r);
+      }
+    }
+    int rc = BTG_OK;
+    try {
+      for (SrvReq* r : batch) jobs.push_back(jobOf(r->req));
+      btg::RunInfo ri;
+      rc = btg::engineSoftBatch(s->fallback, jobs.data(), int(jobs.size()), true, ri, err);
+    } catch (const std::bad_alloc&) {
+      rc = BTG_ERR_MEMORY;
+      err = "host allocation failed";
+    }
+    std::lock_guard<std::mutex> lk(s->m);
+    for (SrvReq* r : batch) finishLocked(s, r, rc, err);
+  }
+}
+
 void pointLoop(btg_server* s) {
   for (;;) {
     SrvReq* r;
@@ -907,10 +958,7 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
   const int jobs = c.jobsPerPass > 0 ? c.jobsPerPass : 512;
   const int rounds = c.roundsPerStep > 0 ? c.roundsPerStep : 8;
   const btg::Options& opt = btg::engineOptions(ctx->engine);
-  if (opt.precision == 2)
-    return failWith(ctx, BTG_ERR_ARG,
-                    "a server's passes run precision 0 or 1; FP64 soft shadows (precision 2) trace each pixel "
-                    "on its own and run through btg_soft / btg_soft_batch");
+  const bool fp64 = opt.precision == 2;  // FP64 batches instead of wavefront passes (fp64Loop)
   // per pass: three sub-beam queues (64 B records) and the shade queue (40 B)
   // in at most a quarter of hbmBytes (the largest power-of-two capacity that
   // fits, and no more than the context's), then per pixel its images and per
@@ -934,7 +982,7 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
     s->queueCap = queueCap;
     s->jobsPerPass = jobs;
     std::string err;
-    for (int g = 0; g < 2 && err.empty(); ++g) {
+    for (int g = 0; g < 2 && err.empty() && !fp64; ++g) {
       s->pass[g] = btg::openCreate(ctx->engine, rootCap, pixCap, jobs, true, queueCap, err);
       if (!s->pass[g] && err.empty()) err = "open pass creation failed";
     }
@@ -956,7 +1004,7 @@ int btg_server_create(btg_ctx* ctx, const btg_server_config* cfg, btg_server** o
     btg::engineOptions(s->fallback).graphs = false;
     btg::engineOptions(s->pointEngine).capacity = std::min<int64_t>(opt.capacity, queueCap);
     btg::engineOptions(s->fallback).capacity = std::min<int64_t>(opt.capacity, queueCap);
-    s->softThread = std::thread(softLoop, s);
+    s->softThread = fp64 ? std::thread(fp64Loop, s) : std::thread(softLoop, s);
     s->pointThread = std::thread(pointLoop, s);
     *out = s;
     return BTG_OK;
@@ -985,6 +1033,12 @@ int btg_server_submit(btg_server* s, const btg_request* req, uint64_t* ticket) {
     SrvReq* r = new SrvReq();
     r->id = s->nextId++;
     r->req = *req;
+    if (req->kind == BTG_REQ_SOFT && req->camD) {  // the caller's camera is read now, not when the batch runs
+      r->camD = *req->camD;
+      r->req.camD = &r->camD;
+    } else {
+      r->req.camD = nullptr;
+    }
     r->tSubmit = now();
     s->reqs[r->id] = r;
     if (req->kind == BTG_REQ_SOFT) {

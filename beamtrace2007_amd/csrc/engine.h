@@ -44,6 +44,20 @@ struct DevScene {
   int numTris = 0;
 };
 
+// Soft-shadow roots are the pixels of tiles of 64 (kRootTileW x kRootTileH;
+// build knob BTG_ROOT_TILE_W = 4, 8, 16 or 32), one wave of the admit kernel
+// per tile; partial tiles at the image's right and bottom edges keep their
+// 64 root slots (the ones outside the image have no beam).
+#ifndef BTG_ROOT_TILE_W
+#define BTG_ROOT_TILE_W 8
+#endif
+constexpr int kRootTileW = BTG_ROOT_TILE_W, kRootTileH = 64 / kRootTileW;
+static_assert(kRootTileW * kRootTileH == 64, "a root tile is one wave of 64 pixels");
+inline int64_t rootTilesX(int64_t W) { return (W + kRootTileW - 1) / kRootTileW; }
+inline int64_t softRootCount(int64_t W, int64_t H) {
+  return rootTilesX(W) * ((H + kRootTileH - 1) / kRootTileH) * 64;
+}
+
 // Per-call run statistics (counters of src/util/stats.h:10-42 that the
 // engine produces).
 enum StatId { sKd, sLeaf, sTests, sHits, sSplits, sBeams, sRoots, sPresplit, sFive, sClimbs, sRays, sMail, sNum };

@@ -2,7 +2,7 @@
 // processed round by round by persistent wave64 kernels.
 //
 // One round = three launches on the context's stream:
-//   admitKernel   one lane per root (a pixel in 8x8 tiles for soft shadows,
+//   admitKernel   one lane per root (a pixel in tiles of 64 for soft shadows,
 //                 a tile for primary visibility, a primary hit polygon for
 //                 point shadows): per-root setup, direction-sign pre-split,
 //                 records appended to the trace queue;
@@ -145,7 +145,7 @@ struct WState {
 // One soft-shadow job of a batch (a single call is a batch of one). Every
 // job's scene lives in the context's scene arena: its kd root is node node0,
 // its triangles are numbered from triBase. Its pixels are numbered from
-// pixBase in the batch's per-pixel arrays, its roots (pixels in 8 x 8 tiles)
+// pixBase in the batch's per-pixel arrays, its roots (pixels in tiles of 64)
 // from rootBase, its 16 x 16 primary-ray tiles from tileBase.
 struct SoftJob {
   int W, H, tilesX;
@@ -2161,12 +2161,12 @@ __device__ int jobOfRoot(const Params& p, int k) {
   return lo;
 }
 
-// Soft-shadow roots are pixels in 8 x 8 tiles: a wave's 64 lanes trace one
+// Soft-shadow roots are pixels in tiles of 64 (kRootTileW x kRootTileH): a wave's 64 lanes trace one
 // tile. Root k of job J -> its pixel in the batch's per-pixel arrays.
 __device__ __forceinline__ bool softPixel(const SoftJob& J, int k, int& pix) {
   const int kl = k - J.rootBase;
   const int t = kl >> 6, l = kl & 63;
-  const int x = (t % J.tilesX) * 8 + (l & 7), y = (t / J.tilesX) * 8 + (l >> 3);
+  const int x = (t % J.tilesX) * kRootTileW + l % kRootTileW, y = (t / J.tilesX) * kRootTileH + l / kRootTileW;
   pix = J.pixBase + y * J.W + x;
   return x < J.W && y < J.H;
 }
@@ -3395,7 +3395,7 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
     SoftJob& J = jt[size_t(j)];
     J.W = h.cam.W;
     J.H = h.cam.H;
-    J.tilesX = (h.cam.W + 7) / 8;
+    J.tilesX = int(rootTilesX(h.cam.W));
     J.node0 = sr.nodeBase;
     J.triBase = int(sr.triBase);
     J.bounds = sr.bounds;
@@ -3413,7 +3413,7 @@ int engineSoftBatch(Engine* e, const SoftJobH* jobs, int n, bool exact, RunInfo&
     J.rootBase = int(R);
     J.tileBase = int(tiles);
     npix += (long long)h.cam.W * h.cam.H;
-    R += (long long)J.tilesX * ((h.cam.H + 7) / 8) * 64;
+    R += (long long)softRootCount(h.cam.W, h.cam.H);
     tiles += (long long)((h.cam.W + 15) / 16) * ((h.cam.H + 15) / 16);
     if (R > (1LL << 30) || npix > (1LL << 30)) {
       err = "batch too large (more than 2^30 pixels)";
@@ -3715,8 +3715,8 @@ int openTryAdd(OpenPass* op, const SoftJobH& h, int* slot, std::string& err) {
     err = "image size must be positive";
     return BTG_ERR_ARG_CODE;
   }
-  const int tilesX = (h.cam.W + 7) / 8;
-  const int64_t roots = int64_t(tilesX) * ((h.cam.H + 7) / 8) * 64, pix = int64_t(h.cam.W) * h.cam.H;
+  const int tilesX = int(rootTilesX(h.cam.W));
+  const int64_t roots = softRootCount(h.cam.W, h.cam.H), pix = int64_t(h.cam.W) * h.cam.H;
   if (roots > op->rootCap || pix > op->pixCap) {
     err = "request larger than a whole open pass";
     return BTG_ERR_LIMIT_CODE;

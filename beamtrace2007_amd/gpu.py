@@ -65,7 +65,7 @@ class btg_server_config(C.Structure):
 class btg_request(C.Structure):
     _fields_ = [("kind", C.c_int), ("scene", C.c_int), ("cam", btg_camera), ("light", btg_area_light),
                 ("pointLight", C.c_float * 3), ("exact", C.c_int), ("tri", C.c_void_p), ("vis", C.c_void_p),
-                ("E", C.c_void_p), ("Eu", C.c_void_p)]
+                ("E", C.c_void_p), ("Eu", C.c_void_p), ("camD", C.POINTER(btg_camera_d))]
 
 
 class btg_request_status(C.Structure):
@@ -465,13 +465,21 @@ class Server:
         self._h = h
         self._outs = {}
 
-    def submit_soft(self, scene: int, cam14, W: int, H: int, light9) -> int:
+    def submit_soft(self, scene: int, cam14, W: int, H: int, light9, cam_d=None) -> int:
+        """A soft-shadow request; cam_d (14 doubles, host.camera_d) is read by
+        FP64 servers (a context at precision 2) and copied at submission."""
         o = {"tri": np.empty((H, W), np.int32), "vis": np.empty((H, W), np.float32),
              "E": np.empty((H, W), np.float32), "Eu": np.empty((H, W), np.float32)}
         q = btg_request()
         q.kind, q.scene, q.exact = 0, int(scene), 1
         q.cam = make_camera(cam14, W, H)
         q.light = _light(light9)
+        if cam_d is not None:
+            c = np.ascontiguousarray(cam_d, np.float64)
+            if c.shape != (14,):
+                raise ValueError("cam_d must hold 14 doubles (eye, fwd, right, down, halfW, halfH)")
+            cd = btg_camera_d.from_buffer_copy(c.tobytes())
+            q.camD = C.pointer(cd)
         for f in ("tri", "vis", "E", "Eu"):
             setattr(q, f, o[f].ctypes.data)
         return self._submit(q, o)

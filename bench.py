@@ -117,6 +117,8 @@ def parse():
                     help="serve: request arrival rates, comma-separated (0: every request at once, the saturation "
                          "capacity)")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the list")
+    ap.add_argument("--fp64-requests", type=int, default=24,
+                    help="serve: FP64 soft requests served in batches against the plain form (0: skip)")
     ap.add_argument("--no-plain-serve", dest="plain_serve", action="store_false",
                     help="serve: skip the plain form on the same request list")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -811,11 +813,78 @@ def bench_serve(args) -> dict | None:
     r.close()
     sat = next((p for p in points if p["qps"] == "saturation"), None)
     served_err = None if args.no_cpu else served_precision(sc, sp, kept)
+    fp64 = serve_fp64(args, sc, sp) if args.fp64_requests > 0 else None
     return {"requests": len(base), "arrival_seed": 5, "pixels": total_px, "precision": served_err,
             "mix": {"soft": sum(d["kind"] == "soft" for d in base), "point": sum(d["kind"] == "point" for d in base)},
             "points": points, "plain": plain,
             "served_over_plain": round(sat["pixels_per_s"] / plain["pixels_per_s"], 3) if sat and plain else None,
-            "config": WORKLOADS["serve"][2]}
+            "fp64": fp64, "config": WORKLOADS["serve"][2]}
+
+
+def serve_fp64(args, sc, sp) -> dict | None:
+    """FP64 soft shadows (precision 2, each request from its double camera)
+    served in batches on the conference hall, against the plain form (one
+    call per request) on the same seeded list: args.fp64_requests requests,
+    sides uniform 64..128 (seed 7), all submitted at once; the error of the
+    served results against the reference's double build on two of them."""
+    from beamtrace2007_amd import gpu
+    from beamtrace2007_amd.host import camera, camera_d
+
+    g = np.random.Generator(np.random.PCG64(7))
+    sides = [int(x) for x in g.integers(64, 129, size=args.fp64_requests)]
+    total_px = sum(x * x for x in sides)
+    r = gpu.GpuRenderer(sc)
+    r.set_options(precision=2)
+    r.soft_shadows(camera(sp["view"], 32, 32), 32, 32, sp["area_light"], exact=True,
+                   cam_d=camera_d(sp["view"], 32, 32))  # warm-up
+    srv = gpu.Server(r)
+    t0 = time.perf_counter()
+    tickets = [srv.submit_soft(0, camera(sp["view"], x, x), x, x, sp["area_light"], cam_d=camera_d(sp["view"], x, x))
+               for x in sides]
+    res, lat = [], []
+    for t in tickets:
+        q = srv.wait(t)
+        if q["status"] != 0:
+            print(f"bench.py: FP64 request failed: {q.get('error')}", file=sys.stderr)
+            srv.close()
+            r.close()
+            return None
+        res.append(q)
+        lat.append(q["latency_s"])
+    served_s = time.perf_counter() - t0
+    srv.close()
+    t0 = time.perf_counter()
+    for x in sides:
+        r.soft_shadows(camera(sp["view"], x, x), x, x, sp["area_light"], exact=True, cam_d=camera_d(sp["view"], x, x))
+    plain_s = time.perf_counter() - t0
+    r.close()
+    print(f"bench.py: FP64 served {args.fp64_requests} requests in {served_s:.2f} s, plain {plain_s:.2f} s",
+          file=sys.stderr, flush=True)
+    err = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from refpy import Ref, ref_available
+        if ref_available("d"):
+            threads, _ = host_cores()
+            a = sc.arrays()
+            refd = Ref("d").load(a["pos"], a["tris"])
+            imgs, refs = [], []
+            for x, q in list(zip(sides, res))[:2]:
+                d = refd.soft(sp["view"], x, x, sp["area_light"], exact=True, mailbox=False, trail=True, threads=threads)
+                imgs.append({k: q[k].ravel() for k in ("tri", "vis", "E")})
+                refs.append({k: d[k].ravel() for k in ("tri", "vis", "E")})
+            refd.free()
+            cat = lambda L: {k: np.concatenate([z[k] for z in L]) for k in ("tri", "vis", "E")}  # noqa: E731
+            err = err_vs_fp64(cat(imgs), cat(refs))
+            err["sample"] = f"the first two served FP64 requests (sides {sides[0]}, {sides[1]}), whole frames"
+    return {"requests": len(sides), "pixels": total_px, "sides": "uniform 64..128, seed 7",
+            "served": {"seconds": round(served_s, 3), "pixels_per_s": round(total_px / served_s, 1),
+                       "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                                      "p99": round(float(np.percentile(lat, 99)) * 1e3, 1)},
+                       "form": "all submitted at once to a server on a precision-2 context (FP64 batches)"},
+            "plain": {"seconds": round(plain_s, 3), "pixels_per_s": round(total_px / plain_s, 1),
+                      "form": "one FP64 softShadows call per request"},
+            "served_over_plain": round(plain_s / served_s, 3), "precision": err}
 
 
 def err_vs_fp64(img: dict, d: dict) -> dict:

@@ -95,7 +95,8 @@ typedef struct {
                          2 = FP64: the double build's algorithm (bt_render_d: depth-first beam trace, its
                          tolerances, its geometry in double) per pixel, from the primary hit in double; with the
                          job's double camera (btg_soft_job.camD) every visible fraction is bt_render_d's to
-                         ~1e-10. Tens of times slower than 0; not served (btg_server_create refuses it). */
+                         ~1e-10. Tens of times slower than 0; a server on such a context serves its soft
+                         requests in FP64 batches (see btg_server_create). */
   int useGraphs;      /* 1 (default): each group of roundsPerCheck rounds is captured once per pass into a HIP graph
                          and replayed (no counterpart in the reference, which launches every kernel from the host,
                          cuda/wavefront.cu:641-676); 0: plain launches. Results are the same either way.
@@ -304,6 +305,16 @@ int btg_images(btg_ctx* ctx, const btg_camera* cam, int cull, int tile, const fl
  * overlapping the soft-shadow work. Results equal those of the one-call
  * entry points on the same inputs.
  *
+ * On a context set to FP64 soft shadows (btg_options.precision = 2) there is
+ * no wavefront pass to join (each pixel is traced on its own): the server
+ * serves soft requests in FP64 batches instead, every request pending when
+ * the previous batch ends going into the next one (at most 64 requests and
+ * the pixels that hbmBytes allows), so requests that arrive together share
+ * the device. Each result equals btg_soft_batch's on the same job (the
+ * per-pixel sums are fixed point, so the batch a request shares does not
+ * change it); with the request's double camera (btg_request.camD) it is the
+ * double build's to the rounding of the float outputs.
+ *
  * While a server exists, the context must not be used by other calls
  * (its resident scenes must not change). btg_server_submit is thread-safe;
  * each ticket is waited on once. */
@@ -330,6 +341,8 @@ typedef struct {
   float* vis;              /* before the request reports done; they must stay valid */
   float* E;                /* until then */
   float* Eu;
+  const btg_camera_d* camD; /* soft, FP64 servers only: NULL, or the same camera in double (as btg_soft_job.camD;
+                               read before btg_server_submit returns) */
 } btg_request;
 
 typedef struct {

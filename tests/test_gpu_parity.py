@@ -387,18 +387,3 @@ def test_fp64_soft_shadows_match_the_double_build(require_gpu, name, scale, side
         assert e_err.max() <= max(tol, 1e-6)
         assert np.array_equal(img["vis"] < 0, img["tri"] < 0)
     assert np.array_equal(g["Eu"] >= 0, gf["Eu"] >= 0)
-
-
-def test_fp64_mode_is_not_served(require_gpu):
-    """A server runs its passes in the float modes: on a context set to
-    precision 2 btg_server_create refuses with a message that names the
-    entry points that run it."""
-    from beamtrace2007_amd.gpu import Server
-    r = renderer("plant")
-    saved = r.options
-    r.set_options(precision=2)
-    try:
-        with pytest.raises(RuntimeError, match="precision 2"):
-            Server(r)
-    finally:
-        r.set_options(**saved)

@@ -166,6 +166,55 @@ def test_guarded_requests_equal_guarded_alone(require_gpu):
     r.close()
 
 
+def test_fp64_requests_equal_fp64_alone(require_gpu):
+    """A server on a context set to FP64 soft shadows (precision 2) serves its
+    soft requests in FP64 batches: requests submitted together share a batch,
+    a request for a scene that is not resident fails alone, a point request
+    runs beside them, and every soft result equals the same job rendered
+    alone in FP64 mode from the same double camera, bit for bit (the
+    per-pixel sums are fixed point); against the reference's double build it
+    is within 1e-6 on every pixel with the same primary hit."""
+    from beamtrace2007_amd.gpu import GpuRenderer, Server
+    from beamtrace2007_amd.host import camera, camera_d
+    from refpy import Ref, ref_available
+    s, a, k, sp = load_scene("room")
+    plan = [(40, 30, 1.0), (24, 24, 2.0), (32, 40, 0.5)]
+    r = GpuRenderer(s)
+    r.set_options(precision=2)
+    alone = []
+    for W, H, sc in plan:
+        cam, cd = camera(sp["view"], W, H), camera_d(sp["view"], W, H)
+        light = scaled_light(sp["area_light"], sc)
+        alone.append((cam, cd, W, H, light, r.soft_shadows(cam, W, H, light, exact=True, cam_d=cd)))
+    srv = Server(r, hbm_bytes=SERVER_HBM)
+    tickets = [srv.submit_soft(0, cam, W, H, light, cam_d=cd) for cam, cd, W, H, light, _ in alone]
+    bad = srv.submit_soft(7, alone[0][0], 40, 30, alone[0][4])
+    pcam = camera(sp["view"], 32, 32)
+    point = srv.submit_point(pcam, 32, 32, sp["point_light"])
+    print(f"submitted {len(tickets)} FP64 soft requests, a bad one and a point request", flush=True)
+    results = [srv.wait(t, timeout=300) for t in tickets]
+    rb, rp = srv.wait(bad, timeout=300), srv.wait(point, timeout=300)
+    srv.close()
+    r.close()
+    assert rb["status"] != 0 and "scene" in rb["error"]
+    assert rp["status"] == 0
+    hits, shadow = point_alone("room", pcam, 32, 32, sp["point_light"])
+    assert same_polys(rp["hits"], hits) and same_polys(rp["polys"], shadow)
+    refd = Ref("d").load(a["pos"], a["tris"]) if ref_available("d") else None
+    for (cam, cd, W, H, light, g), res in zip(alone, results):
+        assert res is not None and res["status"] == 0, res
+        for f in ("tri", "vis", "E", "Eu"):
+            assert np.array_equal(res[f], g[f]), (W, H, f)
+        if refd is not None:
+            d = refd.soft(sp["view"], W, H, light, exact=True, mailbox=False, trail=True, threads=16)
+            m = (d["vis"] >= 0) & (res["vis"] >= 0) & (res["tri"] == d["tri"])
+            err = np.abs(res["vis"].astype(np.float64) - d["vis"])[m]
+            print(f"served FP64 {W}x{H}: max |vis - vis_fp64| {err.max():.2e} on {m.sum()} px", flush=True)
+            assert err.max() <= 1e-6
+    if refd is not None:
+        refd.free()
+
+
 def test_requests_over_budget_are_traced_again(require_gpu):
     """Roots that go over their live sub-beam budget inside a served pass are
     abandoned there, and their request is traced again alone on the server's
