@@ -867,17 +867,33 @@ void fp64Loop(btg_server* s) {
         batch.push_back(r);
       }
     }
-    int rc = BTG_OK;
+    std::vector<int> rcs(batch.size(), BTG_OK);
+    std::vector<std::string> errs(batch.size());
     try {
       for (SrvReq* r : batch) jobs.push_back(jobOf(r->req));
       btg::RunInfo ri;
-      rc = btg::engineSoftBatch(s->fallback, jobs.data(), int(jobs.size()), true, ri, err);
+      const int rc = btg::engineSoftBatch(s->fallback, jobs.data(), int(jobs.size()), true, ri, err);
+      if (rc != BTG_OK && batch.size() > 1) {
+        // a batch that fails is run again request by request, so that a
+        // request that cannot be traced (e.g. a cell over the pools) fails alone
+        for (size_t i = 0; i < batch.size(); ++i) {
+          btg::RunInfo r1;
+          rcs[i] = btg::engineSoftBatch(s->fallback, &jobs[i], 1, true, r1, errs[i]);
+        }
+      } else {
+        for (size_t i = 0; i < batch.size(); ++i) {
+          rcs[i] = rc;
+          errs[i] = err;
+        }
+      }
     } catch (const std::bad_alloc&) {
-      rc = BTG_ERR_MEMORY;
-      err = "host allocation failed";
+      for (size_t i = 0; i < batch.size(); ++i) {
+        rcs[i] = BTG_ERR_MEMORY;
+        errs[i] = "host allocation failed";
+      }
     }
     std::lock_guard<std::mutex> lk(s->m);
-    for (SrvReq* r : batch) finishLocked(s, r, rc, err);
+    for (size_t i = 0; i < batch.size(); ++i) finishLocked(s, batch[i], rcs[i], errs[i]);
   }
 }
 
