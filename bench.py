@@ -75,11 +75,11 @@ METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request 
 # *_sq_counters.txt), valid only for the library build and schedule they were
 # measured with: keyed by the SHA-256 prefix of lib/libbtg.so and the schedule,
 # so that any kernel change or another schedule reports null, never a stale figure.
-TRAFFIC = {("ee3c9643242f2499", "plant", 512): 479700500}
-TRAFFIC_BATCH = {("ee3c9643242f2499", 1 << 28, 1024, 8, 8, False): 5042927077,
-                 ("ee3c9643242f2499", 1 << 28, 1024, 8, 4, False): 3986424051}
-ISSUE_BATCH = {("ee3c9643242f2499", 1 << 28, 1024, 8, 8, False): {"valu_busy": 0.860, "lane_utilisation": 0.247},
-               ("ee3c9643242f2499", 1 << 28, 1024, 8, 4, False): {"valu_busy": 0.863, "lane_utilisation": 0.250}}
+TRAFFIC = {("4720ae1961c978fb", "plant", 512): 479776899}
+TRAFFIC_BATCH = {("4720ae1961c978fb", 1 << 28, 1024, 8, 8, False): 5043150417,
+                 ("4720ae1961c978fb", 1 << 28, 1024, 8, 4, False): 3984880212}
+ISSUE_BATCH = {("4720ae1961c978fb", 1 << 28, 1024, 8, 8, False): {"valu_busy": 0.861, "lane_utilisation": 0.247},
+               ("4720ae1961c978fb", 1 << 28, 1024, 8, 4, False): {"valu_busy": 0.863, "lane_utilisation": 0.250}}
 
 
 def library_hash() -> str:
