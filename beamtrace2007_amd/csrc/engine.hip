@@ -422,36 +422,48 @@ __device__ __forceinline__ void awStore(float* awCol, const Beam& b) {
   }
 }
 
+// Signed distances from the apex to a box's slabs along each axis, in the
+// rays' direction: nIn[k] to the face they enter by, nOut[k] to the face they
+// leave by (the Nmin / Nmax of beam_core.h:146-149, 171-173). The descent
+// carries them instead of the box: narrowing the box to a child moves one
+// face of the split axis onto the split plane, whose distance is the
+// descent's own Ns, so only that one value changes per kd step.
+struct SlabDist {
+  F3 nIn, nOut;
+};
+__device__ __forceinline__ SlabDist slabDist(const Ctx& c, const Beam& b, const Box& box) {
+  SlabDist d;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int sg = sgnOf(b, k);
+    const float s = float(sg);
+    setComp(d.nIn, k, s * relApex(c, sg > 0 ? comp(box.lo, k) : comp(box.hi, k), k));
+    setComp(d.nOut, k, s * relApex(c, sg > 0 ? comp(box.hi, k) : comp(box.lo, k), k));
+  }
+  return d;
+}
+
 // decide (beam_core.h:139-164): near / far / both children of an inner node.
-__device__ int decide(const Ctx& c, const Beam& b, int a, float Ns, const Box& box, const float* awCol) {
+__device__ int decide(const Beam& b, int a, float Ns, const SlabDist& sd, const float* awCol) {
   const C4 awa = awAt(awCol, a);
 #pragma unroll
   for (int k = 1; k <= 2; ++k) {
     const int cc = a + k >= 3 ? a + k - 3 : a + k;
-    const int sg = sgnOf(b, cc);
-    const float s = float(sg);
-    const float entry = sg > 0 ? comp(box.lo, cc) : comp(box.hi, cc);
-    const float exit = sg > 0 ? comp(box.hi, cc) : comp(box.lo, cc);
-    const float nmin = s * relApex(c, entry, cc), nmax = s * relApex(c, exit, cc);
     const C4 lhs = scaled(awAt(awCol, cc), Ns);
-    if (allBelow(lhs, scaled(awa, nmin), kKdRel)) return kFar;
-    if (allAbove(lhs, scaled(awa, nmax), kKdRel)) return kNear;
+    if (allBelow(lhs, scaled(awa, comp(sd.nIn, cc)), kKdRel)) return kFar;
+    if (allAbove(lhs, scaled(awa, comp(sd.nOut, cc)), kKdRel)) return kNear;
   }
   if (b.hasFar && allAbove(scaled(b.farA, Ns), scaled(awa, b.farC), kKdRel)) return kNear;
   return kBoth;
 }
 
-// overlaps (beam_core.h:166-195): false if the beam certainly misses the box.
-__device__ bool overlaps(const Ctx& c, const Beam& b, const Box& box) {
-  float nmin[3], nmax[3];
+// overlaps (beam_core.h:166-195): false if the beam certainly misses the box
+// whose slab distances are sd.
+__device__ bool overlaps(const Beam& b, const SlabDist& sd) {
+  const float nmin[3] = {sd.nIn.x, sd.nIn.y, sd.nIn.z}, nmax[3] = {sd.nOut.x, sd.nOut.y, sd.nOut.z};
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int sg = sgnOf(b, k);
-    const float s = float(sg);
-    nmin[k] = s * relApex(c, sg > 0 ? comp(box.lo, k) : comp(box.hi, k), k);
-    nmax[k] = s * relApex(c, sg > 0 ? comp(box.hi, k) : comp(box.lo, k), k);
+  for (int k = 0; k < 3; ++k)
     if (nmax[k] < 0) return false;
-  }
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -465,6 +477,9 @@ __device__ bool overlaps(const Ctx& c, const Beam& b, const Box& box) {
       if (allAbove(scaled(b.farA, nmin[k]), scaled(awOf(b, k), b.farC), kKdRel)) return false;
   }
   return true;
+}
+__device__ __forceinline__ bool overlaps(const Ctx& c, const Beam& b, const Box& box) {
+  return overlaps(b, slabDist(c, b, box));
 }
 
 __device__ __forceinline__ void setAw(Beam& b, const Ctx& c) {
@@ -1125,7 +1140,7 @@ constexpr int kPathLevels = BTG_PATH_LEVELS;
 
 struct Stepper {
   int phase;
-  Box box;                         // descend: box of r.node
+  SlabDist sd;                     // descend: slab distances of r.node's box
   KdNodeD kn;                      // descend: node r.node
   uint32_t leafOffset, leafCount;  // the current leaf
   int validFrom;                   // path levels [validFrom, min(r.depth, kPathLevels)) are valid
@@ -1232,16 +1247,18 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
         const bool toRight = sga > 0;
         if (d < kPathLevels) path.set(d, toRight ? right : left);
         int dec = kFar;
-        if (Ns > 0) dec = decide(c, B, a, Ns, ss.box, awCol);
+        if (Ns > 0) dec = decide(B, a, Ns, ss.sd, awCol);
         // kFar: into the child the rays move into; otherwise the apex side
         const bool goRight = (dec == kFar) == toRight;
         if (dec == kBoth) r.trail |= uint64_t(1) << d;
         r.node = goRight ? right : left;
         kn = goRight ? kr : kl;
-        if (goRight)
-          setComp(ss.box.lo, a, s);
+        // the child's box has the split plane as its entry face on axis a
+        // (the far child) or as its exit face (the apex-side child)
+        if (dec == kFar)
+          setComp(ss.sd.nIn, a, Ns);
         else
-          setComp(ss.box.hi, a, s);
+          setComp(ss.sd.nOut, a, Ns);
       }
       r.cursor = 0;
       cnt.v[sLeaf]++;
@@ -1294,14 +1311,15 @@ __device__ int advance(const Params& p, const Ctx& c, Rec& r, Work& w, Stepper& 
       const KdNodeD kc = sv.nodes[cs];
       r.node = node;
       r.depth = int16_t(d);
-      const bool ov = overlaps(c, B, cb);
+      const SlabDist sd = slabDist(c, B, cb);
+      const bool ov = overlaps(B, sd);
       PHASE_MARK(kPhBox);
       if (ov) {
         r.node = cs;
         r.depth = int16_t(d + 1);
         r.cursor = -1;
         ss.phase = kDescend;
-        ss.box = cb;
+        ss.sd = sd;
         ss.kn = kc;
         break;
       }
@@ -1598,7 +1616,7 @@ __global__ void __launch_bounds__(kBlock) TRACE_OCCUPANCY traceKernel(Params p, 
             awStore(awCol, w.b);
             if (r.cursor < 0) {
               ss.phase = kDescend;
-              ss.box = p.sv.nodeBox[r.node];
+              ss.sd = slabDist(c, w.b, p.sv.nodeBox[r.node]);
               ss.kn = p.sv.nodes[r.node];
             } else {
               const KdNodeD kn = p.sv.nodes[r.node];
