@@ -27,6 +27,10 @@ DEVICE_FLAGS = [
     "-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # The trace and split kernels are bound by VALU issue and latency: the
+    # scheduler's instruction-level-parallelism strategy keeps their occupancy
+    # (three and two waves per SIMD) and runs them 1-1.5% faster than the default.
+    "-mllvm", "--amdgpu-sched-strategy=max-ilp",
     "-Wall", "-Wno-unused-parameter",
 ]
 

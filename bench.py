@@ -75,11 +75,11 @@ METRIC = "exact soft-shadow pixels/s, batched vs plain (1 GPU); p50/p99 request 
 # *_sq_counters.txt), valid only for the library build and schedule they were
 # measured with: keyed by the SHA-256 prefix of lib/libbtg.so and the schedule,
 # so that any kernel change or another schedule reports null, never a stale figure.
-TRAFFIC = {("2dedb8a5fbc91a58", "plant", 512): 479637792}
-TRAFFIC_BATCH = {("2dedb8a5fbc91a58", 1 << 28, 1024, 8, 8, False): 5054146543,
-                 ("2dedb8a5fbc91a58", 1 << 28, 1024, 8, 4, False): 3999275862}
-ISSUE_BATCH = {("2dedb8a5fbc91a58", 1 << 28, 1024, 8, 8, False): {"valu_busy": 0.832, "lane_utilisation": 0.259},
-               ("2dedb8a5fbc91a58", 1 << 28, 1024, 8, 4, False): {"valu_busy": 0.834, "lane_utilisation": 0.263}}
+TRAFFIC = {("0536bc727452f3d8", "plant", 512): 479315442}
+TRAFFIC_BATCH = {("0536bc727452f3d8", 1 << 28, 1024, 8, 8, False): 5057140028,
+                 ("0536bc727452f3d8", 1 << 28, 1024, 8, 4, False): 3980057483}
+ISSUE_BATCH = {("0536bc727452f3d8", 1 << 28, 1024, 8, 8, False): {"valu_busy": 0.843, "lane_utilisation": 0.259},
+               ("0536bc727452f3d8", 1 << 28, 1024, 8, 4, False): {"valu_busy": 0.845, "lane_utilisation": 0.263}}
 
 
 def library_hash() -> str:
